@@ -1,0 +1,116 @@
+// bmg_chain.h -- structs of the batched chain engine shared by the device
+// code (bmg_chain.hip) and the host side (bmg_capi.hip).
+#pragma once
+#include <stdint.h>
+
+#define BMG_MAXMS 64     // max_move_size supported on the device
+#define BMG_KMAXG 256    // largest compiled model capacity (columns)
+#define BMG_TREE_LEV 8
+
+namespace bmg {
+
+// One dataset as the chain kernel sees it.
+struct DevData {
+  const uint32_t* g;     // p x W32 packed words
+  const double* y;       // n_pad
+  const double* pre;     // p x offset self moments
+  uint64_t p;
+  uint32_t n, W32, n_pad;
+  int offset;
+  int ot[4];             // offset_type
+  int type;              // the sampler's single effect type (A, H, D, R or AH)
+  int n_types;
+  int types[5];          // DataModel::types (Model API terms may mix them)
+  int top;               // sum-tree top level
+  uint64_t lev_len[BMG_TREE_LEV], lev_off[BMG_TREE_LEV];
+  uint64_t tree_size;    // doubles per tree
+  uint64_t lds_lev_size; // doubles of levels >= 2
+};
+
+// Per-chain constants (options + solved prior, src/sampler.hpp:107-203,
+// src/prior.hpp:45-122).
+struct ChainConst {
+  uint32_t seed, chain_id, ds;
+  uint32_t thin, n_rao, n_st, max_ms, max_nbh;
+  int32_t adapt_p_ms, adaptation, flat, trace_iter;
+  double acpt_goal, q_add_min, q_rem_min, k_move_size;
+  double n_plus_nu, minus_n_plus_nu_2, nus2_plus_yy, mu_alpha;
+  double g_a, g_b, m_g, tp_type, tp_sum;
+  double nu_tau2[4], s2_tau2[4], nus2_tau2[4];
+  int32_t allow_terms[4];
+  int32_t use_ind;
+  double inv_tau2_e0;
+  uint64_t tr_cap, th_cap, th_loci_cap;
+};
+
+// Mutable per-chain scalars (device global memory).
+struct ChainScal {
+  uint32_t rng_c0, rng_c1;
+  uint64_t n_iter, n_accepted;
+  double alpha, alpha2, inv_tau2_alpha2[4];
+  int64_t n_rao_burnin, p_rao_n;
+  uint64_t p_proposal_n;
+  double p_ms;
+  uint64_t n_acpt[3], n_moves[3];
+  uint32_t rb_calls;
+  int32_t movesize;
+  double pves[3];
+  // current model scalars
+  int32_t cols, nloci, beta_len, mu_computed;
+  int32_t Ns[5];
+  double sigma2, syx, ldq, ldl, ll;
+  // RB block decided at the end of an RB iteration (src/sampler.cpp:731-811)
+  int32_t rb_pending, rb_do_scan, rb_accum_rao, rb_update_prop, rb_rebuild;
+  double rao_z1, rao_z2, prop_z1, prop_z2;
+  uint32_t rb_call_idx;
+  int32_t err;
+  uint64_t tr_count, th_count, th_loci_len;
+  double tot_add, tot_rem;   // sum-tree totals
+};
+
+// Per-chain device arrays.
+struct ChainArrays {
+  int32_t* model_inds;       // p
+  double* q_add; double* q_rem;          // p
+  double* tree_add; double* tree_rem;    // tree_size each
+  double* p_proposal; double* p_rao; double* p_r;  // p
+  double* Lcur;              // KMAX (KMAX+1) / 2
+  double* xxs;               // KMAX x KMAX slot matrix
+  double* y_hat;             // n_pad
+  double* vec;               // 5 x KMAX: xy, v, d, beta, mu
+  int32_t* idx;              // 8 x KMAX: loci, x_ind1, x_ind2, slot, snp, xtype, tind, -
+  double* adapt;             // 6 x 256: q_pms, q_pms_nbs, q_pms_nbc, r_ms_sum, r_ms_n, q_p0_ms
+  uint8_t* tr_move; uint8_t* tr_jd; uint8_t* tr_ms; uint32_t* tr_size; double* tr_ll;
+  uint32_t* th_size; uint32_t* th_loci; double* th_ll; double* th_s2; double* th_alpha;
+};
+
+enum { ERR_CAPACITY = 1, ERR_TRACE = 2, ERR_TREE = 4, ERR_NONFINITE = 8 };
+
+}  // namespace bmg
+
+// launch arguments of the chain kernels (bmg_chain.hip)
+struct KernelArgs {
+  const bmg::DevData* data;
+  const bmg::ChainConst* consts;
+  bmg::ChainScal* scal;
+  const bmg::ChainArrays* arrays;
+  int n_chains;
+  uint64_t n_iter;     // iterations to run in this launch
+  int mode;            // 0 iterate, 1 set-up
+};
+
+// one Model method on a one-chain state (the drop-in Model API)
+struct ModelOpArgs {
+  const bmg::DevData* data;
+  const bmg::ChainConst* consts;
+  bmg::ChainScal* scal;
+  const bmg::ChainArrays* arrays;
+  int op;              // 0 add, 1 remove, 2 compute_log_likelihood, 3 swapadj
+  uint32_t ind;
+  int t_ind, model_ind;
+  double d0, d1;
+};
+
+namespace bmg {
+
+}  // namespace bmg

@@ -1,0 +1,1583 @@
+// bmg_chain.hip -- the device-resident PMV sampler: FP64 model state in LDS,
+// Cholesky add / remove / full refactorisation, sum-tree proposals, the
+// moves of src/sampler.hpp:899-1669 and the schedule of src/sampler.cpp:
+// 626-834, one workgroup per chain.
+//
+// All threads of a workgroup run the scalar control flow redundantly on
+// identical register copies (the random stream included), so every decision
+// is uniform; the O(n k) parts use all 256 threads and the O(k^2) triangular
+// and Givens sweeps use wave 0.  Triangular solves and the Cholesky
+// factorisation are right-looking: each element receives its subtractions in
+// the same order as the netlib loops of the CPU oracle, so with contraction
+// off the factors agree bit for bit; only the inner products differ in
+// summation order.
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "bmg_chain.h"
+#include "bmg_device.h"
+
+using namespace bmg;
+
+#define LOG_HALF (-0.69314718055994528622676398299518041312694549560546875)
+#define TLEV_LDS 512
+
+__device__ __forceinline__ int P(int c) { return c * (c + 1) / 2; }
+
+// ------------------------------------------------------------- state -----
+template <int KMAX>
+struct Smem {
+  double L[KMAX * (KMAX + 1) / 2];       // proposed model's factor (packed upper)
+  double xy[KMAX], v[KMAX], d[KMAX];     // proposed model vectors
+  double beta[KMAX], mu[KMAX];           // the chain's beta / mu_beta
+  double work[KMAX + 64];
+  double c_xy[KMAX], c_v[KMAX], c_d[KMAX];
+  int32_t slot[KMAX], snp[KMAX];         // per column: slot in xxs, SNP
+  int32_t c_slot[KMAX], c_snp[KMAX];
+  uint32_t loci[KMAX], c_loci[KMAX];     // per locus
+  int32_t xi1[KMAX], xi2[KMAX], c_xi1[KMAX], c_xi2[KMAX];
+  int8_t xtype[KMAX], c_xtype[KMAX], tind[KMAX], c_tind[KMAX];
+  uint64_t mi[BMG_MAXMS], mi_add[BMG_MAXMS], mi_rem[BMG_MAXMS];
+  int64_t mi_map[BMG_MAXMS];
+  uint8_t mi_isadd[BMG_MAXMS];
+  uint64_t rm_list[2 * BMG_MAXMS];
+  uint64_t marks[BMG_MAXMS];
+  int32_t mark_val[BMG_MAXMS];
+  uint64_t sc_perm[BMG_MAXMS], sc_from[BMG_MAXMS], sc_to[BMG_MAXMS];
+  double tlev[2][TLEV_LDS];              // sum-tree levels >= 2 (add, rem)
+  int32_t win[512];                      // window of the current model_inds
+  uint64_t slots_cur[(KMAX + 63) / 64], slots_prop[(KMAX + 63) / 64];
+  double red[8];
+  long long redi[8];
+  double bcd[4];
+  long long bci[4];
+};
+
+struct Reg {
+  Rng rng;
+  int cols, nloci, Ns[5];
+  double syx, ldq, ldl, ll;
+  int c_cols, c_nloci, c_Ns[5];
+  double c_syx, c_ldq, c_ldl, c_ll;
+  double sigma2, alpha, alpha2, ita2[4];
+  int beta_len, mu_computed;
+  int lmin;
+  int n_rm, n_marks;
+  int movesize;
+  double tot_add, tot_rem;
+  int err;
+};
+
+template <int KMAX>
+struct Ctx {
+  Smem<KMAX>& S;
+  Reg& R;
+  const DevData& D;
+  const ChainConst& K;
+  const ChainArrays& A;
+};
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
+__device__ __forceinline__ int wave_id() { return threadIdx.x >> 6; }
+
+// broadcast a value computed on wave 0 lane 0 to every thread
+template <int KMAX>
+__device__ double bcast_d(Smem<KMAX>& S, double v) {
+  __syncthreads();
+  if (threadIdx.x == 0) S.bcd[0] = v;
+  __syncthreads();
+  double r = S.bcd[0];
+  __syncthreads();
+  return r;
+}
+template <int KMAX>
+__device__ long long bcast_i(Smem<KMAX>& S, long long v) {
+  __syncthreads();
+  if (threadIdx.x == 0) S.bci[0] = v;
+  __syncthreads();
+  long long r = S.bci[0];
+  __syncthreads();
+  return r;
+}
+
+// ------------------------------------------------------------ genotypes ---
+__device__ __forceinline__ int coding_of_type(int type, int second) {
+  // AH splits into A then H (src/model.hpp:254-258)
+  return type == 4 ? (second ? 1 : 0) : type;
+}
+
+// x_j^T y (FP64) for one SNP/coding
+template <int KMAX>
+__device__ double snp_dot_y(Ctx<KMAX>& C, uint64_t snp, int coding) {
+  const uint32_t* w = C.D.g + snp * C.D.W32;
+  double s = 0.0;
+  for (uint32_t k = threadIdx.x; k < C.D.W32; k += BMG_BLOCK) {
+    uint32_t x = w[k];
+    const double* yy = C.D.y + 16 * (size_t)k;
+#pragma unroll
+    for (int f = 0; f < 16; ++f) {
+      int iv = code_ival((x >> (2 * f)) & 3u, coding);
+      if (iv) s += (double)iv * yy[f];
+    }
+  }
+  return block_sum(s, C.S.red);
+}
+
+// S.work[c] = x_c^T x_new for c in [0, cols], column `cols` being the new
+// one itself; column 0 is the constant.  Exact integer popcount dots.
+template <int KMAX>
+__device__ void new_column_dots(Ctx<KMAX>& C, uint64_t snp_new, int coding_new, int cols) {
+  const uint32_t* wn = C.D.g + snp_new * C.D.W32;
+  for (int c = wave_id(); c <= cols; c += BMG_BLOCK / BMG_WAVE) {
+    long long s = 0;
+    if (c == 0) {
+      for (uint32_t k = lane_id(); k < C.D.W32; k += BMG_WAVE) {
+        uint32_t p1, p2;
+        planes(wn[k], coding_new, p1, p2);
+        s += __popc(p1) + __popc(p2);
+      }
+    } else {
+      uint64_t sc = (c == cols) ? snp_new : (uint64_t)C.S.snp[c];
+      int cc = (c == cols) ? coding_new : C.S.xtype[c];
+      const uint32_t* wc = C.D.g + sc * C.D.W32;
+      for (uint32_t k = lane_id(); k < C.D.W32; k += BMG_WAVE)
+        s += dot_planes(wc[k], wn[k], cc, coding_new);
+    }
+    s = wave_sum_i64(s);
+    if (lane_id() == 0) C.S.work[c] = (double)s;
+  }
+  __syncthreads();
+}
+
+// ----------------------------------------------------- triangular (wave 0)
+template <int KMAX>
+__device__ __forceinline__ double reg_get(const double (&b)[(KMAX + 63) / 64], int t) {
+  double r = b[0];
+#pragma unroll
+  for (int q = 1; q < (KMAX + 63) / 64; ++q) if (t == q) r = b[q];
+  return r;
+}
+template <int KMAX>
+__device__ __forceinline__ void reg_set(double (&b)[(KMAX + 63) / 64], int t, double v) {
+#pragma unroll
+  for (int q = 0; q < (KMAX + 63) / 64; ++q) if (t == q) b[q] = v;
+}
+
+// x := U^-T x for x[0..m) in LDS (dtrsv Upper/Trans order).  Wave 0 only.
+template <int KMAX>
+__device__ void trsv_ut_w0(Smem<KMAX>& S, double* x, int m) {
+  constexpr int NT = (KMAX + 63) / 64;
+  const int lane = lane_id();
+  double b[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) { int j = lane + 64 * t; b[t] = j < m ? x[j] : 0.0; }
+  for (int i = 0; i < m; ++i) {
+    int owner = i & 63, ti = i >> 6;
+    double mine = 0.0;
+    if (lane == owner) { mine = reg_get<KMAX>(b, ti) / S.L[P(i) + i]; reg_set<KMAX>(b, ti, mine); }
+    double xi = __shfl(mine, owner, 64);
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      int j = lane + 64 * t;
+      if (j > i && j < m) b[t] = b[t] - S.L[P(j) + i] * xi;
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < NT; ++t) { int j = lane + 64 * t; if (j < m) x[j] = b[t]; }
+}
+
+// x := U^-1 x (dtrsv Upper/NoTrans order, columns from the last).  Wave 0.
+template <int KMAX>
+__device__ void trsv_un_w0(Smem<KMAX>& S, double* x, int m) {
+  constexpr int NT = (KMAX + 63) / 64;
+  const int lane = lane_id();
+  double b[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) { int j = lane + 64 * t; b[t] = j < m ? x[j] : 0.0; }
+  for (int j = m - 1; j >= 0; --j) {
+    int owner = j & 63, tj = j >> 6;
+    double mine = 0.0;
+    if (lane == owner) {
+      double xj = reg_get<KMAX>(b, tj);
+      if (xj != 0.0) { xj = xj / S.L[P(j) + j]; reg_set<KMAX>(b, tj, xj); }
+      mine = xj;
+    }
+    double t = __shfl(mine, owner, 64);
+    if (t != 0.0) {
+#pragma unroll
+      for (int q = 0; q < NT; ++q) {
+        int i = lane + 64 * q;
+        if (i < j) b[q] = b[q] - t * S.L[P(j) + i];
+      }
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < NT; ++t) { int j = lane + 64 * t; if (j < m) x[j] = b[t]; }
+}
+
+// netlib drotg (as the oracle restates it)
+__device__ __forceinline__ void drotg(double& a, double& b, double& c, double& s) {
+  double sa = a, sb = b;
+  double roe = (fabs(sa) > fabs(sb)) ? sa : sb;
+  double scale = fabs(sa) + fabs(sb);
+  double r, z;
+  if (scale == 0.0) { c = 1.0; s = 0.0; r = 0.0; z = 0.0; }
+  else {
+    double as = sa / scale, bs = sb / scale;
+    r = scale * sqrt(as * as + bs * bs);
+    r = (roe >= 0.0 ? 1.0 : -1.0) * r;
+    c = sa / r; s = sb / r;
+    z = 1.0;
+    if (fabs(sa) > fabs(sb)) z = s;
+    if (fabs(sb) >= fabs(sa) && c != 0.0) z = 1.0 / c;
+  }
+  a = r; b = z;
+}
+
+// Cholesky downdate removing column cr of the m x m factor (LINPACK dchex
+// job 2 + drop + sign fix, src/symmmatrix.cpp:166-195).  Logical column jl >=
+// cr lives in physical column jl + 1 until the final compaction; rotation t
+// is generated from logical column cr + t and applied to all later columns
+// at once (per element the same sequence as the sequential loop).
+template <int KMAX>
+__device__ void chol_downdate(Smem<KMAX>& S, int m, int cr) {
+  constexpr int NT = (KMAX + 63) / 64;
+  if (wave_id() == 0) {
+    const int lane = lane_id();
+    for (int j = cr; j <= m - 2; ++j) {
+      int owner = j & 63;
+      double cc = 0.0, ss = 0.0;
+      if (lane == owner) {
+        double a = S.L[P(j + 1) + j], b = S.L[P(j + 1) + j + 1];
+        drotg(a, b, cc, ss);
+        S.L[P(j + 1) + j] = a;
+      }
+      cc = __shfl(cc, owner, 64);
+      ss = __shfl(ss, owner, 64);
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        int jl = lane + 64 * t;
+        if (jl > j && jl <= m - 2) {
+          double* col = S.L + P(jl + 1);
+          double x1 = col[j], x2 = col[j + 1];
+          double tt = cc * x1 + ss * x2;
+          col[j + 1] = cc * x2 - ss * x1;
+          col[j] = tt;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // sign fix over the kept rows i in [cr, m-2]: flags first, then negation
+  for (int i = cr + threadIdx.x; i <= m - 2; i += BMG_BLOCK) S.work[i] = S.L[P(i + 1) + i] < 0 ? -1.0 : 1.0;
+  __syncthreads();
+  for (int jl = cr + threadIdx.x; jl <= m - 2; jl += BMG_BLOCK) {
+    double* col = S.L + P(jl + 1);
+    for (int i = cr; i <= jl; ++i)
+      if (S.work[i] < 0) col[i] = -col[i];
+  }
+  __syncthreads();
+  // compaction: logical column jl rows 0..jl <- physical column jl+1
+  const int e0 = P(cr), e1 = P(m - 1);
+  for (int base = e0; base < e1; base += BMG_BLOCK * 8) {
+    double tmp[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      int e = base + q * BMG_BLOCK + threadIdx.x;
+      if (e < e1) {
+        int jl = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
+        while (P(jl + 1) <= e) ++jl;
+        while (P(jl) > e) --jl;
+        tmp[q] = S.L[P(jl + 1) + (e - P(jl))];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      int e = base + q * BMG_BLOCK + threadIdx.x;
+      if (e < e1) S.L[e] = tmp[q];
+    }
+    __syncthreads();
+  }
+}
+
+// Full factorisation of S.L (m x m, in place), right-looking; returns 0 if
+// not positive definite.
+template <int KMAX>
+__device__ int chol_full(Smem<KMAX>& S, int m) {
+  int ok = 1;
+  for (int k = 0; k < m; ++k) {
+    __syncthreads();
+    double dk = S.L[P(k) + k];
+    if (!(dk > 0.0)) { ok = 0; break; }
+    dk = sqrt(dk);
+    __syncthreads();
+    if (threadIdx.x == 0) S.L[P(k) + k] = dk;
+    for (int i = k + 1 + threadIdx.x; i < m; i += BMG_BLOCK) S.L[P(i) + k] = S.L[P(i) + k] / dk;
+    __syncthreads();
+    int w = m - k - 1;
+    for (int idx = threadIdx.x; idx < w * w; idx += BMG_BLOCK) {
+      int j = k + 1 + idx / w, i = k + 1 + idx % w;
+      if (j <= i) S.L[P(i) + j] = S.L[P(i) + j] - S.L[P(j) + k] * S.L[P(i) + k];
+    }
+  }
+  __syncthreads();
+  return ok;
+}
+
+// ------------------------------------------------------------- sum trees --
+// Levels 0 and 1 in global memory, levels >= 2 in LDS; see the oracle's
+// orc_sumtree for the definition (64-ary, Hillis-Steele group sums).
+template <int KMAX>
+__device__ __forceinline__ double tree_val(Ctx<KMAX>& C, int tr, int h, uint64_t i) {
+  if (h < 2) {
+    const double* base = tr == 0 ? C.A.tree_add : C.A.tree_rem;
+    return base[C.D.lev_off[h] + i];
+  }
+  return C.S.tlev[tr][C.D.lev_off[h] - C.D.lev_off[2] + i];
+}
+
+__device__ __forceinline__ double wave_scan(double v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    double t = __shfl_up(v, d, 64);
+    if (lane >= d) v = v + t;
+  }
+  return v;
+}
+
+// draw an index with probability w_i / total (wave 0 computes, all return)
+template <int KMAX>
+__device__ uint64_t tree_pick(Ctx<KMAX>& C, int tr, double u) {
+  if (wave_id() == 0) {
+    const int lane = lane_id();
+    double total = tr == 0 ? C.R.tot_add : C.R.tot_rem;
+    double r = u * total, carry = 0.0;
+    uint64_t g = 0;
+    long long res = 0;
+    for (int h = C.D.top; h >= 0; --h) {
+      double c = tree_val(C, tr, h, 64 * g + lane);
+      double s = wave_scan(c);
+      unsigned long long m1 = __ballot(c > 0.0 && r < carry + s);
+      unsigned long long nz = __ballot(c > 0.0);
+      int pick = m1 ? (__ffsll((long long)m1) - 1) : (nz ? 63 - __clzll((long long)nz) : -1);
+      if (pick < 0) { res = -1; break; }
+      double sp = __shfl(s, pick > 0 ? pick - 1 : 0, 64);
+      if (pick > 0) carry = carry + sp;
+      g = 64 * g + (uint64_t)pick;
+      res = (long long)g;
+    }
+    if (lane == 0) C.S.bci[1] = res;
+  }
+  __syncthreads();
+  long long r = C.S.bci[1];
+  __syncthreads();
+  if (r < 0) { C.R.err |= ERR_TREE; r = 0; }
+  return (uint64_t)r;
+}
+
+// set leaf i to its weight (unzero) or 0 (zero) and update the path
+template <int KMAX>
+__device__ void tree_set(Ctx<KMAX>& C, int tr, uint64_t i, int zero) {
+  double* g0 = tr == 0 ? C.A.tree_add : C.A.tree_rem;
+  const double* w = tr == 0 ? C.A.q_add : C.A.q_rem;
+  double total;
+  if (wave_id() == 0) {
+    const int lane = lane_id();
+    double nv = zero ? 0.0 : w[i];
+    // issue the level-0 and level-1 group loads together
+    uint64_t g1 = i >> 6;
+    double c0 = g0[C.D.lev_off[0] + 64 * g1 + lane];
+    double c1 = C.D.top >= 1 ? g0[C.D.lev_off[1] + 64 * (g1 >> 6) + lane] : 0.0;
+    if (lane == (int)(i & 63)) c0 = nv;
+    double s = wave_scan(c0);
+    double gs = __shfl(s, 63, 64);
+    if (lane == 0) g0[C.D.lev_off[0] + i] = nv;
+    if (C.D.top == 0) {
+      total = gs;
+    } else {
+      if (lane == 0) g0[C.D.lev_off[1] + g1] = gs;
+      if (lane == (int)(g1 & 63)) c1 = gs;
+      s = wave_scan(c1);
+      gs = __shfl(s, 63, 64);
+      uint64_t gi = g1 >> 6;   // entry index at level 2
+      for (int h = 2; h <= C.D.top; ++h) {
+        double* lv = C.S.tlev[tr] + (C.D.lev_off[h] - C.D.lev_off[2]);
+        if (lane == 0) lv[gi] = gs;
+        uint64_t grp = gi >> 6;
+        double c = lv[64 * grp + lane];
+        if (lane == (int)(gi & 63)) c = gs;
+        s = wave_scan(c);
+        gs = __shfl(s, 63, 64);
+        gi = grp;
+      }
+      total = gs;
+    }
+    if (lane == 0) C.S.bcd[1] = total;
+  }
+  __syncthreads();
+  total = C.S.bcd[1];
+  __syncthreads();
+  if (tr == 0) C.R.tot_add = total; else C.R.tot_rem = total;
+}
+
+// -------------------------------------------------------------- model -----
+// Column bookkeeping helpers operate on LDS arrays with all threads.
+template <int KMAX>
+__device__ int alloc_slot(Ctx<KMAX>& C) {
+  // lowest slot free in both the current and proposed models
+  int s = -1;
+  for (int q = 0; q < (KMAX + 63) / 64 && s < 0; ++q) {
+    uint64_t used = C.S.slots_cur[q] | C.S.slots_prop[q];
+    if (~used) s = 64 * q + __ffsll((long long)~used) - 1;
+  }
+  return s < KMAX ? s : -1;
+}
+
+template <int KMAX>
+__device__ void set_slot_bit(Ctx<KMAX>& C, int s, int on) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (on) C.S.slots_prop[s >> 6] |= (1ull << (s & 63));
+    else C.S.slots_prop[s >> 6] &= ~(1ull << (s & 63));
+  }
+  __syncthreads();
+}
+
+// src/model.hpp:432-506 for one column (type A, H, D or R)
+template <int KMAX>
+__device__ void update_on_add(Ctx<KMAX>& C, uint32_t ind, int model_ind, int coding,
+                              int is_x_ind2, double dval) {
+  Smem<KMAX>& S = C.S;
+  Reg& R = C.R;
+  int col = R.cols;
+  int slot = alloc_slot(C);
+  if (slot < 0 || col >= KMAX) {
+    // current and proposed models together need more slots than compiled:
+    // the proposal is rejected (log-likelihood -inf) and flagged
+    R.err |= ERR_CAPACITY;
+    R.ll = -INFINITY; R.syx = INFINITY;
+    return;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (is_x_ind2) S.xi2[model_ind] = col;
+    else { S.xi1[model_ind] = col; S.xi2[model_ind] = 0; }
+    S.snp[col] = (int32_t)ind;
+    S.xtype[col] = (int8_t)coding;
+    S.slot[col] = slot;
+    S.d[col] = dval;
+  }
+  __syncthreads();
+  set_slot_bit(C, slot, 1);
+  R.cols = col + 1;
+  if (R.lmin > col) R.lmin = col;
+  double xy = snp_dot_y(C, ind, coding);
+  new_column_dots(C, ind, coding, col);
+  // record the new X^T X row/column in the slot matrix
+  for (int c = threadIdx.x; c <= col; c += BMG_BLOCK) {
+    int sc = c == col ? slot : S.slot[c];
+    C.A.xxs[(size_t)slot * KMAX + sc] = S.work[c];
+    C.A.xxs[(size_t)sc * KMAX + slot] = S.work[c];
+  }
+  if (threadIdx.x == 0) S.xy[col] = xy;
+  // bordered update (src/symmmatrix.cpp:144-164)
+  for (int i = threadIdx.x; i < col; i += BMG_BLOCK) S.L[P(col) + i] = S.work[i];
+  __syncthreads();
+  if (wave_id() == 0) trsv_ut_w0(S, S.L + P(col), col);
+  __syncthreads();
+  double ll2 = 0.0, lv = 0.0;
+  for (int i = threadIdx.x; i < col; i += BMG_BLOCK) {
+    double li = S.L[P(col) + i];
+    ll2 += li * li;
+    lv += li * S.v[i];
+  }
+  ll2 = block_sum(ll2, S.red);
+  lv = block_sum(lv, S.red);
+  double lmm2 = S.work[col] + dval - ll2;
+  if (lmm2 <= 0) {
+    R.ll = -INFINITY; R.syx = INFINITY;
+    R.mu_computed = 0;
+    __syncthreads();
+    return;
+  }
+  double lcc = sqrt(lmm2);
+  double vc = (xy - lv) / lcc;
+  __syncthreads();
+  if (threadIdx.x == 0) { S.L[P(col) + col] = lcc; S.v[col] = vc; }
+  __syncthreads();
+  R.syx = R.syx - vc * vc;
+  R.ldq += 0.5 * log(dval);
+  R.ldl += log(lcc);
+  R.ll = R.ldq - R.ldl + C.K.minus_n_plus_nu_2 * log(R.syx);
+}
+
+// src/model.hpp:239-268
+template <int KMAX>
+__device__ void add_term(Ctx<KMAX>& C, uint32_t ind, int t_ind, const double* d2) {
+  Smem<KMAX>& S = C.S;
+  Reg& R = C.R;
+  int type = C.D.types[t_ind];
+  int is_ah = type == 4;
+  if (R.cols + 1 + is_ah > KMAX) {
+    R.err |= ERR_CAPACITY;
+    R.ll = -INFINITY; R.syx = INFINITY;
+    return;
+  }
+  int model_ind = R.nloci;
+  __syncthreads();
+  if (threadIdx.x == 0) { S.loci[model_ind] = ind; S.tind[model_ind] = (int8_t)t_ind; }
+  __syncthreads();
+  R.nloci++;
+  R.Ns[type]++;
+  R.mu_computed = 0;
+  if (is_ah) {
+    update_on_add(C, ind, model_ind, 0, 0, d2[0]);
+    if (isfinite(R.syx)) update_on_add(C, ind, model_ind, 1, 1, d2[1]);
+    else if (R.cols < KMAX && alloc_slot(C) >= 0) {
+      // keep the bookkeeping of the second column (not factorised)
+      int col = R.cols, slot = alloc_slot(C);
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        S.xi2[model_ind] = col; S.snp[col] = (int32_t)ind; S.xtype[col] = 1;
+        S.slot[col] = slot; S.d[col] = d2[1];
+      }
+      __syncthreads();
+      set_slot_bit(C, slot, 1);
+      R.cols = col + 1;
+    }
+  } else {
+    update_on_add(C, ind, model_ind, type, 0, d2[0]);
+  }
+}
+
+// src/model.hpp:508-556
+template <int KMAX>
+__device__ void update_on_remove(Ctx<KMAX>& C, int x_ind) {
+  Smem<KMAX>& S = C.S;
+  Reg& R = C.R;
+  int k = R.cols;
+  double inv_val = S.d[x_ind];
+  int slot_rm = S.slot[x_ind];
+  // shift column arrays left (two-phase)
+  double a0 = 0, a1 = 0; int i0 = 0, i1 = 0; int8_t t0 = 0;
+  int t = threadIdx.x;
+  bool mv = t >= x_ind && t < k - 1;
+  if (mv) { a0 = S.xy[t + 1]; a1 = S.d[t + 1]; i0 = S.slot[t + 1]; i1 = S.snp[t + 1]; t0 = S.xtype[t + 1]; }
+  __syncthreads();
+  if (mv) { S.xy[t] = a0; S.d[t] = a1; S.slot[t] = i0; S.snp[t] = i1; S.xtype[t] = t0; }
+  __syncthreads();
+  set_slot_bit(C, slot_rm, 0);
+  R.cols = k - 1;
+  if (R.lmin > x_ind) R.lmin = x_ind;
+  if (x_ind == R.cols) {
+    double lxx = S.L[P(x_ind) + x_ind];
+    double vx = S.v[x_ind];
+    R.ldl -= log(lxx);
+    R.syx += vx * vx;
+  } else {
+    chol_downdate(S, k, x_ind);
+    for (int i = threadIdx.x; i < R.cols; i += BMG_BLOCK) S.v[i] = S.xy[i];
+    __syncthreads();
+    if (wave_id() == 0) trsv_ut_w0(S, S.v, R.cols);
+    __syncthreads();
+    double ld = 0.0, vv = 0.0;
+    for (int i = threadIdx.x; i < R.cols; i += BMG_BLOCK) {
+      ld += log(S.L[P(i) + i]);
+      vv += S.v[i] * S.v[i];
+    }
+    R.ldl = block_sum(ld, S.red);
+    vv = block_sum(vv, S.red);
+    R.syx = C.K.nus2_plus_yy - vv;
+  }
+  R.ldq -= 0.5 * log(inv_val);
+  R.ll = R.ldq - R.ldl + C.K.minus_n_plus_nu_2 * log(R.syx);
+}
+
+// src/model.hpp:270-312
+template <int KMAX>
+__device__ void remove_term(Ctx<KMAX>& C, int model_ind) {
+  Smem<KMAX>& S = C.S;
+  Reg& R = C.R;
+  int type = C.D.types[S.tind[model_ind]];
+  int xi1 = S.xi1[model_ind], xi2 = S.xi2[model_ind];
+  uint32_t ind = S.loci[model_ind];
+  R.Ns[type]--;
+  int nl = R.nloci;
+  int xmove = type == 4 ? 2 : 1;
+  int t = threadIdx.x;
+  bool mv = t >= model_ind && t < nl - 1;
+  uint32_t l0 = 0; int8_t ti = 0; int a1 = 0, a2 = 0;
+  if (mv) { l0 = S.loci[t + 1]; ti = S.tind[t + 1]; a1 = S.xi1[t + 1] - xmove; a2 = S.xi2[t + 1] - xmove; }
+  __syncthreads();
+  if (mv) { S.loci[t] = l0; S.tind[t] = ti; S.xi1[t] = a1; S.xi2[t] = a2; }
+  if (threadIdx.x == 0) S.rm_list[R.n_rm] = ind;
+  __syncthreads();
+  R.n_rm++;
+  R.nloci = nl - 1;
+  if (type == 4) {
+    if (xi1 > xi2) { update_on_remove(C, xi1); update_on_remove(C, xi2); }
+    else { update_on_remove(C, xi2); update_on_remove(C, xi1); }
+  } else {
+    update_on_remove(C, xi1);
+  }
+  R.mu_computed = 0;
+}
+
+// index of SNP `ind` in the proposed model, or -1 (all threads)
+template <int KMAX>
+__device__ int find_locus(Ctx<KMAX>& C, uint64_t ind) {
+  __syncthreads();
+  if (threadIdx.x == 0) C.S.bci[2] = -1;
+  __syncthreads();
+  for (int t = threadIdx.x; t < C.R.nloci; t += BMG_BLOCK)
+    if (C.S.loci[t] == (uint32_t)ind) C.S.bci[2] = t;
+  __syncthreads();
+  int r = (int)C.S.bci[2];
+  __syncthreads();
+  return r;
+}
+
+// src/model.hpp:199-237 on the proposed model
+template <int KMAX>
+__device__ void compute_log_likelihood(Ctx<KMAX>& C) {
+  Smem<KMAX>& S = C.S;
+  Reg& R = C.R;
+  int k = R.cols;
+  if (k < 1) { R.ll = NAN; return; }
+  for (int e = threadIdx.x; e < P(k); e += BMG_BLOCK) {
+    int j = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
+    while (P(j + 1) <= e) ++j;
+    while (P(j) > e) --j;
+    int i = e - P(j);
+    double v = C.A.xxs[(size_t)S.slot[i] * KMAX + S.slot[j]];
+    if (i == j) v += S.d[i];
+    S.L[e] = v;
+  }
+  double ls = 0.0;
+  for (int i = 1 + threadIdx.x; i < k; i += BMG_BLOCK) ls += log(S.d[i]);
+  ls = block_sum(ls, S.red);
+  R.ldq = 0.5 * ls;
+  R.lmin = 0;
+  if (!chol_full(S, k)) {
+    R.ll = -INFINITY; R.syx = INFINITY; R.mu_computed = 0;
+    return;
+  }
+  for (int i = threadIdx.x; i < k; i += BMG_BLOCK) S.v[i] = S.xy[i];
+  __syncthreads();
+  if (wave_id() == 0) trsv_ut_w0(S, S.v, k);
+  __syncthreads();
+  double ld = 0.0, vv = 0.0;
+  for (int i = threadIdx.x; i < k; i += BMG_BLOCK) {
+    ld += log(S.L[P(i) + i]);
+    vv += S.v[i] * S.v[i];
+  }
+  R.ldl = block_sum(ld, S.red);
+  vv = block_sum(vv, S.red);
+  R.syx = C.K.nus2_plus_yy - vv;
+  R.ll = R.ldq - R.ldl + C.K.minus_n_plus_nu_2 * log(R.syx);
+  R.mu_computed = 0;
+}
+
+// --------------------------------------------------- commit / restore -----
+// Accept (*current = *new) or reject (*new = *current) for the single
+// working copy: only factor columns >= lmin differ from the stored current
+// factor.
+template <int KMAX>
+__device__ void commit(Ctx<KMAX>& C) {
+  Smem<KMAX>& S = C.S;
+  Reg& R = C.R;
+  __syncthreads();
+  for (int e = P(R.lmin < R.cols ? R.lmin : R.cols) + threadIdx.x; e < P(R.cols); e += BMG_BLOCK)
+    C.A.Lcur[e] = S.L[e];
+  for (int i = threadIdx.x; i < R.cols; i += BMG_BLOCK) {
+    S.c_xy[i] = S.xy[i]; S.c_v[i] = S.v[i]; S.c_d[i] = S.d[i];
+    S.c_slot[i] = S.slot[i]; S.c_snp[i] = S.snp[i]; S.c_xtype[i] = S.xtype[i];
+  }
+  for (int i = threadIdx.x; i < R.nloci; i += BMG_BLOCK) {
+    S.c_loci[i] = S.loci[i]; S.c_xi1[i] = S.xi1[i]; S.c_xi2[i] = S.xi2[i]; S.c_tind[i] = S.tind[i];
+  }
+  // current model_inds: removed loci first, then the positions of all loci
+  for (int i = threadIdx.x; i < R.n_rm; i += BMG_BLOCK) C.A.model_inds[S.rm_list[i]] = -1;
+  __syncthreads();
+  for (int i = threadIdx.x; i < R.nloci; i += BMG_BLOCK) C.A.model_inds[S.loci[i]] = i;
+  if (threadIdx.x < (KMAX + 63) / 64) S.slots_cur[threadIdx.x] = S.slots_prop[threadIdx.x];
+  __syncthreads();
+  R.c_cols = R.cols; R.c_nloci = R.nloci;
+  for (int q = 0; q < 5; ++q) R.c_Ns[q] = R.Ns[q];
+  R.c_syx = R.syx; R.c_ldq = R.ldq; R.c_ldl = R.ldl; R.c_ll = R.ll;
+  R.lmin = KMAX; R.n_rm = 0;
+  R.mu_computed = 0;
+}
+
+template <int KMAX>
+__device__ void restore(Ctx<KMAX>& C) {
+  Smem<KMAX>& S = C.S;
+  Reg& R = C.R;
+  __syncthreads();
+  int lm = R.lmin < R.c_cols ? R.lmin : R.c_cols;
+  for (int e = P(lm) + threadIdx.x; e < P(R.c_cols); e += BMG_BLOCK) S.L[e] = C.A.Lcur[e];
+  for (int i = threadIdx.x; i < R.c_cols; i += BMG_BLOCK) {
+    S.xy[i] = S.c_xy[i]; S.v[i] = S.c_v[i]; S.d[i] = S.c_d[i];
+    S.slot[i] = S.c_slot[i]; S.snp[i] = S.c_snp[i]; S.xtype[i] = S.c_xtype[i];
+  }
+  for (int i = threadIdx.x; i < R.c_nloci; i += BMG_BLOCK) {
+    S.loci[i] = S.c_loci[i]; S.xi1[i] = S.c_xi1[i]; S.xi2[i] = S.c_xi2[i]; S.tind[i] = S.c_tind[i];
+  }
+  if (threadIdx.x < (KMAX + 63) / 64) S.slots_prop[threadIdx.x] = S.slots_cur[threadIdx.x];
+  __syncthreads();
+  R.cols = R.c_cols; R.nloci = R.c_nloci;
+  for (int q = 0; q < 5; ++q) R.Ns[q] = R.c_Ns[q];
+  R.syx = R.c_syx; R.ldq = R.c_ldq; R.ldl = R.c_ldl; R.ll = R.c_ll;
+  R.lmin = KMAX; R.n_rm = 0;
+}
+
+// --------------------------------------------------------------- prior ----
+// src/prior.hpp:144-158 (single effect type, Ns[type] == n_loci)
+__device__ __forceinline__ double prior_add(const ChainConst& K, const int* Ns, int n_old, int type) {
+  return (log(K.g_a + n_old) - log(K.g_b + K.m_g - n_old - 1)) +
+         (log(K.tp_type + Ns[type]) - log(K.tp_sum + n_old));
+}
+__device__ __forceinline__ double prior_rem(const ChainConst& K, const int* Ns, int n_old, int type) {
+  return (log(K.g_b + K.m_g - n_old) - log(K.g_a + n_old - 1)) +
+         (log(K.tp_sum + n_old - 1) - log(K.tp_type + Ns[type] - 1));
+}
+
+template <int KMAX>
+__device__ double draw_inv_tau2(Ctx<KMAX>& C, int term) {
+  if (C.K.use_ind)
+    return 1.0 / (C.R.alpha2 * rng_sinvchi2(C.R.rng, C.K.nu_tau2[term], C.K.s2_tau2[term]));
+  return C.R.ita2[term];
+}
+
+template <int KMAX>
+__device__ void prepare_add_new_term(Ctx<KMAX>& C, double* d2) {
+  int type = C.D.type;
+  if (type == 4) { d2[0] = draw_inv_tau2(C, 0); d2[1] = draw_inv_tau2(C, 1); }
+  else d2[0] = draw_inv_tau2(C, type);
+}
+
+// ------------------------------------------------- beta / sigma2 / pve ----
+template <int KMAX>
+__device__ void compute_mu_beta(Ctx<KMAX>& C) {
+  if (!C.R.mu_computed) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < C.R.cols; i += BMG_BLOCK) C.S.mu[i] = C.S.v[i];
+    __syncthreads();
+    if (wave_id() == 0) trsv_un_w0(C.S, C.S.mu, C.R.cols);
+    __syncthreads();
+    C.R.mu_computed = 1;
+  }
+}
+
+// counter-based access to draw `k` of the main stream from its current
+// position (the stream itself is sequential: draw k uses counter base + k)
+__device__ __forceinline__ double normal_at(const Rng& r, uint32_t k) {
+  Rng q = r;
+  uint64_t c = ((uint64_t)q.c1 << 32 | q.c0) + k;
+  q.c0 = (uint32_t)c; q.c1 = (uint32_t)(c >> 32);
+  return rng_normal(q);
+}
+__device__ __forceinline__ void rng_skip(Rng& r, uint32_t k) {
+  uint64_t c = ((uint64_t)r.c1 << 32 | r.c0) + k;
+  r.c0 = (uint32_t)c; r.c1 = (uint32_t)(c >> 32);
+}
+
+// src/model.hpp:326-342 with src/rand.hpp:61-78,144-168
+template <int KMAX>
+__device__ void sample_beta_sigma2(Ctx<KMAX>& C) {
+  Reg& R = C.R;
+  double nu = C.K.n_plus_nu;
+  double s2 = R.syx / C.K.n_plus_nu;
+  double val = -1.0;
+  for (int trial = 0; trial < BMG_MAX_TRIALS && (val <= 0 || !isfinite(val)); ++trial)
+    val = nu * s2 / (2.0 * rng_gamma(R.rng, 0.5 * nu));
+  if (!(val > 0) || !isfinite(val)) R.err |= ERR_NONFINITE;
+  R.sigma2 = val;
+  compute_mu_beta(C);
+  R.beta_len = R.cols;
+  for (int i = threadIdx.x; i < R.cols; i += BMG_BLOCK) C.S.beta[i] = normal_at(R.rng, i);
+  rng_skip(R.rng, R.cols);
+  __syncthreads();
+  if (wave_id() == 0) trsv_un_w0(C.S, C.S.beta, R.beta_len);
+  __syncthreads();
+  double sc = sqrt(R.sigma2);
+  for (int i = threadIdx.x; i < R.beta_len; i += BMG_BLOCK)
+    C.S.beta[i] = sc * C.S.beta[i] + C.S.mu[i];
+  __syncthreads();
+}
+
+// y_hat = X_g beta_g (+ constant) into the chain's y_hat; returns sum x_b^2,
+// sum x_b (x_b = X_g beta_g) and sum of y_hat, y_hat^2 for the PVE
+template <int KMAX>
+__device__ void xbeta(Ctx<KMAX>& C, double* out4, int write_yhat) {
+  Smem<KMAX>& S = C.S;
+  Reg& R = C.R;
+  double b0 = S.beta[0];
+  double sxb = 0, sxb2 = 0, sy = 0, sy2 = 0;
+  for (uint32_t i = threadIdx.x; i < C.D.n; i += BMG_BLOCK) {
+    double acc = 0.0;
+    for (int c = 1; c < R.beta_len; ++c) {
+      double t = S.beta[c];
+      if (t != 0.0) {
+        uint32_t w = C.D.g[(uint64_t)S.snp[c] * C.D.W32 + (i >> 4)];
+        int iv = code_ival((w >> (2 * (i & 15))) & 3u, S.xtype[c]);
+        acc = acc + t * (double)iv;
+      }
+    }
+    sxb += acc; sxb2 += acc * acc;
+    double yh = R.nloci > 0 ? acc + b0 : b0;
+    sy += yh; sy2 += yh * yh;
+    if (write_yhat) C.A.y_hat[i] = yh;
+  }
+  out4[0] = block_sum(sxb, S.red);
+  out4[1] = block_sum(sxb2, S.red);
+  out4[2] = block_sum(sy, S.red);
+  out4[3] = block_sum(sy2, S.red);
+}
+
+// src/model.hpp:345-392 (no covariates: pves[2] = 0)
+template <int KMAX>
+__device__ void compute_pve(Ctx<KMAX>& C, double* pves) {
+  double s[4];
+  xbeta(C, s, 1);
+  double n = (double)C.D.n;
+  if (C.R.nloci > 0) {
+    pves[1] = (s[1] - s[0] * s[0] / n) / (n - 1);
+    pves[2] = 0.0;
+    pves[0] = pves[1];
+    double z = pves[0] + C.R.sigma2;
+    pves[0] /= z; pves[1] /= z; pves[2] /= z;
+  } else {
+    pves[0] = pves[1] = pves[2] = 0.0;
+  }
+}
+
+// src/prior.cpp:30-141 on the working model
+template <int KMAX>
+__device__ void sample_alpha_and_tau2(Ctx<KMAX>& C) {
+  Smem<KMAX>& S = C.S;
+  Reg& R = C.R;
+  const ChainConst& K = C.K;
+  // sample_tau2
+  double a2s2 = R.alpha2 * R.sigma2;
+  if (K.use_ind) {
+    for (int i = 1; i < R.beta_len; ++i) {
+      int t = S.xtype[i];
+      double nu = K.nu_tau2[t] + 1.0;
+      double s2 = (K.nus2_tau2[t] + S.beta[i] * S.beta[i] / a2s2) / nu;
+      double dv = 1.0 / (R.alpha2 * rng_sinvchi2(R.rng, nu, s2));
+      if (threadIdx.x == 0) S.d[i] = dv;
+    }
+  } else {
+    double b2[4] = {0, 0, 0, 0}; double mt[4] = {0, 0, 0, 0};
+    for (int i = 1; i < R.beta_len; ++i) {
+      b2[S.xtype[i]] += S.beta[i] * S.beta[i];
+      mt[S.xtype[i]] += 1.0;
+    }
+    for (int t = 0; t < 4; ++t) {
+      if (!K.allow_terms[t]) continue;
+      double nu = K.nu_tau2[t] + mt[t];
+      double s2 = (K.nus2_tau2[t] + b2[t] / a2s2) / nu;
+      R.ita2[t] = 1.0 / (R.alpha2 * rng_sinvchi2(R.rng, nu, s2));
+    }
+    __syncthreads();
+    for (int i = 1 + threadIdx.x; i < R.beta_len; i += BMG_BLOCK) S.d[i] = R.ita2[S.xtype[i]];
+  }
+  R.mu_computed = 0;
+  __syncthreads();
+  // sample_alpha
+  int nterms = R.beta_len - 1;
+  double a = R.alpha;
+  double xbxb = 0, ebxb = 0;
+  if (nterms > 0) {
+    // x_b = X_g beta_g, e_b = beta_0 - y:  e_b . x_b = beta_0 sum x_b - y . x_b
+    double s[4];
+    xbeta(C, s, 0);
+    double yxb = 0.0;
+    Smem<KMAX>& SS = C.S;
+    for (uint32_t i = threadIdx.x; i < C.D.n; i += BMG_BLOCK) {
+      double acc = 0.0;
+      for (int c = 1; c < R.beta_len; ++c) {
+        double t = SS.beta[c];
+        if (t != 0.0) {
+          uint32_t w = C.D.g[(uint64_t)SS.snp[c] * C.D.W32 + (i >> 4)];
+          int iv = code_ival((w >> (2 * (i & 15))) & 3u, SS.xtype[c]);
+          acc = acc + t * (double)iv;
+        }
+      }
+      yxb += (SS.beta[0] - C.D.y[i]) * acc;
+    }
+    ebxb = block_sum(yxb, SS.red);
+    xbxb = s[1];
+  }
+  int trials = 0;
+  do {
+    if (++trials > BMG_MAX_TRIALS) { R.err |= ERR_NONFINITE; a = R.alpha; break; }
+    if (nterms > 0) {
+      if ((a <= 0.0) || log(rng_u01(R.rng)) <= -2.0 * K.mu_alpha * a) a = -a;
+      double as2 = a * R.sigma2;
+      double var = 1.0 / (1.0 + xbxb / (a * as2));
+      double mu = var * (K.mu_alpha - ebxb / as2);
+      a = sqrt(var) * rng_normal(R.rng) + mu;
+    } else {
+      a = rng_normal(R.rng) + K.mu_alpha;
+    }
+  } while (a == 0 || a * a == 0);
+  // set_alpha
+  double old = R.alpha2;
+  R.alpha = a; R.alpha2 = a * a;
+  __syncthreads();
+  if (K.use_ind) {
+    for (int i = 1 + threadIdx.x; i < R.beta_len; i += BMG_BLOCK) S.d[i] = S.d[i] * old / R.alpha2;
+  } else {
+    for (int t = 0; t < 4; ++t) if (K.allow_terms[t]) R.ita2[t] = R.ita2[t] * old / R.alpha2;
+    for (int i = 1 + threadIdx.x; i < R.beta_len; i += BMG_BLOCK) S.d[i] = R.ita2[S.xtype[i]];
+  }
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------- moves ---
+__device__ __forceinline__ int sample_naive(const double* cumsum, int m, double u) {
+  double r = u * cumsum[m - 1];
+  for (int i = 0; i < m; ++i) if (r < cumsum[i]) return i;
+  return m - 1;
+}
+
+template <int KMAX>
+__device__ void undo_dd(Ctx<KMAX>& C, int ms) {
+  for (int i = 0; i < ms; ++i) {
+    uint64_t ind = C.S.mi[i];
+    if (C.S.mi_isadd[i]) { tree_set(C, 0, ind, 0); tree_set(C, 1, ind, 1); }
+    else { tree_set(C, 0, ind, 1); tree_set(C, 1, ind, 0); }
+  }
+}
+
+// src/sampler.hpp:899-1123 without delayed rejection
+template <int KMAX>
+__device__ int move_addrem(Ctx<KMAX>& C, double* adapt) {
+  Smem<KMAX>& S = C.S;
+  Reg& R = C.R;
+  const ChainConst& K = C.K;
+  const int type = C.D.type;
+  int max_ms = (int)K.max_ms;
+  int ms = sample_naive(adapt, max_ms, rng_u01(R.rng)) + 1;
+  R.movesize = ms;
+  double lqf = 0, lqb = 0, lmpc = 0;
+  // prepare_addrem (src/sampler.hpp:1126-1165)
+  uint64_t max_adds = C.D.p - R.c_nloci, max_rems = R.c_nloci;
+  for (int i = 0; i < ms; ++i) {
+    int add;
+    if (max_rems == 0) add = 1;
+    else if (max_adds == 0) add = 0;
+    else { lqf += LOG_HALF; add = rng_u01(R.rng) < 0.5; }
+    double u = rng_u01(R.rng);
+    uint64_t ind;
+    if (add) {
+      --max_adds;
+      ind = tree_pick(C, 0, u);
+      lqf += log(C.A.q_add[ind]) - log(R.tot_add);
+      tree_set(C, 0, ind, 1);
+    } else {
+      --max_rems;
+      ind = tree_pick(C, 1, u);
+      lqf += log(C.A.q_rem[ind]) - log(R.tot_rem);
+      tree_set(C, 1, ind, 1);
+    }
+    if (threadIdx.x == 0) {
+      S.mi_isadd[i] = (uint8_t)add; S.mi[i] = ind; S.mi_map[i] = add ? (int64_t)ind : -1;
+    }
+    __syncthreads();
+  }
+  // do_addrem (src/sampler.hpp:1204-1268): removals first
+  int last = ms - 1;
+  for (int i = 0; i < ms; ++i) {
+    if (S.mi_isadd[i]) continue;
+    uint64_t ind = S.mi[i];
+    tree_set(C, 0, ind, 0);
+    while (S.mi_map[last] >= 0) --last;
+    __syncthreads();
+    if (threadIdx.x == 0) S.mi_map[last] = (int64_t)ind;
+    __syncthreads();
+    int mind = find_locus(C, ind);
+    lmpc += prior_rem(K, R.Ns, R.nloci, type);
+    remove_term(C, mind);
+  }
+  double d2[2];
+  for (int i = 0; i < ms; ++i) {
+    if (!S.mi_isadd[i]) continue;
+    uint64_t ind = S.mi[i];
+    tree_set(C, 1, ind, 0);
+    lmpc += prior_add(K, R.Ns, R.nloci, type);
+    prepare_add_new_term(C, d2);
+    add_term(C, (uint32_t)ind, 0, d2);
+  }
+  // backward_prepare_addrem (src/sampler.hpp:1168-1202)
+  {
+    uint64_t madd = C.D.p - R.nloci, mrem = R.nloci;
+    double w_add = R.tot_add, w_rem = R.tot_rem;
+    int lr = ms - 1;
+    for (int i = 0; i < ms; ++i) {
+      if (madd > 0 && mrem > 0) lqb += LOG_HALF;
+      uint64_t ind;
+      if (S.mi_isadd[i]) {
+        --mrem;
+        while (!S.mi_isadd[lr]) --lr;
+        ind = (uint64_t)S.mi_map[lr];
+        --lr;
+        lqb += log(C.A.q_rem[ind]) - log(w_rem);
+        w_rem -= C.A.q_rem[ind];
+      } else {
+        --madd;
+        ind = (uint64_t)S.mi_map[i];
+        lqb += log(C.A.q_add[ind]) - log(w_add);
+        w_add -= C.A.q_add[ind];
+      }
+    }
+  }
+  double log_r = lqb - lqf;
+  log_r += lmpc + R.ll - R.c_ll;
+  double* r_ms_sum = adapt + 3 * 256;
+  double* r_ms_n = adapt + 4 * 256;
+  if (threadIdx.x == 0) {
+    r_ms_sum[ms - 1] += (log_r >= 0 ? 1.0 : exp(log_r));
+    r_ms_n[ms - 1] += 1.0;
+  }
+  if (log_r >= 0 || log(rng_u01(R.rng)) <= log_r) {
+    commit(C);
+    return ms;
+  }
+  restore(C);
+  undo_dd(C, ms);
+  return 0;
+}
+
+// is SNP j in the current model (with the switch move's temporary marks)?
+template <int KMAX>
+__device__ int cur_in_model(Ctx<KMAX>& C, int64_t j, int64_t wbase) {
+  if (j < 0 || j >= (int64_t)C.D.p) return 0;   // outside the genome: stops a walk
+  for (int q = 0; q < C.R.n_marks; ++q) if ((int64_t)C.S.marks[q] == j) return 1;
+  if (j >= wbase && j < wbase + 512) return C.S.win[j - wbase] >= 0;
+  return C.A.model_inds[j] >= 0;
+}
+
+template <int KMAX>
+__device__ void load_window(Ctx<KMAX>& C, int64_t center, int64_t* wbase) {
+  int64_t b = center - 256;
+  if (b < 0) b = 0;
+  __syncthreads();
+  for (int t = threadIdx.x; t < 512; t += BMG_BLOCK) {
+    int64_t j = b + t;
+    C.S.win[t] = (j < (int64_t)C.D.p) ? C.A.model_inds[j] : -1;
+  }
+  __syncthreads();
+  *wbase = b;
+}
+
+// src/sampler.hpp:1271-1418
+template <int KMAX>
+__device__ int move_switch(Ctx<KMAX>& C, double* adapt) {
+  Smem<KMAX>& S = C.S;
+  Reg& R = C.R;
+  const ChainConst& K = C.K;
+  uint64_t full = C.D.p - R.c_nloci;
+  uint64_t mm2 = (K.max_ms + 1) / 2;
+  uint64_t mx = mm2 < (full + 1) / 2 ? mm2 : (full + 1) / 2;
+  mx = mx < (uint64_t)R.c_nloci ? mx : (uint64_t)R.c_nloci;
+  int ms = sample_naive(adapt + 256, (int)mx, rng_u01(R.rng)) + 1;
+  R.movesize = ms;
+  uint64_t nbh = (full - ms) / 2;
+  if (nbh < 1) nbh = 1;
+  if (nbh > K.max_nbh) nbh = K.max_nbh;
+  int64_t m_g = (int64_t)C.D.p;
+  R.n_marks = 0;
+  for (int i = 0; i < ms; ++i) {
+    int mir = (int)floor(rng_u01(R.rng) * R.nloci);
+    int64_t ir = S.loci[mir];
+    int64_t wb;
+    load_window(C, ir, &wb);
+    int64_t ia;
+    int j = 0;
+    int move = (int)floor(rng_u01(R.rng) * nbh) + 1;
+    if (rng_u01(R.rng) < 0.5) {
+      ia = ir; j = 0;
+      while (j < move) {
+        ++ia;
+        if (ia < m_g) { if (!cur_in_model(C, ia, wb)) ++j; }
+        else break;
+      }
+      if (ia == m_g) {
+        ia = ir; move = move + move - j; j = 0;
+        while (j < move) {
+          --ia;
+          while (cur_in_model(C, ia, wb)) --ia;
+          ++j;
+        }
+      }
+    } else {
+      ia = ir; j = 0;
+      if (ia > 0) {
+        while (j < move) {
+          --ia;
+          if (ia > 0) { if (!cur_in_model(C, ia, wb)) ++j; }
+          else break;
+        }
+      }
+      if (j < move) {
+        if (!cur_in_model(C, ia, wb)) ++j;
+        if (j < move) {
+          ia = ir; move = move + move - j; j = 0;
+          while (j < move) {
+            ++ia;
+            while (cur_in_model(C, ia, wb)) ++ia;
+            ++j;
+          }
+        }
+      }
+    }
+    if (ia < 0 || ia >= m_g) { R.err |= ERR_NONFINITE; ia = ia < 0 ? 0 : m_g - 1; }
+    int t_ind = S.tind[mir];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      S.mi_rem[i] = (uint64_t)ir; S.mi_add[i] = (uint64_t)ia;
+      S.marks[R.n_marks] = (uint64_t)ia; S.mark_val[R.n_marks] = t_ind;
+    }
+    __syncthreads();
+    R.n_marks++;
+    remove_term(C, mir);
+  }
+  double d2[2];
+  for (int i = 0; i < ms; ++i) {
+    uint64_t ia = S.mi_add[i];
+    int t_ind = S.mark_val[i];
+    prepare_add_new_term(C, d2);
+    add_term(C, (uint32_t)ia, t_ind, d2);
+  }
+  R.n_marks = 0;
+  double log_r = R.ll - R.c_ll;
+  if (log_r >= 0 || log(rng_u01(R.rng)) <= log_r) {
+    commit(C);
+    for (int i = 0; i < ms; ++i) {
+      tree_set(C, 0, S.mi_rem[i], 0); tree_set(C, 1, S.mi_rem[i], 1);
+      tree_set(C, 0, S.mi_add[i], 1); tree_set(C, 1, S.mi_add[i], 0);
+    }
+    return 2 * ms;
+  }
+  restore(C);
+  return 0;
+}
+
+// src/sampler.hpp:1421-1669 without delayed rejection
+template <int KMAX>
+__device__ int move_statechange(Ctx<KMAX>& C, double* adapt) {
+  Smem<KMAX>& S = C.S;
+  Reg& R = C.R;
+  const ChainConst& K = C.K;
+  const int type = C.D.type;
+  uint64_t m_g = C.D.p;
+  uint64_t a = (uint64_t)K.max_nbh * 2, b = m_g - 1 < K.max_ms ? m_g - 1 : K.max_ms;
+  uint64_t mx = a < b ? a : b;
+  int ms = sample_naive(adapt + 2 * 256, (int)mx, rng_u01(R.rng)) + 1;
+  R.movesize = ms;
+  uint64_t nbh = K.max_nbh < m_g / 2 ? K.max_nbh : m_g / 2;
+  int mic = (int)floor(rng_u01(R.rng) * R.nloci);
+  uint64_t ic = S.loci[mic];
+  // partial Fisher-Yates over the 2 nbh offsets (only the first ms entries
+  // are used): positions that differ from the identity are kept as a sparse
+  // map in LDS by thread 0; every thread consumes the same draws
+  __syncthreads();
+  int nsw = 0;
+  for (int i = 0; i < ms; ++i) {
+    uint64_t ind = (uint64_t)floor(rng_u01(R.rng) * (double)(nbh * 2 - i)) + i;
+    if (threadIdx.x == 0) {
+      uint64_t vi = i, vind = ind;
+      for (int q = 0; q < nsw; ++q) {
+        if (S.sc_from[q] == (uint64_t)i) vi = S.sc_to[q];
+        if (S.sc_from[q] == ind) vind = S.sc_to[q];
+      }
+      S.sc_perm[i] = vind;
+      int found = 0;
+      for (int q = 0; q < nsw; ++q) if (S.sc_from[q] == ind) { S.sc_to[q] = vi; found = 1; }
+      if (!found) { S.sc_from[nsw] = ind; S.sc_to[nsw] = vi; }
+    }
+    // every thread tracks the map size identically
+    {
+      int found = 0;
+      __syncthreads();
+      for (int q = 0; q < nsw; ++q) if (S.sc_from[q] == ind) found = 1;
+      if (!found) ++nsw;
+      __syncthreads();
+    }
+  }
+  __syncthreads();
+  uint64_t wrap = ic + nbh;
+  if (ic < nbh) wrap += (nbh - ic);
+  else if (wrap > m_g - 1) wrap = m_g - 1;
+  int ms_rem = 0, ms_add = 0;
+  for (int i = 0; i < ms; ++i) {
+    uint64_t ind = S.sc_perm[i] + ic + 1;
+    if (ind > wrap) ind = ic - (ind - wrap);
+    if (ind >= m_g) { R.err |= ERR_NONFINITE; ind = m_g - 1; }
+    int fl = find_locus(C, ind);
+    if (threadIdx.x == 0) {
+      if (fl < 0) S.mi_add[ms_add] = ind; else S.mi_rem[ms_rem] = ind;
+    }
+    if (fl < 0) ++ms_add; else ++ms_rem;
+  }
+  __syncthreads();
+  double lmpc = 0.0;
+  for (int i = 0; i < ms_rem; ++i) {
+    int mir = find_locus(C, S.mi_rem[i]);
+    lmpc += prior_rem(K, R.Ns, R.nloci, type);
+    remove_term(C, mir);
+  }
+  double d2[2];
+  int t_ind = 0;
+  for (int i = 0; i < ms_add; ++i) {
+    uint64_t ia = S.mi_add[i];
+    lmpc += prior_add(K, R.Ns, R.nloci, type);
+    prepare_add_new_term(C, d2);
+    add_term(C, (uint32_t)ia, t_ind, d2);
+  }
+  double log_r = lmpc + R.ll - R.c_ll;
+  log_r += log((double)R.c_nloci) - log((double)R.nloci);
+  if (log_r >= 0 || log(rng_u01(R.rng)) <= log_r) {
+    commit(C);
+    for (int i = 0; i < ms_add; ++i) { tree_set(C, 0, S.mi_add[i], 1); tree_set(C, 1, S.mi_add[i], 0); }
+    for (int i = 0; i < ms_rem; ++i) { tree_set(C, 0, S.mi_rem[i], 0); tree_set(C, 1, S.mi_rem[i], 1); }
+    return ms;
+  }
+  restore(C);
+  return 0;
+}
+
+// src/sampler.hpp:1728-1769
+__device__ void adapt_jd_mb(double* p_ms, double* adapt, int max_ms) {
+  double* q_p0 = adapt + 5 * 256;
+  const double* r_ms = adapt + 3 * 256;
+  const double* r_n = adapt + 4 * 256;
+  double q = 1 - *p_ms;
+  double zp = 1 - pow(q, max_ms);
+  double q_p = *p_ms / zp;
+  for (int i = 0; i < max_ms; ++i) { q_p0[i] += q_p; q_p *= q; }
+  double p_best = 0.01, p = 0.01, h_best = -INFINITY, h;
+  double step = 0.98 / 49.0, ratio;
+  for (int j = 0; j < 50; ++j) {
+    h = 0.0; zp = 0.0; q = 1.0 - p; q_p = p;
+    for (int i = 0; i < max_ms; ++i) {
+      ratio = q_p / q_p0[i];
+      h += (i + 1) * r_ms[i] * ratio;
+      zp += r_n[i] * ratio;
+      q_p *= q;
+    }
+    h /= zp;
+    if (h > h_best) { h_best = h; p_best = p; }
+    p += step;
+  }
+  *p_ms = p_best;
+}
+
+__device__ void geometric_cdf(int maxsize, double p, double* values) {
+  double q = 1 - p, qp = q;
+  for (int i = 0; i < maxsize; ++i) { values[i] = 1 - q; q *= qp; }
+}
+
+// --------------------------------------------------------------- kernel ---
+
+template <int KMAX>
+__device__ void load_chain(Ctx<KMAX>& C, const ChainScal& CS) {
+  Smem<KMAX>& S = C.S;
+  Reg& R = C.R;
+  R.rng = rng_make(C.K.seed, C.K.chain_id, 0, 0);
+  R.rng.c0 = CS.rng_c0; R.rng.c1 = CS.rng_c1;
+  R.cols = R.c_cols = CS.cols;
+  R.nloci = R.c_nloci = CS.nloci;
+  for (int q = 0; q < 5; ++q) R.Ns[q] = R.c_Ns[q] = CS.Ns[q];
+  R.syx = R.c_syx = CS.syx; R.ldq = R.c_ldq = CS.ldq; R.ldl = R.c_ldl = CS.ldl;
+  R.ll = R.c_ll = CS.ll;
+  R.sigma2 = CS.sigma2; R.alpha = CS.alpha; R.alpha2 = CS.alpha2;
+  for (int q = 0; q < 4; ++q) R.ita2[q] = CS.inv_tau2_alpha2[q];
+  R.beta_len = CS.beta_len; R.mu_computed = CS.mu_computed;
+  R.lmin = KMAX; R.n_rm = 0; R.n_marks = 0; R.movesize = CS.movesize;
+  R.tot_add = CS.tot_add; R.tot_rem = CS.tot_rem;
+  R.err = CS.err;
+  const double* vec = C.A.vec;
+  const int32_t* idx = C.A.idx;
+  for (int e = threadIdx.x; e < P(R.cols); e += BMG_BLOCK) S.L[e] = C.A.Lcur[e];
+  for (int i = threadIdx.x; i < KMAX; i += BMG_BLOCK) {
+    S.xy[i] = S.c_xy[i] = vec[i];
+    S.v[i] = S.c_v[i] = vec[KMAX + i];
+    S.d[i] = S.c_d[i] = vec[2 * KMAX + i];
+    S.beta[i] = vec[3 * KMAX + i];
+    S.mu[i] = vec[4 * KMAX + i];
+    S.loci[i] = S.c_loci[i] = (uint32_t)idx[i];
+    S.xi1[i] = S.c_xi1[i] = idx[KMAX + i];
+    S.xi2[i] = S.c_xi2[i] = idx[2 * KMAX + i];
+    S.slot[i] = S.c_slot[i] = idx[3 * KMAX + i];
+    S.snp[i] = S.c_snp[i] = idx[4 * KMAX + i];
+    S.xtype[i] = S.c_xtype[i] = (int8_t)idx[5 * KMAX + i];
+    S.tind[i] = S.c_tind[i] = (int8_t)idx[6 * KMAX + i];
+  }
+  if (threadIdx.x < (KMAX + 63) / 64) {
+    uint64_t m = 0;
+    for (int i = 0; i < R.cols; ++i)
+      if ((idx[3 * KMAX + i] >> 6) == (int)threadIdx.x) m |= 1ull << (idx[3 * KMAX + i] & 63);
+    S.slots_cur[threadIdx.x] = S.slots_prop[threadIdx.x] = m;
+  }
+  // sum-tree levels >= 2
+  if (C.D.top >= 2) {
+    uint64_t o2 = C.D.lev_off[2];
+    for (uint64_t e = threadIdx.x; e < C.D.lds_lev_size; e += BMG_BLOCK) {
+      S.tlev[0][e] = C.A.tree_add[o2 + e];
+      S.tlev[1][e] = C.A.tree_rem[o2 + e];
+    }
+  }
+  __syncthreads();
+}
+
+template <int KMAX>
+__device__ void store_chain(Ctx<KMAX>& C, ChainScal& CS) {
+  Smem<KMAX>& S = C.S;
+  Reg& R = C.R;
+  __syncthreads();
+  double* vec = C.A.vec;
+  int32_t* idx = C.A.idx;
+  for (int i = threadIdx.x; i < KMAX; i += BMG_BLOCK) {
+    vec[i] = S.c_xy[i]; vec[KMAX + i] = S.c_v[i]; vec[2 * KMAX + i] = S.c_d[i];
+    vec[3 * KMAX + i] = S.beta[i]; vec[4 * KMAX + i] = S.mu[i];
+    idx[i] = (int32_t)S.c_loci[i]; idx[KMAX + i] = S.c_xi1[i]; idx[2 * KMAX + i] = S.c_xi2[i];
+    idx[3 * KMAX + i] = S.c_slot[i]; idx[4 * KMAX + i] = S.c_snp[i];
+    idx[5 * KMAX + i] = S.c_xtype[i]; idx[6 * KMAX + i] = S.c_tind[i];
+  }
+  if (C.D.top >= 2) {
+    uint64_t o2 = C.D.lev_off[2];
+    for (uint64_t e = threadIdx.x; e < C.D.lds_lev_size; e += BMG_BLOCK) {
+      C.A.tree_add[o2 + e] = S.tlev[0][e];
+      C.A.tree_rem[o2 + e] = S.tlev[1][e];
+    }
+  }
+  if (threadIdx.x == 0) {
+    CS.rng_c0 = R.rng.c0; CS.rng_c1 = R.rng.c1;
+    CS.cols = R.c_cols; CS.nloci = R.c_nloci;
+    for (int q = 0; q < 5; ++q) CS.Ns[q] = R.c_Ns[q];
+    CS.syx = R.c_syx; CS.ldq = R.c_ldq; CS.ldl = R.c_ldl; CS.ll = R.c_ll;
+    CS.sigma2 = R.sigma2; CS.alpha = R.alpha; CS.alpha2 = R.alpha2;
+    for (int q = 0; q < 4; ++q) CS.inv_tau2_alpha2[q] = R.ita2[q];
+    CS.beta_len = R.beta_len; CS.mu_computed = R.mu_computed;
+    CS.movesize = R.movesize;
+    CS.tot_add = R.tot_add; CS.tot_rem = R.tot_rem;
+    CS.err = R.err;
+  }
+  __syncthreads();
+}
+
+// the tau2/alpha refresh at the top of an iteration (src/sampler.cpp:628-635)
+// acts on the current model; here the working copy equals it, and the
+// refreshed model is committed in full
+template <int KMAX>
+__device__ void refresh_tau2(Ctx<KMAX>& C) {
+  sample_alpha_and_tau2(C);
+  compute_log_likelihood(C);
+  C.R.lmin = 0;
+  commit(C);
+}
+
+template <int KMAX>
+__device__ void run_iterations(Ctx<KMAX>& C, ChainScal& CS, uint64_t n_iter) {
+  Smem<KMAX>& S = C.S;
+  Reg& R = C.R;
+  const ChainConst& K = C.K;
+  double* adapt = C.A.adapt;
+  uint64_t it0 = CS.n_iter;
+  for (uint64_t iter = it0; iter < it0 + n_iter; ++iter) {
+    if ((iter + 1) % K.n_st == 0) refresh_tau2(C);
+    // PMV move mix with one effect type (src/sampler.cpp:468-476)
+    const double pm[3] = {0.7, 0.7 + 0.15, 0.7 + 0.15 + 0.15};
+    double u = rng_u01(R.rng);
+    double r = u * pm[2];
+    int move = r < pm[0] ? 0 : (r < pm[1] ? 1 : 2);
+    int jd = 0;
+    if (move == 0) jd = move_addrem(C, adapt);
+    else if (move == 1) {
+      if (R.c_nloci > 0 && (uint64_t)R.c_nloci < C.D.p - 1) jd = move_switch(C, adapt);
+    } else {
+      if (R.c_nloci > 0 && C.D.p > 2) jd = move_statechange(C, adapt);
+    }
+    if (threadIdx.x == 0) {
+      CS.n_moves[move]++;
+      CS.n_acpt[move] += jd > 0;
+      CS.n_accepted += jd > 0;
+    }
+    if ((iter + 1) % K.thin == 0 || (iter + 2) % K.n_st == 0) sample_beta_sigma2(C);
+    if (K.trace_iter && threadIdx.x == 0) {
+      uint64_t t = CS.tr_count;
+      if (t < K.tr_cap) {
+        C.A.tr_move[t] = (uint8_t)move; C.A.tr_jd[t] = (uint8_t)jd;
+        C.A.tr_ms[t] = (uint8_t)R.movesize; C.A.tr_size[t] = (uint32_t)R.c_nloci;
+        C.A.tr_ll[t] = R.c_ll;
+      } else {
+        R.err |= ERR_TRACE;
+      }
+      CS.tr_count = t + 1;
+    }
+    if ((iter + 1) % K.thin == 0) {
+      double pves[3];
+      compute_pve(C, pves);
+      if (threadIdx.x == 0) {
+        uint64_t t = CS.th_count;
+        uint64_t ll0 = CS.th_loci_len;
+        if (t < K.th_cap && ll0 + R.c_nloci <= K.th_loci_cap) {
+          C.A.th_size[t] = (uint32_t)R.c_nloci;
+          for (int q = 0; q < R.c_nloci; ++q) C.A.th_loci[ll0 + q] = S.c_loci[q];
+          C.A.th_ll[t] = R.c_ll; C.A.th_s2[t] = R.sigma2; C.A.th_alpha[t] = R.alpha;
+          CS.th_count = t + 1; CS.th_loci_len = ll0 + R.c_nloci;
+        } else {
+          R.err |= ERR_TRACE;
+        }
+        CS.pves[0] = pves[0]; CS.pves[1] = pves[1]; CS.pves[2] = pves[2];
+      }
+    }
+    if ((iter + 1) % K.n_rao == 0) {
+      // decide the RB block now; the scan and proposal update run as
+      // separate kernels before the next iteration (src/sampler.cpp:731-811)
+      if (threadIdx.x == 0) {
+        CS.rb_pending = 1;
+        CS.rb_do_scan = (!K.flat || CS.n_rao_burnin <= 0);
+        if (CS.rb_do_scan) { CS.rb_call_idx = CS.rb_calls; CS.rb_calls++; }
+        if (CS.n_rao_burnin <= 0) {
+          CS.rb_accum_rao = 1;
+          CS.rao_z1 = (double)(CS.p_rao_n + 1);
+          CS.rao_z2 = (double)CS.p_rao_n / CS.rao_z1;
+          CS.p_rao_n++;
+        } else {
+          CS.rb_accum_rao = 0;
+          CS.n_rao_burnin--;
+        }
+        CS.rb_update_prop = 0; CS.rb_rebuild = 0;
+        if (K.adaptation || CS.n_rao_burnin > 0) {
+          if (!K.flat) {
+            CS.rb_update_prop = 1;
+            CS.prop_z1 = (double)(CS.p_proposal_n + 1);
+            CS.prop_z2 = (double)CS.p_proposal_n / CS.prop_z1;
+          }
+          if (K.adapt_p_ms) {
+            if (K.acpt_goal > 0) {
+              double t = fmax(1.0, K.k_move_size * (double)CS.p_proposal_n);
+              double ar = (double)CS.n_acpt[0] / (double)CS.n_moves[0];
+              double pm2 = CS.p_ms + (K.acpt_goal - ar) / (2.0 + t);
+              CS.p_ms = fmax(fmin(pm2, 0.99), 0.01);
+              CS.n_acpt[0] = 1; CS.n_moves[0] = 2;
+            } else {
+              double pms = CS.p_ms;
+              adapt_jd_mb(&pms, adapt, (int)K.max_ms);
+              CS.p_ms = pms;
+            }
+            geometric_cdf((int)K.max_ms, CS.p_ms, adapt);
+          }
+          CS.p_proposal_n++;
+          if (!K.flat) CS.rb_rebuild = 1;
+        }
+      }
+      if (threadIdx.x == 0) CS.n_iter = iter + 1;
+      __syncthreads();
+      return;   // the RB block runs before the next iteration
+    }
+    if (threadIdx.x == 0) CS.n_iter = iter + 1;
+  }
+}
+
+template <int KMAX>
+__global__ __launch_bounds__(256) void k_chains(KernelArgs args) {
+  __shared__ Smem<KMAX> S;
+  int c = blockIdx.x;
+  if (c >= args.n_chains) return;
+  ChainScal& CS = args.scal[c];
+  const ChainConst& K = args.consts[c];
+  const DevData& D = args.data[K.ds];
+  const ChainArrays& A = args.arrays[c];
+  Reg R;
+  Ctx<KMAX> C{S, R, D, K, A};
+  if (args.mode == 1) {
+    // setup (src/sampler.cpp:591-620): state was initialised on the host
+    // with an empty model; compute its factor, then sample
+    load_chain(C, CS);
+    compute_log_likelihood(C);
+    commit(C);
+    sample_beta_sigma2(C);
+    refresh_tau2(C);
+    store_chain(C, CS);
+    return;
+  }
+  if (CS.rb_pending) return;   // the RB block must run first
+  load_chain(C, CS);
+  run_iterations(C, CS, args.n_iter);
+  store_chain(C, CS);
+}
+
+// explicit instantiations used by the host
+template __global__ void k_chains<64>(KernelArgs);
+template __global__ void k_chains<128>(KernelArgs);
+template __global__ void k_chains<160>(KernelArgs);
+
+// ------------------------------------------------------- model op kernel --
+
+template <int KMAX>
+__device__ void swapadj_w0(Smem<KMAX>& S, int m, int col) {
+  if (threadIdx.x == 0) {
+    // src/symmmatrix.cpp:220-265 on the packed factor and v
+    for (int i = 0; i < col; ++i) {
+      double t = S.L[P(col) + i]; S.L[P(col) + i] = S.L[P(col + 1) + i]; S.L[P(col + 1) + i] = t;
+    }
+    double a = S.L[P(col + 1) + col];   // old (col, col+1)
+    double b = S.L[P(col + 1) + col + 1];
+    double dcc = S.L[P(col) + col];
+    S.L[P(col + 1) + col] = dcc;         // column col+1 gets old column col
+    S.L[P(col + 1) + col + 1] = 0.0;
+    double c, s;
+    drotg(a, b, c, s);
+    if (a < 0) { a = -a; s = -s; c = -c; }
+    S.L[P(col) + col] = a;
+    for (int i = col + 1; i < m; ++i) {
+      double x1 = S.L[P(i) + col], x2 = S.L[P(i) + col + 1];
+      double tmp = x1 * s - x2 * c;
+      S.L[P(i) + col] = x2 * s + x1 * c;
+      S.L[P(i) + col + 1] = tmp;
+    }
+    double x1 = S.v[col], x2 = S.v[col + 1];
+    double tmp = x1 * s - x2 * c;
+    S.v[col] = x2 * s + x1 * c;
+    S.v[col + 1] = tmp;
+  }
+  __syncthreads();
+}
+
+template <int KMAX>
+__global__ __launch_bounds__(256) void k_model_op(ModelOpArgs a) {
+  __shared__ Smem<KMAX> S;
+  ChainScal& CS = a.scal[0];
+  const ChainConst& K = a.consts[0];
+  const DevData& D = a.data[K.ds];
+  const ChainArrays& A = a.arrays[0];
+  Reg R;
+  Ctx<KMAX> C{S, R, D, K, A};
+  load_chain(C, CS);
+  if (a.op == 0) {
+    double d2[2] = {a.d0, a.d1};
+    add_term(C, a.ind, a.t_ind, d2);
+  } else if (a.op == 1) {
+    remove_term(C, a.model_ind);
+  } else if (a.op == 2) {
+    compute_log_likelihood(C);
+  } else {
+    swapadj_w0(S, R.cols, a.model_ind);
+    R.lmin = a.model_ind;
+  }
+  R.lmin = 0;
+  commit(C);
+  store_chain(C, CS);
+}
+
+template __global__ void k_model_op<64>(ModelOpArgs);
+template __global__ void k_model_op<128>(ModelOpArgs);
+template __global__ void k_model_op<160>(ModelOpArgs);

@@ -1,0 +1,479 @@
+// bmg_data.hip -- dataset kernels: layout, decode, recode, integer per-SNP
+// moments, synthetic genotypes, the X^T r scan and band X^T X (int8 MFMA).
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "bmg_device.h"
+#include "bmg_internal.h"
+
+using namespace bmg;
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+// ------------------------------------------------------------- layout ----
+// The reference stores each SNP in ceil(n/4) bytes (src/data.cpp:245-273).
+// In HBM every SNP starts on a 16-byte boundary (W32 words, multiple of 4),
+// trailing fields are code 00 (value 0), so word loads never straddle SNPs.
+__global__ void k_restride(const uint8_t* __restrict__ raw, uint64_t p, uint32_t stride_raw,
+                           uint32_t W32, uint8_t* __restrict__ out) {
+  uint64_t j = blockIdx.y + (uint64_t)blockIdx.z * gridDim.y;
+  if (j >= p) return;
+  uint32_t bytes = W32 * 4;
+  for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < bytes; b += gridDim.x * blockDim.x)
+    out[j * bytes + b] = b < stride_raw ? raw[j * stride_raw + b] : 0;
+}
+
+// src/data.cpp:324-339: flip 0 <-> 2 where the allele frequency exceeds 0.5.
+// Per word: fields with hi == lo (00 or 11) are inverted, others kept; the
+// fields past n are left at 00.
+__global__ void k_recode(uint32_t* g, uint64_t p, uint32_t W32, uint32_t n) {
+  __shared__ long long red[8];
+  uint64_t j = blockIdx.x;
+  if (j >= p) return;
+  uint32_t* w = g + j * W32;
+  long long s = 0, ng = 0;
+  for (uint32_t k = threadIdx.x; k < W32; k += blockDim.x) {
+    uint32_t x = w[k];
+    uint32_t lo = x & BMG_EVEN, hi = (x >> 1) & BMG_EVEN;
+    s += __popc(hi) + __popc(hi & lo);
+    uint32_t valid = k * 16 + 16 <= n ? 0xFFFFFFFFu
+                     : (k * 16 >= n ? 0u : ((1u << (2 * (n - k * 16))) - 1u));
+    ng += __popc(~(lo & ~hi) & BMG_EVEN & valid);   // not missing
+  }
+  s = block_sum_i64(s, red);
+  ng = block_sum_i64(ng, red);
+  if ((double)s / (2.0 * (double)ng) > 0.5) {
+    for (uint32_t k = threadIdx.x; k < W32; k += blockDim.x) {
+      uint32_t x = w[k];
+      uint32_t lo = x & BMG_EVEN, hi = (x >> 1) & BMG_EVEN;
+      uint32_t valid = k * 16 + 16 <= n ? 0xFFFFFFFFu
+                       : (k * 16 >= n ? 0u : ((1u << (2 * (n - k * 16))) - 1u));
+      uint32_t eq = ~(hi ^ lo) & BMG_EVEN & valid;
+      w[k] = x ^ (eq | (eq << 1));
+    }
+  }
+}
+
+// Per SNP integer counts: c1 = #(code 10), c2 = #(code 11), nm = #not missing.
+// All moments of every coding are integer functions of these.
+__global__ void k_counts(const uint32_t* __restrict__ g, uint64_t p, uint32_t W32, uint32_t n,
+                         long long* __restrict__ out) {
+  __shared__ long long red[8];
+  uint64_t j = blockIdx.x;
+  if (j >= p) return;
+  const uint32_t* w = g + j * W32;
+  long long c1 = 0, c2 = 0, miss = 0;
+  for (uint32_t k = threadIdx.x; k < W32; k += blockDim.x) {
+    uint32_t x = w[k];
+    uint32_t lo = x & BMG_EVEN, hi = (x >> 1) & BMG_EVEN;
+    c1 += __popc(hi & ~lo);
+    c2 += __popc(hi & lo);
+    miss += __popc(lo & ~hi);
+  }
+  c1 = block_sum_i64(c1, red);
+  c2 = block_sum_i64(c2, red);
+  miss = block_sum_i64(miss, red);
+  if (threadIdx.x == 0) {
+    out[3 * j] = c1; out[3 * j + 1] = c2; out[3 * j + 2] = (long long)n - miss;
+  }
+}
+
+// src/precomputed_snp_covariances.hpp:95-131 from the integer counts; the
+// sums are exact, so the doubles equal the reference's loop bit for bit.
+__global__ void k_self_moments(const long long* __restrict__ cnt, uint64_t p, uint32_t n,
+                               int at0, int at1, int at2, int at3, int ah, int offset,
+                               double* __restrict__ pre) {
+  uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (j >= p) return;
+  long long c1 = cnt[3 * j], c2 = cnt[3 * j + 1];
+  const int allow[4] = {at0, at1, at2, at3};
+  // sum and sum of squares of A, H, D, R
+  const long long sum[4] = {c1 + 2 * c2, c1, c1 + c2, c2};
+  const long long sq[4] = {c1 + 4 * c2, c1, c1 + c2, c2};
+  double* o = pre + j * offset;
+  int k = 0;
+  for (int t = 0; t < 4; ++t) {
+    if (!allow[t]) continue;
+    double s = (double)sum[t];
+    o[k++] = s;
+    o[k++] = (double)sq[t] - s * s / (double)n;
+  }
+  if (ah) {
+    double as = (double)sum[0], hs = (double)sum[1];
+    o[k++] = (double)c1 - as * hs / (double)n;  // sum a*h = #(code 10)
+  }
+}
+
+// DataModel::get_genotypes[coding] for a list of SNPs (values incl. -1)
+__global__ void k_decode(const uint32_t* __restrict__ g, uint32_t W32, uint32_t n,
+                         const uint64_t* __restrict__ snps, uint64_t count, int coding,
+                         double* __restrict__ out) {
+  uint64_t s = blockIdx.y + (uint64_t)blockIdx.z * gridDim.y;
+  if (s >= count) return;
+  const uint32_t* w = g + snps[s] * W32;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    uint32_t c = (w[i >> 4] >> (2 * (i & 15))) & 3u;
+    out[s * n + i] = code_value(c, coding);
+  }
+}
+
+// ---------------------------------------------------------- synthetic ----
+// Latent AR(1) haplotypes (see the header for the model).  One thread owns
+// four individuals (one byte) within one LD block and walks the block's SNPs.
+__device__ inline double synth_normal(uint32_t seed, uint64_t j, uint32_t hap) {
+  uint32_t x0 = (uint32_t)j, x1 = (uint32_t)(j >> 32), x2 = hap, x3 = 0x40000000u;
+  philox10(x0, x1, x2, x3, seed, 0x5EED5EEDu);
+  double ua = words_u01(x0, x1), ub = words_u01(x2, x3);
+  return sqrt(-2.0 * log(1.0 - ua)) * cos(6.283185307179586 * ub);
+}
+
+__global__ void k_synth(uint32_t seed, uint32_t n, uint64_t p, uint32_t W32,
+                        const double* __restrict__ thresh, const double* __restrict__ rho,
+                        const uint64_t* __restrict__ blk_start, uint64_t n_blocks,
+                        uint8_t* __restrict__ out) {
+  uint32_t nq = (n + 3) / 4;
+  uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t b = blockIdx.y + (uint64_t)blockIdx.z * gridDim.y;
+  if (q >= nq || b >= n_blocks) return;
+  uint64_t j0 = blk_start[b], j1 = (b + 1 < n_blocks) ? blk_start[b + 1] : p;
+  double z[8];
+  uint32_t bytes = W32 * 4;
+  for (uint64_t j = j0; j < j1; ++j) {
+    double r = rho[j], t = thresh[j];
+    double sq = (r == 0.0) ? 0.0 : sqrt(1.0 - r * r);
+    uint8_t byte = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      uint32_t i = 4 * q + k;
+      if (i >= n) break;
+      double e0 = synth_normal(seed, j, 2 * i), e1 = synth_normal(seed, j, 2 * i + 1);
+      if (r == 0.0) { z[2 * k] = e0; z[2 * k + 1] = e1; }
+      else {
+        z[2 * k] = r * z[2 * k] + sq * e0;
+        z[2 * k + 1] = r * z[2 * k + 1] + sq * e1;
+      }
+      int gv = (z[2 * k] < t) + (z[2 * k + 1] < t);
+      uint8_t code = gv == 0 ? 0 : (gv == 1 ? 2 : 3);
+      byte |= (uint8_t)(code << (2 * k));
+    }
+    out[j * bytes + q] = byte;
+  }
+}
+
+// --------------------------------------------------------- X^T r scan ----
+// The residual r (FP64) is rounded to fixed point R = round(r / 2^e) with
+// |R| < 2^(8D - 2) and split into D balanced base-256 digits in [-128, 127].
+// x^T R is then an exact integer product done with int8 MFMA
+// (v_mfma_i32_16x16x64_i8): A = 16 SNPs x 64 individuals (2-bit codes
+// expanded to int8 in registers), B = 64 individuals x 16 digit columns
+// (vector v, digit d at column v*D + d).  x^T r = 2^e sum_d 256^d C_d.
+__global__ void k_absmax(const double* __restrict__ r, uint32_t n_pad, int nvec,
+                         double* __restrict__ scale, int digits) {
+  __shared__ double red[8];
+  int v = blockIdx.x;
+  double m = 0.0;
+  for (uint32_t i = threadIdx.x; i < n_pad; i += blockDim.x) m = fmax(m, fabs(r[v * (size_t)n_pad + i]));
+  m = wave_max(m);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x / 64] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0;
+    for (int i = 0; i < (int)(blockDim.x / 64); ++i) t = fmax(t, red[i]);
+    int e = 0;
+    if (t > 0) { frexp(t, &e); }          // t < 2^e
+    scale[v] = ldexp(1.0, e - (8 * digits - 2));
+  }
+}
+
+__global__ void k_digits(const double* __restrict__ r, uint32_t n_pad, int nvec, int digits,
+                         const double* __restrict__ scale, int8_t* __restrict__ dig) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  int v = blockIdx.y;
+  if (i >= n_pad) return;
+  long long R = llrint(r[v * (size_t)n_pad + i] / scale[v]);
+  for (int d = 0; d < digits; ++d) {
+    int dg = (int)(((R + 128) & 0xFF)) - 128;
+    dig[(size_t)(v * digits + d) * n_pad + i] = (int8_t)dg;
+    R = (R - dg) >> 8;
+  }
+}
+
+// grid.x: groups of 4 waves x NT tiles x 16 SNPs; grid.y: individual tiles
+template <int NT>
+__global__ __launch_bounds__(256) void k_scan_mfma(const uint32_t* __restrict__ g, uint64_t p,
+                                                   uint32_t W32, int coding,
+                                                   const int8_t* __restrict__ dig, int NC,
+                                                   uint32_t n_pad, uint32_t tile_words,
+                                                   int32_t* __restrict__ acc, int atomic) {
+  extern __shared__ int8_t sdig[];  // NC x (tile_words * 16)
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t w0 = blockIdx.y * tile_words;
+  const uint32_t w1 = min(W32, w0 + tile_words);
+  const uint32_t ind0 = w0 * 16, nind = (w1 - w0) * 16;
+  // stage the digit columns of this individual tile
+  for (uint32_t e = threadIdx.x * 16; e < (uint32_t)NC * nind; e += blockDim.x * 16) {
+    uint32_t col = e / nind, off = e % nind;
+    *(uint4*)(sdig + (size_t)col * nind + off) =
+        *(const uint4*)(dig + (size_t)col * n_pad + ind0 + off);
+  }
+  __syncthreads();
+  const int row = lane & 15, grp = lane >> 4;
+  const uint64_t base = ((uint64_t)blockIdx.x * 4 + wave) * (NT * 16);
+  v4i c[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) c[t] = v4i{0, 0, 0, 0};
+  const int col = row;
+  for (uint32_t wb = w0; wb < w1; wb += 16) {
+    uint4 a4[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      uint64_t snp = base + t * 16 + row;
+      uint32_t wi = wb + 4 * grp;
+      if (snp < p && wi < w1) a4[t] = *(const uint4*)(g + snp * W32 + wi);
+      else a4[t] = make_uint4(0, 0, 0, 0);
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      uint32_t wi = wb + 4 * grp + s;
+      v4i b = {0, 0, 0, 0};
+      if (col < NC && wi < w1) {
+        uint4 bb = *(const uint4*)(sdig + (size_t)col * nind + (wi - w0) * 16);
+        b = v4i{(int)bb.x, (int)bb.y, (int)bb.z, (int)bb.w};
+      }
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        uint32_t w = s == 0 ? a4[t].x : (s == 1 ? a4[t].y : (s == 2 ? a4[t].z : a4[t].w));
+        uint32_t o[4];
+        word_to_i8x16(w, coding, o);
+        v4i a = {(int)o[0], (int)o[1], (int)o[2], (int)o[3]};
+        c[t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, c[t], 0, 0, 0);
+      }
+    }
+  }
+  // lane holds C[4 * grp + r][col]
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      uint64_t snp = base + t * 16 + 4 * grp + r;
+      if (snp < p && col < NC) {
+        if (atomic) atomicAdd(&acc[snp * 16 + col], c[t][r]);
+        else acc[snp * 16 + col] = c[t][r];
+      }
+    }
+  }
+}
+
+__global__ void k_scan_combine(const int32_t* __restrict__ acc, uint64_t p, int nvec, int digits,
+                               const double* __restrict__ scale, double* __restrict__ out) {
+  uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  int v = blockIdx.y;
+  if (j >= p) return;
+  long long s = 0;
+  for (int d = digits - 1; d >= 0; --d) s = s * 256 + (long long)acc[j * 16 + v * digits + d];
+  out[(size_t)v * p + j] = ldexp((double)s, 0) * scale[v];
+}
+
+// ------------------------------------------------------ band X^T X ------
+// out[j * W + d] = sum_i x_j(i) x_{j+d}(i).  A workgroup owns 64 row SNPs
+// [r0, r0 + 64) and the column SNPs [r0, r0 + 64 + W); per chunk of 64
+// individuals it expands those SNPs to int8 in LDS once and then each wave
+// runs 16x16x64 int8 MFMAs over its share of the (row tile, column tile)
+// pairs that intersect the band.
+#define BAND_ROWS 64
+#define BAND_MAXP 36   // (row tile, column tile) pairs per wave; W <= 560
+__global__ __launch_bounds__(256) void k_band(const uint32_t* __restrict__ g, uint64_t p,
+                                              uint32_t W32, uint32_t W, int ca, int cb,
+                                              int32_t* __restrict__ out, uint64_t n_row_blocks) {
+  extern __shared__ uint32_t sx[];  // [2 codings][ncol SNPs][16 words]: int8 of 64 individuals
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // XCD-aware ordering: workgroups are dealt round-robin over the 8 XCDs, so
+  // row block rb goes to XCD rb / per; neighbouring row blocks (whose column
+  // windows overlap) then share that XCD's L2.
+  uint64_t nb = n_row_blocks;
+  uint64_t per = (nb + 7) / 8;
+  uint64_t bid = blockIdx.x;
+  uint64_t rb = (bid % 8) * per + bid / 8;
+  if (rb >= nb) return;
+  uint64_t r0 = rb * BAND_ROWS;
+  const uint32_t ncol_tiles = (BAND_ROWS + W + 15) / 16;
+  const uint32_t ncol = ncol_tiles * 16;
+  // column-tile offsets d per row tile: (ct - rt) * 16 < W + 16
+  const int ND = (int)((W + 16 + 15) / 16);
+  const int npairs = 4 * ND;
+  v4i c[BAND_MAXP];
+#pragma unroll
+  for (int i = 0; i < BAND_MAXP; ++i) c[i] = v4i{0, 0, 0, 0};
+  uint32_t* xa = sx;
+  uint32_t* xb = sx + (size_t)ncol * 16;
+  const int row = lane & 15, grp = lane >> 4;
+  for (uint32_t wb = 0; wb < W32; wb += 4) {
+    __syncthreads();
+    for (uint32_t e = threadIdx.x; e < ncol * 4; e += blockDim.x) {
+      uint32_t sn = e / 4, q = e % 4;
+      uint64_t snp = r0 + sn;
+      uint32_t w = (snp < p && wb + q < W32) ? g[snp * W32 + wb + q] : 0u;
+      uint32_t oa[4], ob[4];
+      word_to_i8x16(w, ca, oa);
+      word_to_i8x16(w, cb, ob);
+      *(uint4*)(xa + (size_t)sn * 16 + q * 4) = make_uint4(oa[0], oa[1], oa[2], oa[3]);
+      *(uint4*)(xb + (size_t)sn * 16 + q * 4) = make_uint4(ob[0], ob[1], ob[2], ob[3]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < BAND_MAXP; ++i) {
+      int k = wave + 4 * i;
+      if (k < npairs) {
+        int rt = k / ND, ct = rt + k % ND;
+        uint32_t sa = rt * 16 + row, sb = ct * 16 + row;
+        uint4 av = *(const uint4*)(xa + (size_t)sa * 16 + grp * 4);
+        uint4 bv = *(const uint4*)(xb + (size_t)sb * 16 + grp * 4);
+        v4i a = {(int)av.x, (int)av.y, (int)av.z, (int)av.w};
+        v4i b = {(int)bv.x, (int)bv.y, (int)bv.z, (int)bv.w};
+        c[i] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, c[i], 0, 0, 0);
+      }
+    }
+  }
+  const int col = lane & 15;
+#pragma unroll
+  for (int i = 0; i < BAND_MAXP; ++i) {
+    int k = wave + 4 * i;
+    if (k < npairs) {
+      int rt = k / ND, ct = rt + k % ND;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        uint64_t j = r0 + rt * 16 + 4 * grp + r;   // row SNP
+        uint64_t jj = r0 + ct * 16 + col;          // column SNP
+        if (j < p && jj >= j && jj - j < W && jj < p) out[j * W + (jj - j)] = c[i][r];
+      }
+    }
+  }
+}
+
+// -------------------------------------------------------------- host ----
+namespace bmg {
+
+int ensure(void** ptr, size_t* have, size_t need) {
+  if (*have >= need) return BMG_OK;
+  if (*ptr) BMG_HIP(hipFree(*ptr));
+  *ptr = nullptr; *have = 0;
+  BMG_HIP(hipMalloc(ptr, need));
+  *have = need;
+  return BMG_OK;
+}
+
+int ctx_prepare(bmg_ctx* c, int recode) {
+  hipStream_t s = c->stream;
+  if (recode) {
+    hipLaunchKernelGGL(k_recode, dim3((unsigned)c->p), dim3(256), 0, s, c->g, c->p, c->W32, c->n);
+    BMG_HIP(hipGetLastError());
+  }
+  return ctx_self_moments(c);
+}
+
+int ctx_self_moments(bmg_ctx* c) {
+  hipStream_t s = c->stream;
+  long long* cnt = nullptr;
+  BMG_HIP(hipMalloc(&cnt, sizeof(long long) * 3 * c->p));
+  hipLaunchKernelGGL(k_counts, dim3((unsigned)c->p), dim3(256), 0, s, c->g, c->p, c->W32, c->n, cnt);
+  BMG_HIP(hipGetLastError());
+  if (c->pre) BMG_HIP(hipFree(c->pre));
+  BMG_HIP(hipMalloc(&c->pre, sizeof(double) * c->offset * c->p));
+  hipLaunchKernelGGL(k_self_moments, dim3((unsigned)((c->p + 255) / 256)), dim3(256), 0, s, cnt,
+                     c->p, c->n, c->allow_terms[0], c->allow_terms[1], c->allow_terms[2],
+                     c->allow_terms[3], c->allow_types[BMG_AH], c->offset, c->pre);
+  BMG_HIP(hipGetLastError());
+  // dataset statistics in SNP order (src/data.cpp:403-434), exact integer sums
+  std::vector<long long> h(3 * c->p);
+  BMG_HIP(hipMemcpyAsync(h.data(), cnt, sizeof(long long) * 3 * c->p, hipMemcpyDeviceToHost, s));
+  BMG_HIP(hipStreamSynchronize(s));
+  BMG_HIP(hipFree(cnt));
+  double tv = 0, tm = 0; uint64_t nsm = 0, nsv = 0, miss = 0;
+  for (uint64_t j = 0; j < c->p; ++j) {
+    long long c1 = h[3 * j], c2 = h[3 * j + 1], ng = h[3 * j + 2];
+    miss += (uint64_t)(c->n - ng);
+    double sq = (double)(c1 + 4 * c2), sm = (double)(c1 + 2 * c2);
+    if (ng > 1) {
+      tm += sm / ng; tv += (sq - sm * sm / ng) / (ng - 1); ++nsm; ++nsv;
+    } else if (ng == 1) {
+      tm += sm / ng; ++nsm;
+    }
+  }
+  c->var_x = tv / nsv;
+  c->mean_x = tm / nsm;
+  c->n_missing = miss;
+  return BMG_OK;
+}
+
+int launch_decode(bmg_ctx* c, const uint64_t* snps_dev, uint64_t count, int coding,
+                  double* out_dev, hipStream_t s) {
+  unsigned gy = (unsigned)std::min<uint64_t>(count, 65535);
+  unsigned gz = (unsigned)((count + gy - 1) / gy);
+  unsigned gx = (unsigned)std::min<uint64_t>((c->n + 255) / 256, 64);
+  hipLaunchKernelGGL(k_decode, dim3(gx, gy, gz), dim3(256), 0, s, c->g, c->W32, c->n, snps_dev,
+                     count, coding, out_dev);
+  BMG_HIP(hipGetLastError());
+  return BMG_OK;
+}
+
+int launch_synth(bmg_ctx* c, uint32_t seed, const double* thresh_dev, const double* rho_dev,
+                 const uint64_t* blk_start_dev, uint64_t n_blocks, hipStream_t s) {
+  uint32_t nq = (c->n + 3) / 4;
+  unsigned gx = (nq + 63) / 64;
+  unsigned gy = (unsigned)std::min<uint64_t>(n_blocks, 65535);
+  unsigned gz = (unsigned)((n_blocks + gy - 1) / gy);
+  BMG_HIP(hipMemsetAsync(c->g, 0, (size_t)c->p * c->W32 * 4, s));
+  hipLaunchKernelGGL(k_synth, dim3(gx, gy, gz), dim3(64), 0, s, seed, c->n, c->p, c->W32,
+                     thresh_dev, rho_dev, blk_start_dev, n_blocks, (uint8_t*)c->g);
+  BMG_HIP(hipGetLastError());
+  return BMG_OK;
+}
+
+int launch_scan(bmg_ctx* c, const double* r_dev, int nvec, int coding, int digits,
+                double* out_dev, hipStream_t s) {
+  if (nvec < 1 || digits < 1 || digits > 6 || nvec * digits > 16) {
+    set_error("scan: need 1 <= nvec*digits <= 16, digits <= 6");
+    return BMG_E_ARG;
+  }
+  int NC = nvec * digits;
+  if (ensure((void**)&c->dig, &c->dig_bytes, (size_t)NC * c->n_pad)) return BMG_E_HIP;
+  if (ensure((void**)&c->acc, &c->acc_bytes, (size_t)c->p * 16 * sizeof(int32_t))) return BMG_E_HIP;
+  if (!c->scale) BMG_HIP(hipMalloc(&c->scale, sizeof(double) * 16));
+  hipLaunchKernelGGL(k_absmax, dim3(nvec), dim3(256), 0, s, r_dev, c->n_pad, nvec, c->scale, digits);
+  hipLaunchKernelGGL(k_digits, dim3((c->n_pad + 255) / 256, nvec), dim3(256), 0, s, r_dev,
+                     c->n_pad, nvec, digits, c->scale, c->dig);
+  BMG_HIP(hipGetLastError());
+  // individual tiles of at most 64 KiB of digits in LDS
+  uint32_t tile_words = (uint32_t)std::max<size_t>(16, (65536 / NC) / 16 / 16 * 16);
+  tile_words = std::min<uint32_t>(tile_words, (c->W32 + 15) / 16 * 16);
+  uint32_t n_tiles = (c->W32 + tile_words - 1) / tile_words;
+  const int NT = 2;
+  uint64_t per_block = 4ull * NT * 16;
+  unsigned gx = (unsigned)((c->p + per_block - 1) / per_block);
+  int atomic = n_tiles > 1;
+  if (atomic) BMG_HIP(hipMemsetAsync(c->acc, 0, (size_t)c->p * 16 * sizeof(int32_t), s));
+  size_t lds = (size_t)NC * tile_words * 16;
+  hipLaunchKernelGGL(k_scan_mfma<NT>, dim3(gx, n_tiles), dim3(256), lds, s, c->g, c->p, c->W32,
+                     coding, c->dig, NC, c->n_pad, tile_words, c->acc, atomic);
+  BMG_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_scan_combine, dim3((unsigned)((c->p + 255) / 256), nvec), dim3(256), 0, s,
+                     c->acc, c->p, nvec, digits, c->scale, out_dev);
+  BMG_HIP(hipGetLastError());
+  return BMG_OK;
+}
+
+int launch_band(bmg_ctx* c, uint32_t W, int ca, int cb, int32_t* out_dev, hipStream_t s) {
+  if (W < 1 || W > 560) { set_error("band: 1 <= W <= 560"); return BMG_E_ARG; }
+  if ((W + 31) / 16 > BAND_MAXP) { set_error("band: W too large"); return BMG_E_ARG; }
+  uint64_t nrb = (c->p + BAND_ROWS - 1) / BAND_ROWS;
+  uint32_t ncol = ((BAND_ROWS + W + 15) / 16) * 16;
+  size_t lds = (size_t)2 * ncol * 16 * sizeof(uint32_t);
+  unsigned grid = (unsigned)(((nrb + 7) / 8) * 8);
+  hipLaunchKernelGGL(k_band, dim3(grid), dim3(256), lds, s, c->g, c->p, c->W32, W, ca, cb, out_dev, nrb);
+  BMG_HIP(hipGetLastError());
+  return BMG_OK;
+}
+
+}  // namespace bmg

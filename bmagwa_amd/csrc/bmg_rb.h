@@ -1,0 +1,18 @@
+// bmg_rb.h -- launchers of the RB block kernels (bmg_rb.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "bmg_chain.h"
+
+namespace bmg {
+int launch_residuals(const double* y, uint32_t n, uint32_t n_pad, const ChainArrays* arrays,
+                     const int* list, int nv, double* r, double* rsum, hipStream_t s);
+int launch_rb_finalize(const DevData* data, const ChainConst* consts, const ChainScal* scal,
+                       const ChainArrays* arrays, const int* list, int nv, int kmax, uint64_t p,
+                       const double* xa, const double* xh, const double* rsum, hipStream_t s);
+int launch_rb_update(const DevData* data, const ChainConst* consts, const ChainScal* scal,
+                     const ChainArrays* arrays, const int* list, int nv, uint64_t p, hipStream_t s);
+int launch_tree_build(const DevData* data, const ChainConst* consts, ChainScal* scal,
+                      const ChainArrays* arrays, const int* list, int nv, const uint64_t* lev_len,
+                      int top, int only_rebuild, int clear_pending, hipStream_t s);
+}  // namespace bmg

@@ -1,0 +1,211 @@
+/*
+ * bmagwa_hip.h -- C ABI of the MI355X (gfx950) engine for BMAGWA's MCMC step
+ * over SNP inclusion.
+ *
+ * Plain pointers and sizes only.  Pointers named *_dev are device pointers;
+ * every other pointer is host memory.  Every call returns a status code
+ * (BMG_OK = 0); bmg_last_error() gives the message of the last failure on the
+ * calling thread.  A non-positive-definite Cholesky factor is not an error:
+ * as in the reference it yields a log-likelihood of -INFINITY
+ * (src/model.hpp:210-220,476-484).
+ *
+ * Handles: a bmg_ctx is one dataset resident in HBM (Data + DataModel +
+ * PrecomputedSNPCovariances of the reference); a bmg_model is one FP64 linear
+ * model state on the GPU (Model); a bmg_batch is a set of chains, possibly on
+ * several datasets, advanced by one persistent launch (Sampler x chains).
+ */
+#ifndef BMAGWA_HIP_H_
+#define BMAGWA_HIP_H_
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  BMG_OK = 0,
+  BMG_E_ARG = -1,         /* invalid argument */
+  BMG_E_HIP = -2,         /* HIP runtime error */
+  BMG_E_UNSUPPORTED = -3, /* valid for the reference, not built for the GPU */
+  BMG_E_CAPACITY = -4,    /* model larger than the compiled capacity */
+  BMG_E_STATE = -5        /* call out of order */
+};
+
+/* effect types, src/data_model.hpp:41 */
+enum { BMG_A = 0, BMG_H = 1, BMG_D = 2, BMG_R = 3, BMG_AH = 4 };
+
+typedef struct bmg_ctx bmg_ctx;
+typedef struct bmg_model bmg_model;
+typedef struct bmg_batch bmg_batch;
+
+const char* bmg_last_error(void);
+int bmg_device_count(int* count);
+
+/* Sampler options: the INI keys of src/options.hpp:138-321 (same field order
+ * and meaning as the [sizes]/[sampler]/[thread]/[model]/[prior] sections). */
+typedef struct {
+  uint32_t n; uint32_t m_e; uint64_t m_g;
+  uint64_t do_n_iter;
+  uint32_t n_rao, n_rao_burnin, thin, n_sample_tau2_and_missing;
+  uint32_t max_move_size, max_SNP_neighborhood_size, delay_rejection;
+  uint32_t seed;
+  double   p_move_size, p_move_size_nbs, p_move_size_nbc;
+  double   p_move_size_acpt_goal;
+  int32_t  adapt_p_move_size, adaptation, flat_proposal_dist, save_beta;
+  int32_t  sampler_type;
+  int32_t  n_types; int32_t types[5];
+  double   e_qg, var_qg, nu_sigma2, R2mode_sigma2, s2_sigma2, mu_alpha;
+  double   inv_tau2_e_const_val, inv_tau2_e_val;
+  int32_t  use_individual_tau2;
+  double   types_prior[5];
+  double   nu_tau2[4], eh_tau2[4], s2_tau2[4];
+  int32_t  recode_g_to_minor_allele_count;
+  uint32_t chain;
+} bmg_opts;
+
+/* ------------------------------------------------------------ dataset --- */
+/* Replaces Data(n, m_g, m_e, file_fam, file_g, ...) (src/data.hpp:45-72) with
+ * the .bed body already read (src/data.cpp:245-273: p * ceil(n/4) bytes after
+ * the 3-byte header) and y from the .fam/.y file; DataModel(data, types)
+ * (src/data_model.hpp:52-89) and PrecomputedSNPCovariances(data_model)
+ * (src/precomputed_snp_covariances.hpp:58-87) are built on the device.
+ * Covariates (m_e > 0) are not supported by the engine (BMG_E_UNSUPPORTED);
+ * the constant column is always present.  recode != 0 applies
+ * recode_g_to_minor_allele_count (src/data.cpp:324-339). */
+int bmg_ctx_create(int device, const uint8_t* bed_body, uint32_t n, uint64_t p,
+                   const double* y, const int32_t* types, int n_types, int recode,
+                   bmg_ctx** out);
+/* Same, genotypes generated on the device by the synthetic-data model of the
+ * benchmark (latent AR(1) haplotypes, thresh = Phi^-1(MAF), rho_j links SNP j
+ * to j-1, 0 starts a block); y is supplied by the caller afterwards. */
+int bmg_ctx_create_synthetic(int device, uint32_t seed, uint32_t n, uint64_t p,
+                             const double* thresh, const double* rho,
+                             const int32_t* types, int n_types, bmg_ctx** out);
+int bmg_ctx_set_y(bmg_ctx* ctx, const double* y);
+int bmg_ctx_destroy(bmg_ctx* ctx);
+/* packed body back to the host, reference layout (p * ceil(n/4) bytes) */
+int bmg_ctx_read_body(bmg_ctx* ctx, uint8_t* out);
+/* yy, var_y, var_x, mean_x (src/data.hpp:67-71, src/data.cpp:403-434) */
+int bmg_ctx_stats(bmg_ctx* ctx, double out[4]);
+/* number of missing genotype codes in the dataset */
+int bmg_ctx_missing_count(bmg_ctx* ctx, uint64_t* out);
+
+/* Replaces DataModel::get_genotypes[type](snp, v) (src/data_model.hpp:44-59,
+ * src/data.cpp:56-138); -1 marks a missing genotype as in Data. */
+int bmg_decode(bmg_ctx* ctx, uint64_t snp, int coding, double* out_n);
+/* many SNPs at once: out[i * n + k] */
+int bmg_decode_many(bmg_ctx* ctx, const uint64_t* snps, uint64_t count, int coding,
+                    double* out);
+
+/* Replaces PrecomputedSNPCovariances::xx (src/precomputed_snp_covariances.hpp:
+ * 95-131): p rows of `offset` doubles (offset from bmg_self_moments_layout),
+ * bit-exact (integer sums). */
+int bmg_self_moments_layout(bmg_ctx* ctx, int* offset, int offset_type[4]);
+int bmg_self_moments(bmg_ctx* ctx, double* out);
+
+/* Band covariances (no reference counterpart; the reference's precomputed
+ * covariances are per-SNP only): out[j * W + d] = sum_i x_j(i) x_{j+d}(i),
+ * d < W, exact int32, codings ca/cb in {A,H,D,R}, int8 MFMA. */
+int bmg_band_xtx(bmg_ctx* ctx, uint32_t W, int coding_a, int coding_b, int32_t* out);
+int bmg_band_xtx_dev(bmg_ctx* ctx, uint32_t W, int coding_a, int coding_b,
+                     int32_t* out_dev, void* stream);
+
+/* X^T R for nvec residual vectors (the inner product of the RB scan,
+ * src/sampler.cpp:171,173,188): r is nvec x n (row-major, host), out is
+ * nvec x p.  Exact integer MFMA on the fixed-point residual; relative
+ * quantisation error <= 2^-(8*digits-2) of max|r| per product term. */
+int bmg_scan_xtr(bmg_ctx* ctx, const double* r, int nvec, int coding, int digits,
+                 double* out);
+/* device-pointer form for timing: r_dev nvec x n_pad (double), out_dev nvec x p */
+int bmg_scan_xtr_dev(bmg_ctx* ctx, const double* r_dev, int nvec, int coding,
+                     int digits, double* out_dev, void* stream);
+int bmg_ctx_n_pad(bmg_ctx* ctx, uint32_t* n_pad);
+
+/* -------------------------------------------------------------- model --- */
+/* Prior inputs of Model/Prior (src/prior.hpp:45-122), already solved. */
+typedef struct {
+  double n_plus_nu, minus_n_plus_nu_2, nus2_plus_yy;
+  double inv_tau2_e0;          /* d of the constant column */
+} bmg_model_prior;
+
+/* Replaces Model(data, data_model, prior) (src/model.hpp:49-113) for data
+ * without covariates; kmax = column capacity (<= bmg_model_kmax()). */
+int bmg_model_kmax(void);
+int bmg_model_create(bmg_ctx* ctx, const bmg_model_prior* prior, bmg_model** out);
+int bmg_model_destroy(bmg_model* m);
+/* Model::add_term(ind, t_ind, inv_tau2_alpha2) (src/model.hpp:239-268) */
+int bmg_model_add_term(bmg_model* m, uint32_t ind, int t_ind, const double* d);
+/* Model::remove_term(model_ind) (src/model.hpp:270-312) */
+int bmg_model_remove_term(bmg_model* m, int model_ind);
+/* Model::compute_log_likelihood() (src/model.hpp:199-237) */
+int bmg_model_compute_log_likelihood(bmg_model* m);
+/* Model::log_likelihood(), size() */
+int bmg_model_log_likelihood(bmg_model* m, double* out);
+int bmg_model_size(bmg_model* m, int* out);
+/* Model::operator= (src/model.hpp:115-162) */
+int bmg_model_copy(bmg_model* dst, const bmg_model* src);
+/* ExhModel support: SymmMatrix::cholesky_swapadj on the model's factor and v
+ * (src/symmmatrix.cpp:220-265) */
+int bmg_model_swapadj(bmg_model* m, int col);
+/* state: xx (k x k col-major, upper), l (same), xy, v, d (k each), scal[6] =
+ * loglik, syx_plus_vs2, log_det_invQ, log_det_invQ_plus_xx, sigma2, k */
+int bmg_model_get(bmg_model* m, double* xx, double* l, double* xy, double* v,
+                  double* d, double scal[6]);
+int bmg_model_set_d(bmg_model* m, int x_ind, double val);
+
+/* ------------------------------------------------------ chains (batch) --- */
+/* A batch runs n_chains PMV chains (src/sampler.cpp:551-880, moves 0-2 of
+ * src/sampler.hpp:899-1669) entirely on the device: one workgroup per chain,
+ * FP64 model in LDS, all-SNP RB scans between segments.  chain_ctx[c] picks
+ * the dataset of chain c (phenotypes with different individual sets are
+ * separate datasets).  opts[c] are the options of chain c (seed = its
+ * reference seed).  Unsupported on the device: delay_rejection > 0,
+ * multiple effect types, covariates, missing genotypes. */
+typedef struct {
+  int32_t trace_every_iter;    /* keep per-iteration move/jd/size/loglik */
+  int32_t proposal;            /* 0 plain (RB running mean), 1 learned */
+  int32_t learn_every;         /* learned proposal: retrain period (iters) */
+  int32_t learn_freeze_iter;   /* learned proposal: frozen after this iter */
+  int32_t scan_digits;         /* fixed-point digits of the RB scan (4..5) */
+  int32_t use_graph;           /* replay segments as a captured hipGraph */
+  uint64_t trace_capacity;     /* thinned loci entries kept per chain */
+} bmg_batch_cfg;
+
+int bmg_batch_create(bmg_ctx** ctxs, int n_ctx, int n_chains, const int32_t* chain_ctx,
+                     const bmg_opts* opts, const bmg_batch_cfg* cfg, bmg_batch** out);
+int bmg_batch_destroy(bmg_batch* b);
+/* Sampler::initialize_p_proposal_flat + the set-up of sample()
+ * (src/sampler.hpp:351-362, src/sampler.cpp:591-620) */
+int bmg_batch_setup(bmg_batch* b);
+/* n_iter more iterations of every chain (the loop of src/sampler.cpp:626-834) */
+int bmg_batch_run(bmg_batch* b, uint64_t n_iter);
+int bmg_batch_sync(bmg_batch* b);
+/* per-chain state: loglik, size, sigma2, alpha, p_move_size, n_accepted,
+ * n_iter, loci_len */
+int bmg_batch_state(bmg_batch* b, int chain, double out[8]);
+int bmg_batch_loci(bmg_batch* b, int chain, uint32_t* loci, int cap, int* k);
+/* p_rao (RB PIP estimate, src/sampler.cpp:739-757) and p_proposal */
+int bmg_batch_p_rao(bmg_batch* b, int chain, double* out);
+int bmg_batch_p_proposal(bmg_batch* b, int chain, double* out);
+/* per-iteration trace (needs trace_every_iter): move, jd, movesize, size,
+ * loglik; returns the number of iterations in *len */
+int bmg_batch_trace(bmg_batch* b, int chain, uint8_t* move, uint8_t* jd, uint8_t* ms,
+                    uint32_t* size, double* loglik, int64_t cap, int64_t* len);
+/* thinned trace (src/sampler.cpp:691-728): model sizes, concatenated loci,
+ * loglik, sigma2, alpha */
+int bmg_batch_thinned(bmg_batch* b, int chain, uint32_t* sizes, int64_t cap_sizes,
+                      uint32_t* loci, int64_t cap_loci, double* loglik, double* sigma2,
+                      double* alpha, int64_t* n_samples, int64_t* n_loci);
+/* RB scan of one chain's current state with a given y_hat (the drop-in for
+ * RaoBlackwellizer::p_raoblackwell, src/sampler.cpp:32-261) */
+int bmg_batch_rb(bmg_batch* b, int chain, const double* y_hat, double* p_r);
+/* event timing of the last bmg_batch_run: total ms, ms in chain kernels,
+ * ms in RB scans, number of scan launches */
+int bmg_batch_timing(bmg_batch* b, double out[4]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BMAGWA_HIP_H_ */

@@ -1,0 +1,95 @@
+"""The batched PMV sampler on the GPU against the oracle under the same
+random stream: identical moves, jump distances, move sizes and model sizes
+per iteration, log-likelihoods within 1e-10 relative; several chains of one
+batch each reproduce their own oracle run."""
+import numpy as np
+import pytest
+
+import oracle
+from bmagwa_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def bmg():
+    import bmagwa_amd
+    bmagwa_amd.lib()
+    return bmagwa_amd
+
+
+def dataset(n, p, ld, n_causal, seed, types="A"):
+    thresh, rho, _ = synth.snp_layout(p, (0.05, 0.5), ld, seed)
+    body = oracle.synth_genotypes(seed, n, p, thresh, rho)
+    dec = lambda snps: np.stack([oracle.decode(body, n, int(j), 0) for j in snps])
+    y, causal, beta = synth.phenotype(dec, n, p, n_causal, 0.5, seed, dominance=(types == "AH"))
+    return body, y, causal
+
+
+def oracle_run(cfg, body, y, n_iter):
+    d = oracle.Data(cfg["n"], cfg["m_g"], body, y)
+    s = oracle.Sampler(cfg, d)
+    s.init_flat()
+    assert s.setup() == 0
+    s.iterate(n_iter)
+    return s
+
+
+def compare(tr_g, tr_o, n_iter):
+    assert len(tr_g["move"]) == n_iter
+    first_bad = None
+    for key in ("move", "jd", "ms", "size"):
+        neq = np.flatnonzero(tr_g[key] != tr_o[key])
+        if len(neq):
+            first_bad = neq[0] if first_bad is None else min(first_bad, neq[0])
+    assert first_bad is None, "diverged at iteration %d" % first_bad
+    ll_o, ll_g = tr_o["loglik"], tr_g["loglik"]
+    assert np.allclose(ll_g, ll_o, rtol=1e-10, atol=0), np.max(np.abs(ll_g - ll_o) / np.abs(ll_o))
+
+
+@pytest.mark.parametrize("types,ld,use_ind", [("A", (0.9, 0.9, 10, 30), 1), ("A", None, 0),
+                                              ("AH", (0.8, 0.9, 10, 20), 1)])
+def test_trajectory_parity_single_chain(bmg, types, ld, use_ind):
+    n, p, n_iter = 300, 400, 3000
+    body, y, causal = dataset(n, p, ld, 5, 1234, types)
+    cfg = synth.sampler_cfg(n, p, 5, types=types, seed=77, n_rao=100, n_rao_burnin=5,
+                            thin=10, n_sample_tau2_and_missing=10, max_move_size=10,
+                            use_individual_tau2=use_ind, do_n_iter=n_iter)
+    s = oracle_run(cfg, body, y, n_iter)
+    ctx = bmg.Ctx(n, p, body, y, types=(types,))
+    b = bmg.Batch([ctx], [0], [cfg], dict(trace_every_iter=1))
+    b.setup()
+    b.run(n_iter)
+    compare(b.trace(0), s.trace(), n_iter)
+    assert np.allclose(b.p_rao(0), s.p_rao(), rtol=1e-8, atol=1e-10)
+    assert np.array_equal(np.sort(b.loci(0)), np.sort(s.loci()))
+
+
+def test_batch_of_chains_each_match_their_oracle_run(bmg):
+    n, p, n_iter = 256, 300, 1500
+    body, y, _ = dataset(n, p, (0.9, 0.9, 10, 30), 4, 99)
+    cfgs = [synth.sampler_cfg(n, p, 4, seed=1000 + c, chain=c, n_rao=100, n_rao_burnin=5,
+                              thin=10, max_move_size=8, do_n_iter=n_iter) for c in range(4)]
+    ctx = bmg.Ctx(n, p, body, y)
+    b = bmg.Batch([ctx], [0] * 4, cfgs, dict(trace_every_iter=1))
+    b.setup()
+    b.run(n_iter // 3)
+    b.run(n_iter - n_iter // 3)       # runs resume exactly
+    for c in range(4):
+        s = oracle_run(cfgs[c], body, y, n_iter)
+        compare(b.trace(c), s.trace(), n_iter)
+
+
+def test_rb_scan_matches_oracle(bmg):
+    n, p = 300, 350
+    body, y, _ = dataset(n, p, (0.9, 0.9, 10, 30), 5, 5)
+    cfg = synth.sampler_cfg(n, p, 5, seed=3, n_rao=50, thin=10, do_n_iter=500)
+    s = oracle_run(cfg, body, y, 200)
+    ctx = bmg.Ctx(n, p, body, y)
+    b = bmg.Batch([ctx], [0], [cfg], dict(trace_every_iter=1))
+    b.setup()
+    b.run(200)
+    yh = np.random.default_rng(1).normal(size=n) * 0.1
+    pg = b.rb(0, yh)
+    po = s.rb(yh)
+    assert np.allclose(pg, po, rtol=1e-9, atol=1e-12)
