@@ -122,6 +122,7 @@ SIGNATURES = {
     "bmg_scan_xtr": (C.c_int, _P, _D, C.c_int, C.c_int, C.c_int, _D),
     "bmg_scan_xtr_dev": (C.c_int, _P, _P, C.c_int, C.c_int, C.c_int, _P, _P),
     "bmg_ctx_n_pad": (C.c_int, _P, _U32),
+    "bmg_scan_xtr_timed": (C.c_int, _P, _P, C.c_int, C.c_int, C.c_int, _P, C.c_int, _D, _P),
     "bmg_model_kmax": (C.c_int,),
     "bmg_model_create": (C.c_int, _P, _D, _PP),
     "bmg_model_destroy": (C.c_int, _P),
@@ -241,6 +242,15 @@ class Ctx:
         out = np.zeros((r.shape[0], self.p))
         check(lib().bmg_scan_xtr(self.h, _p(r), r.shape[0], coding, digits, _p(out)))
         return out
+
+    def scan_timed(self, r_dev, nvec, out_dev, coding=0, digits=5, reps=20, stream=None):
+        """Mean duration (ms) of the MFMA scan kernel over `reps` launches on
+        device buffers r_dev (nvec x n_pad doubles) -> out_dev (nvec x p)."""
+        ms = C.c_double()
+        check(lib().bmg_scan_xtr_timed(self.h, C.c_void_p(r_dev), nvec, coding, digits,
+                                       C.c_void_p(out_dev), reps, C.byref(ms),
+                                       C.c_void_p(stream) if stream else None))
+        return ms.value
 
     def n_pad(self):
         v = C.c_uint32()

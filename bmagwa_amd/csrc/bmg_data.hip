@@ -431,8 +431,7 @@ int launch_synth(bmg_ctx* c, uint32_t seed, const double* thresh_dev, const doub
   return BMG_OK;
 }
 
-int launch_scan(bmg_ctx* c, const double* r_dev, int nvec, int coding, int digits,
-                double* out_dev, hipStream_t s) {
+static int scan_prepare(bmg_ctx* c, const double* r_dev, int nvec, int digits, hipStream_t s) {
   if (nvec < 1 || digits < 1 || digits > 6 || nvec * digits > 16) {
     set_error("scan: need 1 <= nvec*digits <= 16, digits <= 6");
     return BMG_E_ARG;
@@ -445,19 +444,76 @@ int launch_scan(bmg_ctx* c, const double* r_dev, int nvec, int coding, int digit
   hipLaunchKernelGGL(k_digits, dim3((c->n_pad + 255) / 256, nvec), dim3(256), 0, s, r_dev,
                      c->n_pad, nvec, digits, c->scale, c->dig);
   BMG_HIP(hipGetLastError());
+  return BMG_OK;
+}
+
+struct ScanShape {
+  uint32_t tile_words, n_tiles;
+  unsigned gx;
+  int atomic;
+  size_t lds;
+};
+
+static ScanShape scan_shape(const bmg_ctx* c, int NC) {
+  ScanShape sh;
   // individual tiles of at most 64 KiB of digits in LDS
   uint32_t tile_words = (uint32_t)std::max<size_t>(16, (65536 / NC) / 16 / 16 * 16);
   tile_words = std::min<uint32_t>(tile_words, (c->W32 + 15) / 16 * 16);
-  uint32_t n_tiles = (c->W32 + tile_words - 1) / tile_words;
+  sh.tile_words = tile_words;
+  sh.n_tiles = (c->W32 + tile_words - 1) / tile_words;
   const int NT = 2;
   uint64_t per_block = 4ull * NT * 16;
-  unsigned gx = (unsigned)((c->p + per_block - 1) / per_block);
-  int atomic = n_tiles > 1;
-  if (atomic) BMG_HIP(hipMemsetAsync(c->acc, 0, (size_t)c->p * 16 * sizeof(int32_t), s));
-  size_t lds = (size_t)NC * tile_words * 16;
-  hipLaunchKernelGGL(k_scan_mfma<NT>, dim3(gx, n_tiles), dim3(256), lds, s, c->g, c->p, c->W32,
-                     coding, c->dig, NC, c->n_pad, tile_words, c->acc, atomic);
+  sh.gx = (unsigned)((c->p + per_block - 1) / per_block);
+  sh.atomic = sh.n_tiles > 1;
+  sh.lds = (size_t)NC * tile_words * 16;
+  return sh;
+}
+
+static int scan_kernel(bmg_ctx* c, int nvec, int coding, int digits, const ScanShape& sh, hipStream_t s) {
+  int NC = nvec * digits;
+  if (sh.atomic) BMG_HIP(hipMemsetAsync(c->acc, 0, (size_t)c->p * 16 * sizeof(int32_t), s));
+  hipLaunchKernelGGL(k_scan_mfma<2>, dim3(sh.gx, sh.n_tiles), dim3(256), sh.lds, s, c->g, c->p, c->W32,
+                     coding, c->dig, NC, c->n_pad, sh.tile_words, c->acc, sh.atomic);
   BMG_HIP(hipGetLastError());
+  return BMG_OK;
+}
+
+int launch_scan(bmg_ctx* c, const double* r_dev, int nvec, int coding, int digits,
+                double* out_dev, hipStream_t s) {
+  int rc = scan_prepare(c, r_dev, nvec, digits, s);
+  if (rc) return rc;
+  ScanShape sh = scan_shape(c, nvec * digits);
+  rc = scan_kernel(c, nvec, coding, digits, sh, s);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_scan_combine, dim3((unsigned)((c->p + 255) / 256), nvec), dim3(256), 0, s,
+                     c->acc, c->p, nvec, digits, c->scale, out_dev);
+  BMG_HIP(hipGetLastError());
+  return BMG_OK;
+}
+
+int launch_scan_timed(bmg_ctx* c, const double* r_dev, int nvec, int coding, int digits,
+                      double* out_dev, int reps, double* ms, hipStream_t s) {
+  int rc = scan_prepare(c, r_dev, nvec, digits, s);
+  if (rc) return rc;
+  ScanShape sh = scan_shape(c, nvec * digits);
+  if (sh.atomic) { set_error("timed scan needs one individual tile (n_pad * NC <= 64 KiB per tile)"); }
+  hipEvent_t e0, e1;
+  BMG_HIP(hipEventCreate(&e0));
+  BMG_HIP(hipEventCreate(&e1));
+  double total = 0.0;
+  for (int r = 0; r < reps && !rc; ++r) {
+    if (sh.atomic) BMG_HIP(hipMemsetAsync(c->acc, 0, (size_t)c->p * 16 * sizeof(int32_t), s));
+    BMG_HIP(hipEventRecord(e0, s));
+    hipLaunchKernelGGL(k_scan_mfma<2>, dim3(sh.gx, sh.n_tiles), dim3(256), sh.lds, s, c->g, c->p, c->W32,
+                       coding, c->dig, nvec * digits, c->n_pad, sh.tile_words, c->acc, sh.atomic);
+    BMG_HIP(hipEventRecord(e1, s));
+    BMG_HIP(hipEventSynchronize(e1));
+    float t = 0;
+    BMG_HIP(hipEventElapsedTime(&t, e0, e1));
+    total += t;
+  }
+  hipEventDestroy(e0); hipEventDestroy(e1);
+  *ms = total / (reps > 0 ? reps : 1);
   hipLaunchKernelGGL(k_scan_combine, dim3((unsigned)((c->p + 255) / 256), nvec), dim3(256), 0, s,
                      c->acc, c->p, nvec, digits, c->scale, out_dev);
   BMG_HIP(hipGetLastError());

@@ -122,6 +122,11 @@ int bmg_scan_xtr(bmg_ctx* ctx, const double* r, int nvec, int coding, int digits
 int bmg_scan_xtr_dev(bmg_ctx* ctx, const double* r_dev, int nvec, int coding,
                      int digits, double* out_dev, void* stream);
 int bmg_ctx_n_pad(bmg_ctx* ctx, uint32_t* n_pad);
+/* Timing form of bmg_scan_xtr_dev: prepares the digit planes once, then
+ * launches the int8-MFMA scan kernel `reps` times on `stream`, bracketing
+ * each launch with HIP events; *ms_per_launch is the mean kernel time. */
+int bmg_scan_xtr_timed(bmg_ctx* ctx, const double* r_dev, int nvec, int coding, int digits,
+                       double* out_dev, int reps, double* ms_per_launch, void* stream);
 
 /* -------------------------------------------------------------- model --- */
 /* Prior inputs of Model/Prior (src/prior.hpp:45-122), already solved. */

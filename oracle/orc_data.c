@@ -19,6 +19,35 @@ static const double CODE_VALUE[4][4] = {
   /* R */ {0.0, -1.0, 0.0, 1.0},
 };
 
+/* per byte: number of fields with code 10 (value 1), 11 (value 2) and 01
+ * (missing); used to take exact integer sums a byte at a time */
+static unsigned char BYTE_C1[256], BYTE_C2[256], BYTE_CM[256];
+static int byte_tables_ready = 0;
+static void byte_tables(void)
+{
+  if (byte_tables_ready) return;
+  for (int b = 0; b < 256; ++b) {
+    int c1 = 0, c2 = 0, cm = 0;
+    for (int f = 0; f < 4; ++f) {
+      int c = (b >> (2 * f)) & 3;
+      c1 += c == 2; c2 += c == 3; cm += c == 1;
+    }
+    BYTE_C1[b] = (unsigned char)c1; BYTE_C2[b] = (unsigned char)c2; BYTE_CM[b] = (unsigned char)cm;
+  }
+  byte_tables_ready = 1;
+}
+
+/* counts of codes 10, 11, 01 of one SNP (padding fields are 00) */
+static void snp_counts(const uint8_t* g, size_t stride, long long* c1, long long* c2,
+                       long long* cm)
+{
+  long long a = 0, b = 0, m = 0;
+  for (size_t k = 0; k < stride; ++k) {
+    a += BYTE_C1[g[k]]; b += BYTE_C2[g[k]]; m += BYTE_CM[g[k]];
+  }
+  *c1 = a; *c2 = b; *cm = m;
+}
+
 /* src/data.cpp:40-54 */
 double orc_get_genotype(const uint8_t* body, size_t stride, size_t ind,
                         size_t snp)
@@ -100,12 +129,13 @@ orc_data* orc_data_create(uint32_t n, uint64_t m_g, uint32_t m_e,
 
   if (recode) {
     /* src/data.cpp:324-339 */
-    double* tmp = (double*)malloc(sizeof(double) * n);
+    byte_tables();
     for (uint64_t snp = 0; snp < m_g; ++snp) {
-      orc_get_genotypes(d->g, d->stride, n, ORC_A, snp, tmp);
-      double mac = 0; size_t ng = 0;
-      for (uint32_t i = 0; i < n; ++i)
-        if (tmp[i] >= 0) { mac += tmp[i]; ++ng; }
+      /* allele count and non-missing count are exact integers (the
+       * reference sums them as doubles, src/utils.cpp:33-44) */
+      long long c1, c2, cm;
+      snp_counts(d->g + d->stride * snp, d->stride, &c1, &c2, &cm);
+      double mac = (double)(c1 + 2 * c2); size_t ng = n - (size_t)cm;
       if (mac / (2.0 * ng) > 0.5) {
         for (uint32_t i = 0; i < n; ++i) {
           double gv = orc_get_genotype(d->g, d->stride, i, snp);
@@ -114,16 +144,16 @@ orc_data* orc_data_create(uint32_t n, uint64_t m_g, uint32_t m_e,
         }
       }
     }
-    free(tmp);
   }
 
   /* src/data.cpp:341-376 */
+  byte_tables();
   d->miss_loc = (size_t**)calloc(m_g, sizeof(size_t*));
   d->miss_prior = (double**)calloc(m_g, sizeof(double*));
   for (uint64_t snp = 0; snp < m_g; ++snp) {
-    int n_miss = 0;
-    for (uint32_t i = 0; i < n; ++i)
-      if (orc_get_genotype(d->g, d->stride, i, snp) < 0) ++n_miss;
+    long long c1, c2, cm;
+    snp_counts(d->g + d->stride * snp, d->stride, &c1, &c2, &cm);
+    int n_miss = (int)cm;
     if (n_miss > 0) {
       d->miss_loc[snp] = (size_t*)malloc(sizeof(size_t) * (n_miss + 1));
       d->miss_prior[snp] = (double*)calloc(3, sizeof(double));
@@ -145,11 +175,11 @@ orc_data* orc_data_create(uint32_t n, uint64_t m_g, uint32_t m_e,
   d->var_y = vec_var(d->y, n);
   double tv = 0, tm = 0; size_t nsm = 0, nsv = 0;
   for (uint64_t snp = 0; snp < m_g; ++snp) {
-    double sq = 0, s = 0; size_t ng = 0;
-    for (uint32_t i = 0; i < n; ++i) {
-      double gv = orc_get_genotype(d->g, d->stride, i, snp);
-      if (gv >= 0) { sq += gv * gv; s += gv; ++ng; }
-    }
+    /* the per-SNP sums are exact integers */
+    long long c1, c2, cm;
+    snp_counts(d->g + d->stride * snp, d->stride, &c1, &c2, &cm);
+    double sq = (double)(c1 + 4 * c2), s = (double)(c1 + 2 * c2);
+    size_t ng = n - (size_t)cm;
     if (ng > 1) {
       tm += s / ng; tv += (sq - s * s / ng) / (ng - 1); ++nsm; ++nsv;
     } else if (ng == 1) {
@@ -327,7 +357,27 @@ void orc_self_moments(const orc_datamodel* dm, double* out)
   double* x = (double*)malloc(sizeof(double) * n);
   double* xah = (double*)malloc(sizeof(double) * n);
   double* xxloc = out;
+  byte_tables();
   for (uint64_t i = 0; i < d->m_g; ++i) {
+    if (!d->miss_loc[i]) {
+      /* no missing value: the sums of the loop below are exact integer
+       * functions of the code counts, taken a byte at a time */
+      long long c1, c2, cm;
+      snp_counts(d->g + d->stride * i, d->stride, &c1, &c2, &cm);
+      const long long sum[4] = {c1 + 2 * c2, c1, c1 + c2, c2};
+      const long long sq[4] = {c1 + 4 * c2, c1, c1 + c2, c2};
+      for (int t = 0; t < 4; ++t) {
+        if (!dm->allow_terms[t]) continue;
+        double xsum = (double)sum[t];
+        *xxloc++ = xsum;
+        *xxloc++ = (double)sq[t] - xsum * xsum / (double)n;
+      }
+      if (dm->allow_types[ORC_AH]) {
+        double as = (double)sum[0], hs = (double)sum[1];
+        *xxloc++ = (double)c1 - as * hs / (double)n;
+      }
+      continue;
+    }
     for (int t = 0; t < 4; ++t) {
       if (!dm->allow_terms[t]) continue;
       orc_datamodel_get(dm, t, i, x);
