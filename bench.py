@@ -1,0 +1,289 @@
+#!/usr/bin/env python3
+"""PIP ESS/s and MCMC iterations/s of the device PMV sampler, batched vs plain.
+
+Workload: n=10,000 individuals, p=500,000 SNPs, AR(1) LD blocks (rho ~
+U(0.7, 0.95), block length U{20..100}), MAF ~ U(0.02, 0.5), 100 causal SNPs,
+h2 = 0.5, additive coding, generated on the GPU from a fixed seed.
+
+A "step" is one `Batch.run(iters_per_step)`: every chain of the batch does
+`iters_per_step` MCMC iterations (moves, tau2/alpha refreshes, thinning and
+the Rao-Blackwellised all-SNP scans of its schedule).  Two batches run
+back to back on the same dataset:
+
+* plain:   1 chain (the reference's one-chain-per-thread sampler on the GPU);
+* batched: `--chains` chains (16) advanced together in one kernel.
+
+PIP ESS (median over the monitored SNPs: the causal SNPs plus those with a
+pooled PIP in [0.05, 0.95]) is Geyer's initial-monotone-sequence ESS of the
+thinned inclusion indicators of the timed steps, summed over chains, per
+wall second of the timed steps.  `value` is the batched figure.
+
+`roofline` covers the all-SNP X^T r scan kernel (HBM-bound) at the launch
+shape the batch uses (3 residual vectors x 5 digit planes), timed with HIP
+events on the stream it is launched on.  `cpu_baseline` is the CPU oracle's
+plain sampler (one chain, one core) on the same dataset and settings, timed
+over a bounded sample after an untimed warm part.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+N, P, N_CAUSAL, H2 = 10000, 500000, 100, 0.5
+MAF, LD = (0.02, 0.5), (0.7, 0.95, 20, 100)
+SEED = 20261202 + 4000
+HBM_PEAK_GBS = 8000.0
+INT8_PEAK_TOPS = 5000.0
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def indicator_rows(th, snps, first):
+    """Thinned samples first.. of one chain -> bool (len(snps), T)."""
+    from bmagwa_amd import ess
+    return ess.indicator_matrix(th["sizes"], th["loci"], snps, burn=first)
+
+
+def pooled_counts(th, first, p):
+    sizes = th["sizes"].astype(np.int64)
+    off = int(sizes[:first].sum())
+    return np.bincount(th["loci"][off:].astype(np.int64), minlength=p), len(sizes) - first
+
+
+def run_batch(bmg, synth, ctx, n_chains, steps, warmup, ips, seed0, batch_cfg=None):
+    cfgs = [synth.sampler_cfg(N, P, N_CAUSAL, types="A", seed=seed0 + c, chain=c,
+                              do_n_iter=10 ** 8, n_rao_burnin=4)
+            for c in range(n_chains)]
+    t0 = time.perf_counter()
+    b = bmg.Batch([ctx], [0] * n_chains, cfgs, batch_cfg)
+    b.setup()
+    t_setup = time.perf_counter() - t0
+    for w in range(warmup):
+        b.run(ips)
+        log("%d chains warmup step %d/%d size %d" % (n_chains, w + 1, warmup, b.state(0)["size"]))
+    first = [len(b.thinned(c)["sizes"]) for c in range(n_chains)]
+    tm0 = b.timing()
+    b.sync()
+    t0 = time.perf_counter()
+    for s in range(steps):
+        b.run(ips)
+    b.sync()
+    wall = time.perf_counter() - t0
+    tm1 = b.timing()
+    log("%d chains: %d timed steps in %.2f s" % (n_chains, steps, wall))
+    ths = [b.thinned(c) for c in range(n_chains)]
+    sizes = [b.state(c)["size"] for c in range(n_chains)]
+    acc = [b.state(c)["n_accepted"] / max(b.state(c)["n_iter"], 1) for c in range(n_chains)]
+    b.close()
+    return dict(wall=wall, setup=t_setup, ths=ths, first=first, sizes=sizes, acc=acc,
+                scan_ms=tm1["scan_ms"] - tm0["scan_ms"], chain_ms=tm1["chain_ms"] - tm0["chain_ms"],
+                n_scans=tm1["n_scans"] - tm0["n_scans"])
+
+
+def ess_of(run, monitored, wall):
+    from bmagwa_amd import ess
+    tot = np.zeros(len(monitored))
+    for th, f in zip(run["ths"], run["first"]):
+        M = indicator_rows(th, monitored, f)
+        if M.shape[1] >= 4:
+            tot += ess.ess_geyer(M)
+    return float(np.median(tot)) / wall, float(np.min(tot)) / wall
+
+
+def scan_roofline(bmg, ctx):
+    """k_scan_mfma at the batch's launch shape, HIP events on its stream."""
+    n_pad = ctx.n_pad()
+    nvec, digits = 3, 5
+    rng = np.random.default_rng(1)
+    r = np.zeros((nvec, n_pad))
+    r[:, :N] = rng.normal(size=(nvec, N))
+    r_dev = bmg.DevBuf(r.nbytes)
+    r_dev.upload(r)
+    out_dev = bmg.DevBuf(nvec * P * 8)
+    ms = ctx.scan_timed(r_dev.ptr, nvec, out_dev.ptr, 0, digits, 30)
+    r_dev.free()
+    out_dev.free()
+    bytes_per_launch = P * ((N + 3) // 4) + nvec * (N * 8 + P * 8)
+    gbs = bytes_per_launch / (ms * 1e-3) / 1e9
+    return dict(ms=ms, bytes=bytes_per_launch, gbs=gbs, nvec=nvec, digits=digits)
+
+
+def band_mfma(bmg, ctx, W=500):
+    """k_band (int8 MFMA, exact int32) over all p SNPs at window W."""
+    out = bmg.DevBuf(P * W * 4)
+    ctx.band_timed(W, 0, 0, out.ptr, reps=1)  # first launch
+    ms = ctx.band_timed(W, 0, 0, out.ptr, reps=3)
+    out.free()
+    ops = 2.0 * N * (P * W - W * (W - 1) / 2)  # 2n * sum_j min(W, p - j)
+    return dict(ms=ms, W=W, tops=ops / (ms * 1e-3) / 1e12)
+
+
+def copy_calibration(bmg, nbytes=1 << 30):
+    """A plain copy of known traffic: scales the profiler's HBM counters."""
+    a, b = bmg.DevBuf(nbytes), bmg.DevBuf(nbytes)
+    ms = bmg.copy_timed(a, b, nbytes, reps=5)
+    a.free()
+    b.free()
+    return dict(bytes=2 * nbytes, ms=ms, gbs=2 * nbytes / (ms * 1e-3) / 1e9)
+
+
+def cpu_baseline(ctx, y, monitored, seconds):
+    import oracle
+    oracle.load()
+    body = ctx.read_body()
+    t0 = time.perf_counter()
+    d = oracle.Data(N, P, body, y)
+    del body
+    from bmagwa_amd import synth
+    cfg = synth.sampler_cfg(N, P, N_CAUSAL, types="A", seed=9001, chain=0, do_n_iter=10 ** 8,
+                            n_rao_burnin=4)
+    s = oracle.Sampler(cfg, d)
+    s.init_flat()
+    s.setup()
+    t_setup = time.perf_counter() - t0
+    log("cpu oracle setup %.1f s" % t_setup)
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds / 3:
+        s.iterate(100)
+    first = len(s.thinned()["sizes"])
+    it0 = s.state()["n_iter"]
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        s.iterate(100)
+    wall = time.perf_counter() - t0
+    its = s.state()["n_iter"] - it0
+    from bmagwa_amd import ess
+    th = s.thinned()
+    M = ess.indicator_matrix(th["sizes"], th["loci"], monitored, burn=first)
+    e = ess.ess_geyer(M) if M.shape[1] >= 4 else np.zeros(len(monitored))
+    log("cpu oracle: %d iterations in %.1f s" % (its, wall))
+    return dict(value=float(np.median(e)) / wall, iters_per_s=its / wall, iters=int(its),
+                wall=wall, samples=int(M.shape[1]), setup_s=t_setup)
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--iters-per-step", type=int, default=1000)
+    ap.add_argument("--chains", type=int, default=16)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the band and scan timings")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="time only the scan, band and copy kernels (for profiler runs)")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU only: chains and phenotypes are independent, so more "
+              "GPUs would each run their own share of the chains; run with --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import bmagwa_amd as bmg
+    from bmagwa_amd import synth
+
+    t0 = time.perf_counter()
+    thresh, rho, _ = synth.snp_layout(P, MAF, LD, SEED)
+    ctx = bmg.Ctx(N, P, synth=(SEED, thresh, rho), types=("A",))
+    y, causal, _ = synth.phenotype(lambda s: ctx.decode(np.asarray(s), 0), N, P, N_CAUSAL, H2, SEED)
+    ctx.set_y(y)
+    t_data = time.perf_counter() - t0
+    log("dataset on device in %.1f s" % t_data)
+
+    if args.kernels_only:
+        sc = scan_roofline(bmg, ctx)
+        bd = band_mfma(bmg, ctx)
+        cp = copy_calibration(bmg)
+        print(json.dumps(dict(scan=sc, band=bd, copy=cp)), flush=True)
+        return
+
+    ips = args.iters_per_step
+    bat = run_batch(bmg, synth, ctx, args.chains, args.steps, args.warmup, ips, 100)
+    pl = None if args.no_plain else run_batch(bmg, synth, ctx, 1, args.steps, args.warmup, ips, 100)
+
+    counts = np.zeros(P, np.int64)
+    total = 0
+    for run in [bat] + ([pl] if pl else []):
+        for th, f in zip(run["ths"], run["first"]):
+            c, t = pooled_counts(th, f, P)
+            counts += c
+            total += t
+    pip = counts / max(total, 1)
+    monitored = np.union1d(causal, np.nonzero((pip >= 0.05) & (pip <= 0.95))[0])
+    b_med, b_min = ess_of(bat, monitored, bat["wall"])
+    res_pl = None
+    if pl:
+        p_med, p_min = ess_of(pl, monitored, pl["wall"])
+        res_pl = dict(ess_per_s_median=p_med, ess_per_s_min=p_min,
+                      iters_per_s=ips * args.steps / pl["wall"],
+                      ms_per_step=pl["wall"] / args.steps * 1e3, model_size=pl["sizes"][0])
+
+    roof = None
+    extra = {}
+    if not args.no_extra:
+        sc = scan_roofline(bmg, ctx)
+        roof = dict(bound="hbm", achieved=round(sc["gbs"], 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(sc["gbs"] / HBM_PEAK_GBS, 4), traffic=None,
+                    kernel="k_scan_mfma<2> (3 vectors x 5 digit planes)",
+                    ms_per_launch=round(sc["ms"], 4), algorithmic_bytes=sc["bytes"])
+        bd = band_mfma(bmg, ctx)
+        extra["band_xtx_int8"] = dict(W=bd["W"], ms=round(bd["ms"], 3), tops=round(bd["tops"], 1),
+                                      frac_of_int8_peak=round(bd["tops"] / INT8_PEAK_TOPS, 4))
+        cp = copy_calibration(bmg)
+        extra["copy_calibration"] = dict(bytes=cp["bytes"], ms=round(cp["ms"], 4),
+                                         gbs=round(cp["gbs"], 1))
+
+    cpu = None
+    if not args.no_cpu:
+        c = cpu_baseline(ctx, y, monitored, args.cpu_seconds)
+        cpu = dict(value=c["value"], unit="PIP ESS/s (median)", cores=1, kind="port",
+                   iters_per_s=round(c["iters_per_s"], 1),
+                   sample="oracle plain PMV, 1 chain on 1 core, same dataset and settings: "
+                          "%d iterations (%d thinned samples) timed over %.1f s after an untimed "
+                          "warm part; setup %.1f s excluded" % (c["iters"], c["samples"], c["wall"],
+                                                               c["setup_s"]))
+
+    wall = bat["wall"]
+    line = {
+        "metric": "PIP ESS/sec + MCMC iters/sec, learned+batched vs plain, 1 MI355X",
+        "value": round(b_med, 3),
+        "unit": "PIP ESS/s (median over monitored SNPs, summed over chains)",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (generated on device, seeded)",
+        "config": {"workload": "PMV MCMC over SNP inclusion, n=10000 p=500000 AR(1) LD, "
+                               "100 causal, h2=0.5, additive",
+                   "chains": args.chains, "iters_per_step": ips, "proposal": "plain"},
+        "batched": dict(ess_per_s_median=round(b_med, 3), ess_per_s_min=round(b_min, 3),
+                        iters_per_s=round(ips * args.steps * args.chains / wall, 1),
+                        iters_per_s_per_chain=round(ips * args.steps / wall, 1),
+                        model_size_mean=float(np.mean(bat["sizes"])),
+                        accept_rate_mean=round(float(np.mean(bat["acc"])), 4),
+                        scan_ms=round(bat["scan_ms"], 1), chain_ms=round(bat["chain_ms"], 1),
+                        n_scans=bat["n_scans"]),
+        "plain": res_pl,
+        "batched_vs_plain_ess": (round(b_med / res_pl["ess_per_s_median"], 2)
+                                 if res_pl and res_pl["ess_per_s_median"] > 0 else None),
+        "monitored_snps": int(len(monitored)),
+        "setup_s": dict(dataset=round(t_data, 2), batch=round(bat["setup"], 2)),
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    line.update(extra)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -123,6 +123,11 @@ SIGNATURES = {
     "bmg_scan_xtr_dev": (C.c_int, _P, _P, C.c_int, C.c_int, C.c_int, _P, _P),
     "bmg_ctx_n_pad": (C.c_int, _P, _U32),
     "bmg_scan_xtr_timed": (C.c_int, _P, _P, C.c_int, C.c_int, C.c_int, _P, C.c_int, _D, _P),
+    "bmg_band_xtx_timed": (C.c_int, _P, C.c_uint32, C.c_int, C.c_int, _P, C.c_int, _D, _P),
+    "bmg_dev_alloc": (C.c_int, C.c_int, C.c_uint64, _PP),
+    "bmg_dev_free": (C.c_int, C.c_int, _P),
+    "bmg_memcpy_h2d": (C.c_int, C.c_int, _P, _P, C.c_uint64),
+    "bmg_copy_timed": (C.c_int, C.c_int, _P, _P, C.c_uint64, C.c_int, _D),
     "bmg_model_kmax": (C.c_int,),
     "bmg_model_create": (C.c_int, _P, _D, _PP),
     "bmg_model_destroy": (C.c_int, _P),
@@ -237,6 +242,18 @@ class Ctx:
         check(lib().bmg_band_xtx(self.h, W, ca, cb, _p(out, C.c_int32)))
         return out.reshape(self.p, W)
 
+    def band_timed(self, W, ca, cb, out_dev, reps=3, stream=None):
+        """Mean duration (ms) of the band kernel over `reps` launches."""
+        ms = C.c_double()
+        check(lib().bmg_band_xtx_timed(self.h, W, ca, cb, C.c_void_p(out_dev), reps, C.byref(ms),
+                                       C.c_void_p(stream) if stream else None))
+        return ms.value
+
+    def band_xtx_dev(self, W, ca, cb, out_dev, stream=None):
+        """Band X^T X into a device buffer (p x W int32) on `stream`."""
+        check(lib().bmg_band_xtx_dev(self.h, W, ca, cb, C.c_void_p(out_dev),
+                                     C.c_void_p(stream) if stream else None))
+
     def scan_xtr(self, r, coding=0, digits=5):
         r = np.ascontiguousarray(np.atleast_2d(r), dtype=np.float64)
         out = np.zeros((r.shape[0], self.p))
@@ -267,6 +284,42 @@ class Ctx:
             self.close()
         except Exception:
             pass
+
+
+class DevBuf:
+    """A device allocation made through the engine's own runtime."""
+
+    def __init__(self, nbytes, device=0):
+        self.device, self.nbytes = device, int(nbytes)
+        h = C.c_void_p()
+        check(lib().bmg_dev_alloc(device, self.nbytes, C.byref(h)))
+        self.ptr = h.value
+
+    def upload(self, arr):
+        a = np.ascontiguousarray(arr)
+        if a.nbytes > self.nbytes:
+            raise ValueError("upload larger than the buffer")
+        check(lib().bmg_memcpy_h2d(self.device, C.c_void_p(self.ptr), a.ctypes.data_as(C.c_void_p),
+                                   a.nbytes))
+
+    def free(self):
+        if self.ptr:
+            lib().bmg_dev_free(self.device, C.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def copy_timed(src, dst, nbytes, reps=10, device=0):
+    """Mean duration (ms) of a plain device copy of nbytes."""
+    ms = C.c_double()
+    check(lib().bmg_copy_timed(device, C.c_void_p(src.ptr), C.c_void_p(dst.ptr), nbytes, reps,
+                               C.byref(ms)))
+    return ms.value
 
 
 class Model:
@@ -355,6 +408,9 @@ class Batch:
 
     def run(self, n_iter):
         check(lib().bmg_batch_run(self.h, n_iter))
+
+    def sync(self):
+        check(lib().bmg_batch_sync(self.h))
 
     def state(self, c):
         o = np.zeros(8)

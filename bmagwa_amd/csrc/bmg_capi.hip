@@ -263,6 +263,49 @@ extern "C" int bmg_scan_xtr_timed(bmg_ctx* c, const double* r_dev, int nvec, int
   return launch_scan_timed(c, r_dev, nvec, coding, digits, out_dev, reps, ms_per_launch, s);
 }
 
+extern "C" int bmg_band_xtx_timed(bmg_ctx* c, uint32_t W, int ca, int cb, int32_t* out_dev, int reps,
+                                  double* ms_per_launch, void* stream) {
+  if (!c || !out_dev || !ms_per_launch || ca < 0 || ca > 3 || cb < 0 || cb > 3 || reps < 1) {
+    set_error("bad argument"); return BMG_E_ARG;
+  }
+  BMG_HIP(hipSetDevice(c->device));
+  hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  return launch_band_timed(c, W, ca, cb, out_dev, reps, ms_per_launch, s);
+}
+
+extern "C" int bmg_dev_alloc(int device, uint64_t bytes, void** ptr) {
+  if (!ptr || bytes == 0) { set_error("bad argument"); return BMG_E_ARG; }
+  *ptr = nullptr;
+  BMG_HIP(hipSetDevice(device));
+  BMG_HIP(hipMalloc(ptr, bytes));
+  return BMG_OK;
+}
+
+extern "C" int bmg_dev_free(int device, void* ptr) {
+  BMG_HIP(hipSetDevice(device));
+  if (ptr) BMG_HIP(hipFree(ptr));
+  return BMG_OK;
+}
+
+extern "C" int bmg_memcpy_h2d(int device, void* dst, const void* src, uint64_t bytes) {
+  if (!dst || !src) { set_error("null argument"); return BMG_E_ARG; }
+  BMG_HIP(hipSetDevice(device));
+  BMG_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return BMG_OK;
+}
+
+extern "C" int bmg_copy_timed(int device, const void* src_dev, void* dst_dev, uint64_t bytes, int reps,
+                              double* ms_per_launch) {
+  if (!src_dev || !dst_dev || !ms_per_launch || reps < 1) { set_error("bad argument"); return BMG_E_ARG; }
+  BMG_HIP(hipSetDevice(device));
+  hipStream_t s = nullptr;
+  BMG_HIP(hipStreamCreate(&s));
+  int rc = launch_copy_timed(src_dev, dst_dev, bytes, reps, ms_per_launch, s);
+  hipStreamSynchronize(s);
+  hipStreamDestroy(s);
+  return rc;
+}
+
 extern "C" int bmg_scan_xtr(bmg_ctx* c, const double* r, int nvec, int coding, int digits, double* out) {
   if (!c || !r || !out) { set_error("null argument"); return BMG_E_ARG; }
   BMG_HIP(hipSetDevice(c->device));

@@ -520,6 +520,38 @@ int launch_scan_timed(bmg_ctx* c, const double* r_dev, int nvec, int coding, int
   return BMG_OK;
 }
 
+// plain 16-byte copy: the reference kernel of known traffic against which
+// the FETCH_SIZE / WRITE_SIZE counters of the other kernels are scaled
+__global__ void k_copy16(const uint4* __restrict__ src, uint4* __restrict__ dst, uint64_t n16) {
+  uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < n16; i += stride) dst[i] = src[i];
+}
+
+int launch_copy_timed(const void* src, void* dst, uint64_t bytes, int reps, double* ms, hipStream_t s) {
+  if (bytes % 16 || ((uintptr_t)src | (uintptr_t)dst) % 16) {
+    set_error("copy: 16-byte aligned sizes and pointers only"); return BMG_E_ARG;
+  }
+  uint64_t n16 = bytes / 16;
+  unsigned grid = (unsigned)std::min<uint64_t>((n16 + 255) / 256, 256 * 64);
+  hipEvent_t e0, e1;
+  BMG_HIP(hipEventCreate(&e0));
+  BMG_HIP(hipEventCreate(&e1));
+  double total = 0.0;
+  for (int r = 0; r < reps; ++r) {
+    BMG_HIP(hipEventRecord(e0, s));
+    hipLaunchKernelGGL(k_copy16, dim3(grid), dim3(256), 0, s, (const uint4*)src, (uint4*)dst, n16);
+    BMG_HIP(hipEventRecord(e1, s));
+    BMG_HIP(hipEventSynchronize(e1));
+    float t = 0;
+    BMG_HIP(hipEventElapsedTime(&t, e0, e1));
+    total += t;
+  }
+  hipEventDestroy(e0); hipEventDestroy(e1);
+  *ms = total / (reps > 0 ? reps : 1);
+  return BMG_OK;
+}
+
 int launch_band(bmg_ctx* c, uint32_t W, int ca, int cb, int32_t* out_dev, hipStream_t s) {
   if (W < 1 || W > 560) { set_error("band: 1 <= W <= 560"); return BMG_E_ARG; }
   if ((W + 31) / 16 > BAND_MAXP) { set_error("band: W too large"); return BMG_E_ARG; }
@@ -530,6 +562,27 @@ int launch_band(bmg_ctx* c, uint32_t W, int ca, int cb, int32_t* out_dev, hipStr
   hipLaunchKernelGGL(k_band, dim3(grid), dim3(256), lds, s, c->g, c->p, c->W32, W, ca, cb, out_dev, nrb);
   BMG_HIP(hipGetLastError());
   return BMG_OK;
+}
+
+int launch_band_timed(bmg_ctx* c, uint32_t W, int ca, int cb, int32_t* out_dev, int reps, double* ms,
+                      hipStream_t s) {
+  hipEvent_t e0, e1;
+  BMG_HIP(hipEventCreate(&e0));
+  BMG_HIP(hipEventCreate(&e1));
+  double total = 0.0;
+  int rc = BMG_OK;
+  for (int r = 0; r < reps && !rc; ++r) {
+    BMG_HIP(hipEventRecord(e0, s));
+    rc = launch_band(c, W, ca, cb, out_dev, s);
+    BMG_HIP(hipEventRecord(e1, s));
+    BMG_HIP(hipEventSynchronize(e1));
+    float t = 0;
+    BMG_HIP(hipEventElapsedTime(&t, e0, e1));
+    total += t;
+  }
+  hipEventDestroy(e0); hipEventDestroy(e1);
+  *ms = total / (reps > 0 ? reps : 1);
+  return rc;
 }
 
 }  // namespace bmg

@@ -51,6 +51,9 @@ int launch_scan(bmg_ctx* c, const double* r_dev, int nvec, int coding, int digit
 int launch_scan_timed(bmg_ctx* c, const double* r_dev, int nvec, int coding, int digits,
                       double* out_dev, int reps, double* ms, hipStream_t s);
 int launch_band(bmg_ctx* c, uint32_t W, int ca, int cb, int32_t* out_dev, hipStream_t s);
+int launch_band_timed(bmg_ctx* c, uint32_t W, int ca, int cb, int32_t* out_dev, int reps, double* ms,
+                      hipStream_t s);
+int launch_copy_timed(const void* src, void* dst, uint64_t bytes, int reps, double* ms, hipStream_t s);
 int ensure(void** ptr, size_t* have, size_t need);
 }  // namespace bmg
 

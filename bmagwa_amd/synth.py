@@ -64,7 +64,7 @@ def phenotype(decode_causal, n, p, n_causal, h2, seed, dominance=False):
 
 
 def sampler_cfg(n, p, n_causal, types="A", seed=1, chain=0, **over):
-    """Plain PMV settings of the benchmark (SURVEY-restated): e_qg = #causal,
+    """Plain PMV settings of the benchmark: e_qg = #causal,
     var_qg = #causal^2, the reference example's move settings, no delayed
     rejection on the device."""
     e_qg = max(float(n_causal), 1.0)

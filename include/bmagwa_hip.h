@@ -127,6 +127,20 @@ int bmg_ctx_n_pad(bmg_ctx* ctx, uint32_t* n_pad);
  * each launch with HIP events; *ms_per_launch is the mean kernel time. */
 int bmg_scan_xtr_timed(bmg_ctx* ctx, const double* r_dev, int nvec, int coding, int digits,
                        double* out_dev, int reps, double* ms_per_launch, void* stream);
+/* Timing form of bmg_band_xtx_dev: `reps` launches, each bracketed by HIP
+ * events on `stream`; *ms_per_launch is the mean kernel time. */
+int bmg_band_xtx_timed(bmg_ctx* ctx, uint32_t W, int coding_a, int coding_b, int32_t* out_dev,
+                       int reps, double* ms_per_launch, void* stream);
+
+/* Measurement helpers (no counterpart in the reference): device buffers for
+ * the timing forms above without a second runtime in the process, and a
+ * plain 16-byte copy kernel of known traffic that calibrates the HBM
+ * counters (FETCH_SIZE, WRITE_SIZE) of a profiler run. */
+int bmg_dev_alloc(int device, uint64_t bytes, void** ptr);
+int bmg_dev_free(int device, void* ptr);
+int bmg_memcpy_h2d(int device, void* dst, const void* src, uint64_t bytes);
+int bmg_copy_timed(int device, const void* src_dev, void* dst_dev, uint64_t bytes, int reps,
+                   double* ms_per_launch);
 
 /* -------------------------------------------------------------- model --- */
 /* Prior inputs of Model/Prior (src/prior.hpp:45-122), already solved. */
