@@ -758,7 +758,6 @@ extern "C" int bmg_batch_run(bmg_batch* b, uint64_t n_iter) {
   hipSetDevice(b->device);
   hipEvent_t e0, e1, c0, c1;
   hipEventCreate(&e0); hipEventCreate(&e1); hipEventCreate(&c0); hipEventCreate(&c1);
-  b->t_chain = 0; b->t_scan = 0; b->n_scans = 0;
   hipEventRecord(e0, b->stream);
   uint64_t remaining = n_iter;
   int rc = BMG_OK;
@@ -784,7 +783,7 @@ extern "C" int bmg_batch_run(bmg_batch* b, uint64_t n_iter) {
   }
   hipEventRecord(e1, b->stream);
   hipEventSynchronize(e1);
-  float tt = 0; hipEventElapsedTime(&tt, e0, e1); b->t_total = tt;
+  float tt = 0; hipEventElapsedTime(&tt, e0, e1); b->t_total += tt;
   hipEventDestroy(e0); hipEventDestroy(e1); hipEventDestroy(c0); hipEventDestroy(c1);
   if (rc) return rc;
   hipError_t e = hipStreamSynchronize(b->stream);

@@ -220,8 +220,8 @@ int bmg_batch_thinned(bmg_batch* b, int chain, uint32_t* sizes, int64_t cap_size
 /* RB scan of one chain's current state with a given y_hat (the drop-in for
  * RaoBlackwellizer::p_raoblackwell, src/sampler.cpp:32-261) */
 int bmg_batch_rb(bmg_batch* b, int chain, const double* y_hat, double* p_r);
-/* event timing of the last bmg_batch_run: total ms, ms in chain kernels,
- * ms in RB scans, number of scan launches */
+/* event timing summed over every bmg_batch_run since set-up: total ms, ms
+ * in chain kernels, ms in RB blocks, number of scan launches */
 int bmg_batch_timing(bmg_batch* b, double out[4]);
 
 #ifdef __cplusplus
