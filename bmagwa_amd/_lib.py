@@ -145,6 +145,7 @@ SIGNATURES = {
     "bmg_batch_setup": (C.c_int, _P),
     "bmg_batch_run": (C.c_int, _P, C.c_uint64),
     "bmg_batch_sync": (C.c_int, _P),
+    "bmg_batch_set_learned": (C.c_int, _P, C.c_int, C.POINTER(C.c_float), C.c_int, C.c_double),
     "bmg_batch_state": (C.c_int, _P, C.c_int, _D),
     "bmg_batch_loci": (C.c_int, _P, C.c_int, _U32, C.c_int, C.POINTER(C.c_int)),
     "bmg_batch_p_rao": (C.c_int, _P, C.c_int, _D),
@@ -411,6 +412,16 @@ class Batch:
 
     def sync(self):
         check(lib().bmg_batch_sync(self.h))
+
+    def set_learned(self, ds, w, H, p_learned):
+        """Frozen learned-swap weights (p x 2H float32) for dataset ds, or None."""
+        if w is None:
+            check(lib().bmg_batch_set_learned(self.h, ds, None, H, p_learned))
+            return
+        a = np.ascontiguousarray(w, dtype=np.float32)
+        if a.shape != (self.ctxs[ds].p, 2 * H):
+            raise ValueError("weights must be p x 2H")
+        check(lib().bmg_batch_set_learned(self.h, ds, _p(a, C.c_float), H, p_learned))
 
     def state(self, c):
         o = np.zeros(8)

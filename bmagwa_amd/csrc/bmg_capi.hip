@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstddef>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -355,6 +356,8 @@ struct bmg_batch {
   // graph of one full segment
   hipGraphExec_t graph_exec = nullptr;
   hipGraph_t graph = nullptr;
+  // learned-swap weights per dataset (p x 2H floats)
+  std::vector<float*> lw_dev;
 };
 
 struct bmg_model {
@@ -524,6 +527,7 @@ extern "C" int bmg_batch_destroy(bmg_batch* b) {
   hipSetDevice(b->device);
   if (b->stream) hipStreamSynchronize(b->stream);
   hipFree(b->pool); hipFree(b->d_data); hipFree(b->d_consts); hipFree(b->d_scal); hipFree(b->d_arrays);
+  for (float* w : b->lw_dev) hipFree(w);
   hipFree(b->r); hipFree(b->xa); hipFree(b->xh); hipFree(b->rsum); hipFree(b->d_list);
   if (b->graph_exec) hipGraphExecDestroy(b->graph_exec);
   if (b->graph) hipGraphDestroy(b->graph);
@@ -789,6 +793,40 @@ extern "C" int bmg_batch_run(bmg_batch* b, uint64_t n_iter) {
   hipError_t e = hipStreamSynchronize(b->stream);
   if (e != hipSuccess) return hip_fail(e, "bmg_batch_run");
   return BMG_OK;
+}
+
+// learned swap on/off for the chains of dataset ds (see move_learned_swap)
+static int set_learn_on(bmg_batch* b, int ds, int on, double p_learned) {
+  for (int c = 0; c < b->n_chains; ++c) {
+    if (b->chain_ds[c] != ds) continue;
+    b->h_consts[c].p_learned = p_learned;
+    int32_t v = on;
+    BMG_HIP(hipMemcpy((char*)(b->d_scal + c) + offsetof(ChainScal, learn_on), &v, sizeof(v),
+                      hipMemcpyHostToDevice));
+  }
+  BMG_HIP(hipMemcpy(b->d_consts, b->h_consts.data(), sizeof(ChainConst) * b->n_chains, hipMemcpyHostToDevice));
+  return BMG_OK;
+}
+
+extern "C" int bmg_batch_set_learned(bmg_batch* b, int ds, const float* w, int H, double p_learned) {
+  if (!b || ds < 0 || ds >= (int)b->ctxs.size()) { set_error("bad batch or dataset"); return BMG_E_ARG; }
+  if (H < 1 || H > BMG_LH_MAX || !(p_learned >= 0.0 && p_learned <= 1.0)) {
+    set_error("learned swap: 1 <= H <= 32 and 0 <= p_learned <= 1"); return BMG_E_ARG;
+  }
+  BMG_HIP(hipSetDevice(b->device));
+  BMG_HIP(hipStreamSynchronize(b->stream));
+  if (!w) return set_learn_on(b, ds, 0, 0.0);
+  if (b->lw_dev.size() < b->ctxs.size()) b->lw_dev.resize(b->ctxs.size(), nullptr);
+  DevData& D = b->h_data[ds];
+  size_t cnt = (size_t)D.p * 2 * H;
+  if (b->lw_dev[ds] && D.lH != H) { hipFree(b->lw_dev[ds]); b->lw_dev[ds] = nullptr; }
+  if (!b->lw_dev[ds]) BMG_HIP(hipMalloc(&b->lw_dev[ds], sizeof(float) * cnt));
+  std::vector<float> hw(w, w + cnt);
+  for (float& x : hw) x = x > 0.0f && std::isfinite(x) ? x : 0.0f;
+  BMG_HIP(hipMemcpy(b->lw_dev[ds], hw.data(), sizeof(float) * cnt, hipMemcpyHostToDevice));
+  D.lw = b->lw_dev[ds]; D.lH = H;
+  BMG_HIP(hipMemcpy(b->d_data, b->h_data.data(), sizeof(DevData) * b->h_data.size(), hipMemcpyHostToDevice));
+  return set_learn_on(b, ds, 1, p_learned);
 }
 
 extern "C" int bmg_batch_sync(bmg_batch* b) {

@@ -6,6 +6,7 @@
 #define BMG_MAXMS 64     // max_move_size supported on the device
 #define BMG_KMAXG 256    // largest compiled model capacity (columns)
 #define BMG_TREE_LEV 8
+#define BMG_LH_MAX 32    // largest learned-swap window
 
 namespace bmg {
 
@@ -25,6 +26,8 @@ struct DevData {
   uint64_t lev_len[BMG_TREE_LEV], lev_off[BMG_TREE_LEV];
   uint64_t tree_size;    // doubles per tree
   uint64_t lds_lev_size; // doubles of levels >= 2
+  const float* lw;       // learned-swap weights p x 2*lH (null: none)
+  int lH;                // learned-swap window (SNPs each side, <= 32)
 };
 
 // Per-chain constants (options + solved prior, src/sampler.hpp:107-203,
@@ -41,6 +44,7 @@ struct ChainConst {
   int32_t use_ind;
   double inv_tau2_e0;
   uint64_t tr_cap, th_cap, th_loci_cap;
+  double p_learned;      // probability of a learned swap per iteration
 };
 
 // Mutable per-chain scalars (device global memory).
@@ -51,7 +55,8 @@ struct ChainScal {
   int64_t n_rao_burnin, p_rao_n;
   uint64_t p_proposal_n;
   double p_ms;
-  uint64_t n_acpt[3], n_moves[3];
+  uint64_t n_acpt[4], n_moves[4];   // add/remove, swap, state change, learned swap
+  int32_t learn_on;
   uint32_t rb_calls;
   int32_t movesize;
   double pves[3];

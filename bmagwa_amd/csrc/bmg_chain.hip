@@ -66,6 +66,7 @@ struct Reg {
   int movesize;
   double tot_add, tot_rem;
   int err;
+  int learn_on;
 };
 
 template <int KMAX>
@@ -1168,6 +1169,86 @@ __device__ int move_switch(Ctx<KMAX>& C, double* adapt) {
   return 0;
 }
 
+// Learned swap (no counterpart in the reference): the swap of
+// src/sampler.hpp:1271-1418 with the partner of the removed SNP j drawn from
+// the frozen weights W[j][.] over the SNPs within lH of j that are not in
+// the model, and the exact Hastings ratio
+//   log W[k][j] - log Z_r - log W[j][k] + log Z_f
+// (Z_f, Z_r: forward / reverse normalisers; the uniform choice of the
+// removed SNP and the prior cancel because the size is unchanged).  Column
+// c of a weight row is the offset c < lH ? c - lH : c - lH + 1.  Lanes
+// gather the masked weights into LDS; every lane then runs the same
+// sequential sums, so the draw and the ratio are those of the oracle.
+__device__ __forceinline__ int lw_off(int H, int c) { return c < H ? c - H : c - H + 1; }
+__device__ __forceinline__ int lw_col(int H, int o) { return o < 0 ? o + H : o + H - 1; }
+
+template <int KMAX>
+__device__ int move_learned_swap(Ctx<KMAX>& C) {
+  Smem<KMAX>& S = C.S;
+  Reg& R = C.R;
+  const int H = C.D.lH;
+  const int64_t m_g = (int64_t)C.D.p;
+  R.movesize = 1;
+  int mir = (int)floor(rng_u01(R.rng) * R.nloci);
+  if (mir >= R.nloci) mir = R.nloci - 1;
+  const int64_t j = S.loci[mir];
+  double* wf = S.work;            // 2H forward weights (0: not a candidate)
+  double* wr = S.work + 2 * H;    // 2H reverse weights
+  __syncthreads();
+  if (threadIdx.x < 2 * H) {
+    int64_t k = j + lw_off(H, threadIdx.x);
+    double w = 0.0;
+    if (k >= 0 && k < m_g && C.A.model_inds[k] < 0) w = (double)C.D.lw[(size_t)j * 2 * H + threadIdx.x];
+    wf[threadIdx.x] = w;
+  }
+  __syncthreads();
+  double zf = 0.0;
+  for (int c = 0; c < 2 * H; ++c) zf += wf[c];
+  if (!(zf > 0.0)) return 0;
+  double u = rng_u01(R.rng) * zf, acc = 0.0;
+  int pick = -1;
+  for (int c = 0; c < 2 * H; ++c) {
+    if (wf[c] > 0.0) {
+      acc += wf[c];
+      pick = c;
+      if (u < acc) break;
+    }
+  }
+  if (pick < 0) return 0;
+  const int64_t k = j + lw_off(H, pick);
+  const double wjk = wf[pick];
+  __syncthreads();
+  if (threadIdx.x < 2 * H) {
+    int64_t q = k + lw_off(H, threadIdx.x);
+    double w = 0.0;
+    if (q >= 0 && q < m_g && (q == j || C.A.model_inds[q] < 0))
+      w = (double)C.D.lw[(size_t)k * 2 * H + threadIdx.x];
+    wr[threadIdx.x] = w;
+  }
+  __syncthreads();
+  double zr = 0.0;
+  for (int c = 0; c < 2 * H; ++c) zr += wr[c];
+  const double wkj = wr[lw_col(H, (int)(j - k))];
+  if (!(wkj > 0.0)) return 0;
+  const double lq = log(wkj) - log(zr) - log(wjk) + log(zf);
+  const int t_ind = S.tind[mir];
+  __syncthreads();
+  if (threadIdx.x == 0) { S.mi_rem[0] = (uint64_t)j; S.mi_add[0] = (uint64_t)k; }
+  remove_term(C, mir);
+  double d2[2];
+  prepare_add_new_term(C, d2);
+  add_term(C, (uint32_t)k, t_ind, d2);
+  double log_r = R.ll - R.c_ll + lq;
+  if (log_r >= 0 || log(rng_u01(R.rng)) <= log_r) {
+    commit(C);
+    tree_set(C, 0, (uint64_t)j, 0); tree_set(C, 1, (uint64_t)j, 1);
+    tree_set(C, 0, (uint64_t)k, 1); tree_set(C, 1, (uint64_t)k, 0);
+    return 2;
+  }
+  restore(C);
+  return 0;
+}
+
 // src/sampler.hpp:1421-1669 without delayed rejection
 template <int KMAX>
 __device__ int move_statechange(Ctx<KMAX>& C, double* adapt) {
@@ -1302,6 +1383,7 @@ __device__ void load_chain(Ctx<KMAX>& C, const ChainScal& CS) {
   R.lmin = KMAX; R.n_rm = 0; R.n_marks = 0; R.movesize = CS.movesize;
   R.tot_add = CS.tot_add; R.tot_rem = CS.tot_rem;
   R.err = CS.err;
+  R.learn_on = CS.learn_on && C.D.lw != nullptr && C.D.lH >= 1 && C.D.lH <= BMG_LH_MAX;
   const double* vec = C.A.vec;
   const int32_t* idx = C.A.idx;
   for (int e = threadIdx.x; e < P(R.cols); e += BMG_BLOCK) S.L[e] = C.A.Lcur[e];
@@ -1392,13 +1474,21 @@ __device__ void run_iterations(Ctx<KMAX>& C, ChainScal& CS, uint64_t n_iter) {
   uint64_t it0 = CS.n_iter;
   for (uint64_t iter = it0; iter < it0 + n_iter; ++iter) {
     if ((iter + 1) % K.n_st == 0) refresh_tau2(C);
-    // PMV move mix with one effect type (src/sampler.cpp:468-476)
-    const double pm[3] = {0.7, 0.7 + 0.15, 0.7 + 0.15 + 0.15};
-    double u = rng_u01(R.rng);
-    double r = u * pm[2];
-    int move = r < pm[0] ? 0 : (r < pm[1] ? 1 : 2);
+    // PMV move mix with one effect type (src/sampler.cpp:468-476); with a
+    // learned table, a learned swap first with probability p_learned
+    int move;
+    if (R.learn_on && rng_u01(R.rng) < K.p_learned) {
+      move = 3;
+    } else {
+      const double pm[3] = {0.7, 0.7 + 0.15, 0.7 + 0.15 + 0.15};
+      double u = rng_u01(R.rng);
+      double r = u * pm[2];
+      move = r < pm[0] ? 0 : (r < pm[1] ? 1 : 2);
+    }
     int jd = 0;
-    if (move == 0) jd = move_addrem(C, adapt);
+    if (move == 3) {
+      if (R.c_nloci > 0) jd = move_learned_swap(C);
+    } else if (move == 0) jd = move_addrem(C, adapt);
     else if (move == 1) {
       if (R.c_nloci > 0 && (uint64_t)R.c_nloci < C.D.p - 1) jd = move_switch(C, adapt);
     } else {

@@ -201,6 +201,14 @@ int bmg_batch_setup(bmg_batch* b);
 /* n_iter more iterations of every chain (the loop of src/sampler.cpp:626-834) */
 int bmg_batch_run(bmg_batch* b, uint64_t n_iter);
 int bmg_batch_sync(bmg_batch* b);
+/* Learned swap (no counterpart in the reference): frozen partner weights
+ * w[p][2H] for the chains of dataset ds (column c <-> SNP offset
+ * c < H ? c - H : c - H + 1; negative or non-finite entries count as 0).
+ * Each iteration first takes a learned swap with probability p_learned:
+ * remove a uniformly chosen model SNP j, add k ~ w[j][.] over the SNPs
+ * within H of j outside the model, accept with the exact Metropolis-Hastings
+ * ratio.  w = NULL switches it off.  1 <= H <= 32. */
+int bmg_batch_set_learned(bmg_batch* b, int ds, const float* w, int H, double p_learned);
 /* per-chain state: loglik, size, sigma2, alpha, p_move_size, n_accepted,
  * n_iter, loci_len */
 int bmg_batch_state(bmg_batch* b, int chain, double out[8]);

@@ -123,6 +123,8 @@ def load():
     _sig(lib, "orc_sampler_init_p_proposal_flat", None, _P)
     _sig(lib, "orc_sampler_sample", C.c_int, _P, C.c_uint64)
     _sig(lib, "orc_sampler_setup", C.c_int, _P)
+    _sig(lib, "orc_sampler_set_learned", C.c_int, _P, C.POINTER(C.c_float), C.c_int, C.c_double)
+    _sig(lib, "orc_sampler_set_move_mix", C.c_int, _P, _D)
     _sig(lib, "orc_sampler_iterate", C.c_int, _P, C.c_uint64)
     _sig(lib, "orc_sampler_p_rao", _D, _P)
     _sig(lib, "orc_sampler_p_proposal", _D, _P)
@@ -346,6 +348,18 @@ class Sampler:
 
     def setup(self):
         return lib().orc_sampler_setup(self.h)
+
+    def set_move_mix(self, p):
+        a = np.ascontiguousarray(p, dtype=np.float64)
+        return lib().orc_sampler_set_move_mix(self.h, ptr(a))
+
+    def set_learned(self, w, H, p_learned):
+        """Frozen learned-swap weights w (p x 2H float32), or None."""
+        if w is None:
+            return lib().orc_sampler_set_learned(self.h, None, H, p_learned)
+        self._lw = np.ascontiguousarray(w, dtype=np.float32)
+        assert self._lw.shape == (self.data.p, 2 * H)
+        return lib().orc_sampler_set_learned(self.h, ptr(self._lw, C.c_float), H, p_learned)
 
     def iterate(self, n):
         rc = lib().orc_sampler_iterate(self.h, n)

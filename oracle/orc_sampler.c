@@ -242,6 +242,8 @@ struct orc_sampler {
   double pves[3];
   double k_move_size;
   int setup_done;
+  /* learned swap: frozen proposal weights W[j][2H] (see do_learned_swap) */
+  float* lw; int lH; double p_learned; int learn_on;
   /* traces */
   int64_t tr_len, tr_cap;
   uint8_t *tr_move, *tr_jd, *tr_ms; uint32_t* tr_size; double* tr_ll;
@@ -359,7 +361,7 @@ void orc_sampler_destroy(orc_sampler* s)
   orc__model_destroy(s->cur); orc__model_destroy(s->nw); orc__model_destroy(s->exh);
   orc_datamodel_destroy(s->dm);
   orc_sumtree_destroy(s->dd_add); orc_sumtree_destroy(s->dd_rem);
-  free(s->pre_xx);
+  free(s->pre_xx); free(s->lw);
   free(s->p_proposal); free(s->p_rao); free(s->q_add); free(s->q_rem); free(s->p_r);
   free(s->mi_add); free(s->mi_rem); free(s->mi); free(s->mi_map); free(s->mi_isadd);
   free(s->q_pms); free(s->r_ms_sum); free(s->r_ms_n); free(s->q_p0_ms);
@@ -1011,6 +1013,99 @@ static int do_switch_of_nearby_snps(orc_sampler* s)
   return 0;
 }
 
+/* Offset o in [-H, -1] U [1, H] <-> column of a weight row. */
+static int lw_col(int H, int o) { return o < 0 ? o + H : o + H - 1; }
+static int lw_off(int H, int c) { return c < H ? c - H : c - H + 1; }
+
+/* Learned swap (no counterpart in the reference; the Metropolis-Hastings
+ * form of the swap move of src/sampler.hpp:1271-1418 with a non-uniform,
+ * state-dependent partner draw).  A model SNP j is chosen uniformly; its
+ * partner k is drawn from the frozen weights W[j][.] restricted to the SNPs
+ * within H of j that are not in the current model; gamma' = gamma - j + k.
+ * The reverse move chooses k uniformly (same model size) and draws j from
+ * W[k][.] restricted to the SNPs within H of k not in gamma'.  Hence
+ *   log r = l(gamma') - l(gamma) + log W[k][j] - log Z_r - log W[j][k] + log Z_f
+ * with Z_f, Z_r the two normalisers (the prior and the uniform choices
+ * cancel).  The sums run sequentially in offset order so that the device
+ * reproduces them exactly. */
+static int do_learned_swap(orc_sampler* s)
+{
+  orc_model* cm = s->cur; orc_model* nm = s->nw;
+  const int H = s->lH;
+  const int64_t m_g = (int64_t)s->m_g;
+  s->movesize = 1;
+  size_t mir = (size_t)floor(orc_rng_u01(&s->rng) * nm->nloci);
+  int64_t j = nm->loci[mir];
+  const float* Wj = s->lw + (size_t)j * 2 * H;
+  double zf = 0.0;
+  for (int c = 0; c < 2 * H; ++c) {
+    int64_t k = j + lw_off(H, c);
+    if (k >= 0 && k < m_g && cm->model_inds[k] < 0) zf += (double)Wj[c];
+  }
+  if (!(zf > 0.0)) return 0;
+  double u = orc_rng_u01(&s->rng) * zf, acc = 0.0;
+  int pick = -1;
+  for (int c = 0; c < 2 * H; ++c) {
+    int64_t k = j + lw_off(H, c);
+    if (k >= 0 && k < m_g && cm->model_inds[k] < 0 && Wj[c] > 0.0f) {
+      acc += (double)Wj[c];
+      pick = c;
+      if (u < acc) break;
+    }
+  }
+  int64_t k = j + lw_off(H, pick);
+  const float* Wk = s->lw + (size_t)k * 2 * H;
+  double zr = 0.0, wr = 0.0;
+  for (int c = 0; c < 2 * H; ++c) {
+    int64_t q = k + lw_off(H, c);
+    if (q < 0 || q >= m_g) continue;
+    if (q == j || cm->model_inds[q] < 0) {
+      zr += (double)Wk[c];
+      if (q == j) wr = (double)Wk[c];
+    }
+  }
+  if (!(wr > 0.0)) return 0;
+  double lq = log(wr) - log(zr) - log((double)Wj[pick]) + log(zf);
+  s->mi_rem[0] = (size_t)j; s->mi_add[0] = (size_t)k;
+  int t_ind = nm->type_inds[mir];
+  orc_model_remove_term(nm, (int)mir);
+  double d2[2];
+  prepare_add_new_term(s, (size_t)k, orc__dm_type(s->dm, t_ind), d2, 1);
+  orc_model_add_term(nm, (uint32_t)k, t_ind, d2);
+  double log_r = nm->loglik - cm->loglik + lq;
+  if (log_r >= 0 || log(orc_rng_u01(&s->rng)) <= log_r) {
+    orc_model_copy(cm, nm);
+    orc_sumtree_unzero(s->dd_add, (size_t)j); orc_sumtree_zero(s->dd_rem, (size_t)j);
+    orc_sumtree_zero(s->dd_add, (size_t)k); orc_sumtree_unzero(s->dd_rem, (size_t)k);
+    return 2;
+  }
+  orc_model_copy(nm, cm);
+  return 0;
+}
+
+int orc_sampler_set_move_mix(orc_sampler* s, const double p[3])
+{
+  if (!s || p[0] < 0 || p[1] < 0 || p[2] < 0 || p[0] + p[1] + p[2] <= 0) return -1;
+  for (int i = 0; i < NMOVES; ++i) s->p_moves[i] = 0.0;
+  for (int i = 0; i < 3; ++i) s->p_moves[i] = p[i];
+  s->p_moves_cumsum[0] = s->p_moves[0];
+  for (int i = 1; i < NMOVES; ++i) s->p_moves_cumsum[i] = s->p_moves_cumsum[i - 1] + s->p_moves[i];
+  return 0;
+}
+
+int orc_sampler_set_learned(orc_sampler* s, const float* w, int H, double p_learned)
+{
+  if (!s || H < 1 || H > 32 || p_learned < 0 || p_learned > 1) return -1;
+  free(s->lw);
+  s->lw = NULL; s->learn_on = 0;
+  if (!w) return 0;
+  size_t cnt = s->m_g * (size_t)(2 * H);
+  s->lw = (float*)malloc(sizeof(float) * cnt);
+  for (size_t i = 0; i < cnt; ++i) s->lw[i] = w[i] > 0.0f ? w[i] : 0.0f;
+  s->lH = H; s->p_learned = p_learned; s->learn_on = 1;
+  return 0;
+}
+
 /* src/sampler.hpp:1421-1669 */
 static int do_statechange_of_nearby_snps(orc_sampler* s)
 {
@@ -1221,10 +1316,15 @@ int orc_sampler_iterate(orc_sampler* s, uint64_t n_iter)
       orc_model_compute_log_likelihood(s->cur);
       orc_model_copy(s->nw, s->cur);
     }
-    int move = sample_discrete_naive(s->p_moves_cumsum, NMOVES, &s->rng);
+    int move;
+    if (s->learn_on && orc_rng_u01(&s->rng) < s->p_learned) move = 3;
+    else move = sample_discrete_naive(s->p_moves_cumsum, NMOVES, &s->rng);
     int jd = 0;
     if (!isfinite(s->cur->loglik)) return -2;
     switch (move) {
+      case 3:
+        if (s->cur->nloci > 0) jd = do_learned_swap(s);
+        break;
       case 0: jd = do_multistep_additions_and_removals(s); break;
       case 1:
         if (s->cur->nloci > 0 && (size_t)s->cur->nloci < s->m_g - 1)
