@@ -128,6 +128,16 @@ SIGNATURES = {
     "bmg_dev_free": (C.c_int, C.c_int, _P),
     "bmg_memcpy_h2d": (C.c_int, C.c_int, _P, _P, C.c_uint64),
     "bmg_copy_timed": (C.c_int, C.c_int, _P, _P, C.c_uint64, C.c_int, _D),
+    "bmg_mlp_nparams": (C.c_int,),
+    "bmg_mlp_create": (C.c_int, C.c_int, _PP),
+    "bmg_mlp_destroy": (C.c_int, _P),
+    "bmg_mlp_set_params": (C.c_int, _P, C.POINTER(C.c_float)),
+    "bmg_mlp_get_params": (C.c_int, _P, C.POINTER(C.c_float)),
+    "bmg_mlp_forward": (C.c_int, _P, C.POINTER(C.c_float), C.c_int64, C.POINTER(C.c_float)),
+    "bmg_mlp_gradients": (C.c_int, _P, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int64,
+                          C.POINTER(C.c_float)),
+    "bmg_mlp_adam_step": (C.c_int, _P, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int64,
+                          C.c_float),
     "bmg_model_kmax": (C.c_int,),
     "bmg_model_create": (C.c_int, _P, _D, _PP),
     "bmg_model_destroy": (C.c_int, _P),
@@ -311,6 +321,57 @@ class DevBuf:
     def __del__(self):
         try:
             self.free()
+        except Exception:
+            pass
+
+
+class Mlp:
+    """The learned-proposal network's device kernels, driven from the host."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        check(lib().bmg_mlp_create(device, C.byref(h)))
+        self.h = h
+        self.npar = lib().bmg_mlp_nparams()
+
+    def set_params(self, p):
+        a = np.ascontiguousarray(p, dtype=np.float32)
+        assert a.shape == (self.npar,)
+        check(lib().bmg_mlp_set_params(self.h, _p(a, C.c_float)))
+
+    def get_params(self):
+        a = np.zeros(self.npar, np.float32)
+        check(lib().bmg_mlp_get_params(self.h, _p(a, C.c_float)))
+        return a
+
+    def forward(self, X):
+        x = np.ascontiguousarray(X, dtype=np.float32)
+        out = np.zeros(x.shape[0], np.float32)
+        check(lib().bmg_mlp_forward(self.h, _p(x, C.c_float), x.shape[0], _p(out, C.c_float)))
+        return out
+
+    def gradients(self, X, labels):
+        x = np.ascontiguousarray(X, dtype=np.float32)
+        lab = np.ascontiguousarray(labels, dtype=np.float32)
+        g = np.zeros(self.npar, np.float32)
+        check(lib().bmg_mlp_gradients(self.h, _p(x, C.c_float), _p(lab, C.c_float), x.shape[0],
+                                      _p(g, C.c_float)))
+        return g
+
+    def adam_step(self, X, labels, lr):
+        x = np.ascontiguousarray(X, dtype=np.float32)
+        lab = np.ascontiguousarray(labels, dtype=np.float32)
+        check(lib().bmg_mlp_adam_step(self.h, _p(x, C.c_float), _p(lab, C.c_float), x.shape[0],
+                                      C.c_float(lr)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().bmg_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
         except Exception:
             pass
 

@@ -39,6 +39,7 @@ enum { BMG_A = 0, BMG_H = 1, BMG_D = 2, BMG_R = 3, BMG_AH = 4 };
 typedef struct bmg_ctx bmg_ctx;
 typedef struct bmg_model bmg_model;
 typedef struct bmg_batch bmg_batch;
+typedef struct bmg_mlp bmg_mlp;
 
 const char* bmg_last_error(void);
 int bmg_device_count(int* count);
@@ -231,6 +232,26 @@ int bmg_batch_rb(bmg_batch* b, int chain, const double* y_hat, double* p_r);
 /* event timing summed over every bmg_batch_run since set-up: total ms, ms
  * in chain kernels, ms in RB blocks, number of scan launches */
 int bmg_batch_timing(bmg_batch* b, double out[4]);
+
+/* --------------------------------------- learned-proposal network ----- */
+/* No counterpart in the reference.  The network of the learned proposal:
+ * 16 features -> 256 -> 256 (ReLU) -> 1, bf16 weights and activations,
+ * fp32 accumulation on the MFMA units, fp32 master weights and Adam.
+ * Parameters are one fp32 array of bmg_mlp_nparams() values:
+ * W1[16][256], b1[256], W2[256][256], b2[256], w3[256], b3.  Features are
+ * rounded to bf16 (nearest even) on the way in.  These entry points expose
+ * the kernels the batch trains with, for checking them in isolation. */
+int bmg_mlp_nparams(void);
+int bmg_mlp_create(int device, bmg_mlp** out);        /* all parameters 0 */
+int bmg_mlp_destroy(bmg_mlp* m);
+int bmg_mlp_set_params(bmg_mlp* m, const float* params);   /* resets Adam */
+int bmg_mlp_get_params(bmg_mlp* m, float* params);
+int bmg_mlp_forward(bmg_mlp* m, const float* X, int64_t rows, float* out);
+/* gradient of the mean Huber(1) loss of out - labels */
+int bmg_mlp_gradients(bmg_mlp* m, const float* X, const float* labels, int64_t rows,
+                      float* grads);
+/* one Adam step (beta 0.9 / 0.999, eps 1e-8) on that gradient */
+int bmg_mlp_adam_step(bmg_mlp* m, const float* X, const float* labels, int64_t rows, float lr);
 
 #ifdef __cplusplus
 }

@@ -125,6 +125,12 @@ def load():
     _sig(lib, "orc_sampler_setup", C.c_int, _P)
     _sig(lib, "orc_sampler_set_learned", C.c_int, _P, C.POINTER(C.c_float), C.c_int, C.c_double)
     _sig(lib, "orc_sampler_set_move_mix", C.c_int, _P, _D)
+    _F = C.POINTER(C.c_float)
+    _sig(lib, "orc_bf16_round", C.c_float, C.c_float)
+    _sig(lib, "orc_mlp_forward", None, _F, _F, C.c_int64, _F)
+    _sig(lib, "orc_mlp_gradients", C.c_double, _F, _F, _F, C.c_int64, _D)
+    _sig(lib, "orc_mlp_loss_exact", C.c_double, _F, _F, _F, C.c_int64, _D)
+    _sig(lib, "orc_mlp_adam", None, _F, _F, _F, _F, C.c_float, C.c_int)
     _sig(lib, "orc_sampler_iterate", C.c_int, _P, C.c_uint64)
     _sig(lib, "orc_sampler_p_rao", _D, _P)
     _sig(lib, "orc_sampler_p_proposal", _D, _P)
@@ -451,6 +457,61 @@ class Sampler:
             lib().orc_sampler_destroy(self.h)
         except Exception:
             pass
+
+
+MLP_IN, MLP_HID = 16, 256
+MLP_OFF = dict(W1=0, b1=MLP_IN * MLP_HID)
+MLP_OFF["W2"] = MLP_OFF["b1"] + MLP_HID
+MLP_OFF["b2"] = MLP_OFF["W2"] + MLP_HID * MLP_HID
+MLP_OFF["w3"] = MLP_OFF["b2"] + MLP_HID
+MLP_OFF["b3"] = MLP_OFF["w3"] + MLP_HID
+MLP_NPAR = MLP_OFF["b3"] + 1
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def mlp_forward(params, X):
+    p, x = _f32(params), _f32(X)
+    out = np.zeros(x.shape[0], np.float32)
+    lib().orc_mlp_forward(ptr(p, C.c_float), ptr(x, C.c_float), x.shape[0], ptr(out, C.c_float))
+    return out
+
+
+def mlp_gradients(params, X, labels, exact=False):
+    p, x, lab = _f32(params), _f32(X), _f32(labels)
+    g = np.zeros(MLP_NPAR)
+    f = lib().orc_mlp_loss_exact if exact else lib().orc_mlp_gradients
+    loss = f(ptr(p, C.c_float), ptr(x, C.c_float), ptr(lab, C.c_float), x.shape[0], ptr(g))
+    return loss, g
+
+
+def mlp_loss_exact(params, X, labels):
+    p, x, lab = _f32(params), _f32(X), _f32(labels)
+    return lib().orc_mlp_loss_exact(ptr(p, C.c_float), ptr(x, C.c_float), ptr(lab, C.c_float),
+                                    x.shape[0], None)
+
+
+def mlp_adam(params, m, v, grads, lr, t):
+    """In place on float32 arrays params, m, v."""
+    g = _f32(grads)
+    lib().orc_mlp_adam(ptr(params, C.c_float), ptr(m, C.c_float), ptr(v, C.c_float),
+                       ptr(g, C.c_float), C.c_float(lr), t)
+
+
+def mlp_init(seed, scale3=0.1):
+    """He-initialised parameters (biases small), float32."""
+    rng = np.random.default_rng(seed)
+    p = np.zeros(MLP_NPAR, np.float32)
+    o = MLP_OFF
+    p[o["W1"]:o["b1"]] = rng.normal(scale=np.sqrt(2.0 / MLP_IN), size=MLP_IN * MLP_HID)
+    p[o["b1"]:o["W2"]] = rng.normal(scale=0.05, size=MLP_HID)
+    p[o["W2"]:o["b2"]] = rng.normal(scale=np.sqrt(2.0 / MLP_HID), size=MLP_HID * MLP_HID)
+    p[o["b2"]:o["w3"]] = rng.normal(scale=0.05, size=MLP_HID)
+    p[o["w3"]:o["b3"]] = rng.normal(scale=scale3, size=MLP_HID)
+    p[o["b3"]] = 0.1
+    return p
 
 
 class SumTree:

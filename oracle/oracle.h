@@ -230,6 +230,29 @@ void   orc_geometric_dist_cdf(int maxsize, double p, double* values);
 int    orc_sample_discrete_naive(const double* cumsum, int m, double u01);
 double orc_ess_geyer(const double* x, int64_t n);
 
+/* ------------------------------------------- learned-proposal network --- */
+/* 16 -> 256 -> 256 -> 1 MLP, parameters in one fp32 array:
+ * W1[16][256], b1[256], W2[256][256], b2[256], w3[256], b3 */
+#define ORC_MLP_IN 16
+#define ORC_MLP_HID 256
+#define ORC_MLP_OFF_W1 0
+#define ORC_MLP_OFF_B1 (ORC_MLP_IN * ORC_MLP_HID)
+#define ORC_MLP_OFF_W2 (ORC_MLP_OFF_B1 + ORC_MLP_HID)
+#define ORC_MLP_OFF_B2 (ORC_MLP_OFF_W2 + ORC_MLP_HID * ORC_MLP_HID)
+#define ORC_MLP_OFF_W3 (ORC_MLP_OFF_B2 + ORC_MLP_HID)
+#define ORC_MLP_OFF_B3 (ORC_MLP_OFF_W3 + ORC_MLP_HID)
+#define ORC_MLP_NPAR (ORC_MLP_OFF_B3 + 1)
+float  orc_bf16_round(float f);
+void   orc_mlp_forward(const float* params, const float* X, int64_t B, float* out);
+/* gradients of the mean Huber loss with the device's bf16 rounding points;
+ * returns the loss */
+double orc_mlp_gradients(const float* params, const float* X, const float* labels, int64_t B,
+                         double* grads);
+/* the same loss and gradients in double without any rounding */
+double orc_mlp_loss_exact(const float* params, const float* X, const float* labels, int64_t B,
+                          double* grads);
+void   orc_mlp_adam(float* params, float* m, float* v, const float* grads, float lr, int t);
+
 #ifdef __cplusplus
 }
 #endif
