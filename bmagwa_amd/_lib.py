@@ -54,11 +54,15 @@ class BatchCfg(C.Structure):
         ("trace_every_iter", C.c_int32), ("proposal", C.c_int32), ("learn_every", C.c_int32),
         ("learn_freeze_iter", C.c_int32), ("scan_digits", C.c_int32), ("use_graph", C.c_int32),
         ("trace_capacity", C.c_uint64),
+        ("learn_window", C.c_int32), ("learn_steps", C.c_int32), ("learn_batch", C.c_int32),
+        ("learn_pad", C.c_int32), ("p_learned", C.c_double), ("learn_rate", C.c_double),
     ]
 
 
-BATCH_CFG_DEFAULT = dict(trace_every_iter=0, proposal=0, learn_every=1000, learn_freeze_iter=0,
-                         scan_digits=5, use_graph=0, trace_capacity=0)
+BATCH_CFG_DEFAULT = dict(trace_every_iter=0, proposal=0, learn_every=500, learn_freeze_iter=0,
+                         scan_digits=5, use_graph=0, trace_capacity=0, learn_window=20,
+                         learn_steps=100, learn_batch=4096, learn_pad=0, p_learned=0.3,
+                         learn_rate=1e-3)
 
 OPT_DEFAULTS = dict(max_move_size=20, max_SNP_neighborhood_size=10, p_move_size=0.2,
                     p_move_size_nbs=0.2, p_move_size_nbc=0.2, adapt_p_move_size=1,
@@ -156,6 +160,12 @@ SIGNATURES = {
     "bmg_batch_run": (C.c_int, _P, C.c_uint64),
     "bmg_batch_sync": (C.c_int, _P),
     "bmg_batch_set_learned": (C.c_int, _P, C.c_int, C.POINTER(C.c_float), C.c_int, C.c_double),
+    "bmg_batch_learn_state": (C.c_int, _P, C.c_int, _D),
+    "bmg_batch_learn_table": (C.c_int, _P, C.c_int, C.POINTER(C.c_float)),
+    "bmg_batch_learn_samples": (C.c_int, _P, C.c_int, _U32, _I32, C.POINTER(C.c_float), C.c_int64, _I64),
+    "bmg_batch_learn_params": (C.c_int, _P, C.c_int, C.POINTER(C.c_float)),
+    "bmg_batch_learn_static": (C.c_int, _P, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float), _I32),
+    "bmg_batch_learn_features": (C.c_int, _P, C.c_int, _U32, _I32, C.c_int64, C.POINTER(C.c_float)),
     "bmg_batch_state": (C.c_int, _P, C.c_int, _D),
     "bmg_batch_loci": (C.c_int, _P, C.c_int, _U32, C.c_int, C.POINTER(C.c_int)),
     "bmg_batch_p_rao": (C.c_int, _P, C.c_int, _D),
@@ -473,6 +483,47 @@ class Batch:
 
     def sync(self):
         check(lib().bmg_batch_sync(self.h))
+
+    def learn_state(self, ds=0):
+        o = np.zeros(4)
+        check(lib().bmg_batch_learn_state(self.h, ds, _p(o)))
+        return dict(rounds=int(o[0]), frozen=bool(o[1]), labels=int(o[2]), H=int(o[3]))
+
+    def learn_table(self, ds=0):
+        st = self.learn_state(ds)
+        w = np.zeros((self.ctxs[ds].p, 2 * st["H"]), np.float32)
+        check(lib().bmg_batch_learn_table(self.h, ds, _p(w, C.c_float)))
+        return w
+
+    def learn_samples(self, ds=0, cap=1 << 19):
+        j = np.zeros(cap, np.uint32); o = np.zeros(cap, np.int32); lab = np.zeros(cap, np.float32)
+        cnt = C.c_int64()
+        check(lib().bmg_batch_learn_samples(self.h, ds, _p(j, C.c_uint32), _p(o, C.c_int32),
+                                            _p(lab, C.c_float), cap, C.byref(cnt)))
+        k = min(cnt.value, cap)
+        return j[:k], o[:k], lab[:k], cnt.value
+
+    def learn_params(self, ds=0):
+        a = np.zeros(lib().bmg_mlp_nparams(), np.float32)
+        check(lib().bmg_batch_learn_params(self.h, ds, _p(a, C.c_float)))
+        return a
+
+    def learn_static(self, ds=0):
+        st = self.learn_state(ds)
+        p = self.ctxs[ds].p
+        z = np.zeros(p, np.float32); pb = np.zeros(p, np.float32)
+        band = np.zeros((p, st["H"] + 1), np.int32)
+        check(lib().bmg_batch_learn_static(self.h, ds, _p(z, C.c_float), _p(pb, C.c_float),
+                                           _p(band, C.c_int32)))
+        return z, pb, band
+
+    def learn_features(self, ds, j, off):
+        jj = np.ascontiguousarray(j, dtype=np.uint32)
+        oo = np.ascontiguousarray(off, dtype=np.int32)
+        out = np.zeros((len(jj), 16), np.float32)
+        check(lib().bmg_batch_learn_features(self.h, ds, _p(jj, C.c_uint32), _p(oo, C.c_int32),
+                                             len(jj), _p(out, C.c_float)))
+        return out
 
     def set_learned(self, ds, w, H, p_learned):
         """Frozen learned-swap weights (p x 2H float32) for dataset ds, or None."""

@@ -12,6 +12,7 @@
 #include "bmg_chain.h"
 #include "bmg_device.h"
 #include "bmg_internal.h"
+#include "bmg_learn.h"
 #include "bmg_rb.h"
 
 using namespace bmg;
@@ -358,6 +359,9 @@ struct bmg_batch {
   hipGraph_t graph = nullptr;
   // learned-swap weights per dataset (p x 2H floats)
   std::vector<float*> lw_dev;
+  // online learned proposal per dataset (proposal = 1)
+  std::vector<LearnDS*> learn;
+  std::vector<int> learn_on;
 };
 
 struct bmg_model {
@@ -455,8 +459,7 @@ static int ensure_rb_scratch(bmg_batch* b, int nv, uint32_t n_pad, uint64_t p) {
 }
 
 // RB block for the chains in `list` (all on dataset ds)
-static int rb_block_ds(bmg_batch* b, int ds, const std::vector<int>& chains, int do_scan_only_flagged) {
-  (void)do_scan_only_flagged;
+static int rb_block_ds(bmg_batch* b, int ds, const std::vector<int>& chains, int learn_now) {
   bmg_ctx* c = b->ctxs[ds];
   const DevData& D = b->h_data[ds];
   int digits = b->cfg.scan_digits > 0 ? b->cfg.scan_digits : 5;
@@ -481,12 +484,51 @@ static int rb_block_ds(bmg_batch* b, int ds, const std::vector<int>& chains, int
                            b->xa, b->xh, b->rsum, b->stream)) {
       set_error("rb finalize launch failed"); return BMG_E_HIP;
     }
+    if (learn_now) {
+      rc = learn_labels(b->learn[ds], b->d_data, b->d_consts, b->d_scal, b->d_arrays, b->d_list, nv, b->kmax,
+                        b->xa, b->rsum, b->stream);
+      if (rc) return rc;
+    }
   }
   return BMG_OK;
 }
 
 // the full RB block after an RB iteration: scans, element-wise updates,
 // tree rebuilds, clearing the pending flags
+static int set_learn_on(bmg_batch* b, int ds, int on, double p_learned);
+
+// learned proposal after an RB block of dataset ds: a training round every
+// learn_every iterations of burn-in and at its last block, which freezes the
+// table (no sample after burn-in sees it change)
+static int learn_after_block(bmg_batch* b, int ds, const std::vector<int>& all,
+                             const std::vector<ChainScal>& cs) {
+  LearnDS* L = b->learn[ds];
+  int burn = 0, last = 1;
+  for (int c : all) {
+    if (!cs[c].rb_accum_rao) burn = 1;        // this block was a burn-in block
+    if (cs[c].n_rao_burnin > 0) last = 0;     // more burn-in blocks follow
+  }
+  if (!burn) {
+    if (!learn_frozen(L)) learn_freeze(L);    // burn-in already over: never train
+    return BMG_OK;
+  }
+  const uint64_t every = (uint64_t)std::max(b->cfg.learn_every, 1);
+  int freeze = last || (b->cfg.learn_freeze_iter > 0 && b->iter >= (uint64_t)b->cfg.learn_freeze_iter);
+  if (!freeze && b->iter % every != 0 && b->iter / every == (b->iter - b->n_rao) / every) return BMG_OK;
+  std::vector<int> chains;
+  for (int c = 0; c < b->n_chains; ++c) if (b->chain_ds[c] == ds) chains.push_back(c);
+  int rc = learn_round(L, b->d_arrays, chains.data(), (int)chains.size(), b->stream);
+  if (rc) return rc;
+  if (learn_rounds(L) > 0 && !b->learn_on[ds]) {
+    rc = set_learn_on(b, ds, 1, b->cfg.p_learned);
+    if (rc) return rc;
+    b->learn_on[ds] = 1;
+  }
+  if (freeze) learn_freeze(L);
+  BMG_HIP(hipStreamSynchronize(b->stream));
+  return BMG_OK;
+}
+
 static int rb_block(bmg_batch* b) {
   std::vector<ChainScal> cs(b->n_chains);
   BMG_HIP(hipMemcpyAsync(cs.data(), b->d_scal, sizeof(ChainScal) * b->n_chains, hipMemcpyDeviceToHost,
@@ -499,8 +541,11 @@ static int rb_block(bmg_batch* b) {
       all.push_back(c);
       if (cs[c].rb_do_scan) scan_list.push_back(c);
     }
+    int learn_now = 0;
+    if (ds < b->learn.size() && b->learn[ds] && !learn_frozen(b->learn[ds]))
+      for (int c : scan_list) if (!cs[c].rb_accum_rao) learn_now = 1;
     if (!scan_list.empty()) {
-      int rc = rb_block_ds(b, (int)ds, scan_list, 1);
+      int rc = rb_block_ds(b, (int)ds, scan_list, learn_now);
       if (rc) return rc;
     }
     if (!all.empty()) {
@@ -517,6 +562,10 @@ static int rb_block(bmg_batch* b) {
         }
         BMG_HIP(hipStreamSynchronize(b->stream));
       }
+      if (ds < b->learn.size() && b->learn[ds] && !learn_frozen(b->learn[ds])) {
+        int rc = learn_after_block(b, (int)ds, all, cs);
+        if (rc) return rc;
+      }
     }
   }
   return BMG_OK;
@@ -528,6 +577,7 @@ extern "C" int bmg_batch_destroy(bmg_batch* b) {
   if (b->stream) hipStreamSynchronize(b->stream);
   hipFree(b->pool); hipFree(b->d_data); hipFree(b->d_consts); hipFree(b->d_scal); hipFree(b->d_arrays);
   for (float* w : b->lw_dev) hipFree(w);
+  for (LearnDS* L : b->learn) learn_destroy(L);
   hipFree(b->r); hipFree(b->xa); hipFree(b->xh); hipFree(b->rsum); hipFree(b->d_list);
   if (b->graph_exec) hipGraphExecDestroy(b->graph_exec);
   if (b->graph) hipGraphDestroy(b->graph);
@@ -703,6 +753,34 @@ static int batch_create(bmg_ctx** ctxs, int n_ctx, int n_chains, const int32_t* 
     BMG_HIP(hipMemcpyAsync(A.adapt, adapt.data(), sizeof(double) * 6 * 256, hipMemcpyHostToDevice, b->stream));
     BMG_HIP(hipStreamSynchronize(b->stream));
   }
+  // online learned proposal: one learner per dataset that has chains
+  b->learn.assign(n_ctx, nullptr);
+  b->learn_on.assign(n_ctx, 0);
+  if (b->cfg.proposal == 1) {
+    bmg_batch_cfg& bc = b->cfg;
+    if (bc.learn_window <= 0) bc.learn_window = 20;
+    if (bc.learn_steps <= 0) bc.learn_steps = 100;
+    if (bc.learn_batch <= 0) bc.learn_batch = 4096;
+    if (!(bc.learn_rate > 0)) bc.learn_rate = 1e-3;
+    if (!(bc.p_learned > 0)) bc.p_learned = 0.3;
+    if (bc.learn_window > BMG_LH_MAX || bc.p_learned > 1.0 || bc.learn_batch > (1 << 20)) {
+      bmg_batch_destroy(b); set_error("learned proposal: window <= 32, p_learned <= 1, batch <= 2^20");
+      return BMG_E_ARG;
+    }
+    for (int ds = 0; ds < n_ctx; ++ds) {
+      int nc = 0;
+      for (int c = 0; c < n_chains; ++c) nc += chain_ctx[c] == ds;
+      if (!nc) continue;
+      LearnCfg lc{bc.learn_window, bc.learn_steps, bc.learn_batch, (float)bc.learn_rate,
+                  opts[0].seed * 7919u + (uint32_t)ds};
+      int rc = learn_create(&b->learn[ds], ctxs[ds], b->h_data[ds], lc, nc, b->stream);
+      if (rc) { bmg_batch_destroy(b); return rc; }
+      b->h_data[ds].lw = learn_table(b->learn[ds]);
+      b->h_data[ds].lH = bc.learn_window;
+    }
+  } else if (b->cfg.proposal != 0) {
+    bmg_batch_destroy(b); set_error("proposal must be 0 (plain) or 1 (learned)"); return BMG_E_ARG;
+  }
   BMG_HIP(hipMalloc(&b->d_data, sizeof(DevData) * n_ctx));
   BMG_HIP(hipMalloc(&b->d_consts, sizeof(ChainConst) * n_chains));
   BMG_HIP(hipMalloc(&b->d_scal, sizeof(ChainScal) * n_chains));
@@ -797,14 +875,16 @@ extern "C" int bmg_batch_run(bmg_batch* b, uint64_t n_iter) {
 
 // learned swap on/off for the chains of dataset ds (see move_learned_swap)
 static int set_learn_on(bmg_batch* b, int ds, int on, double p_learned) {
+  const int32_t v = on;
   for (int c = 0; c < b->n_chains; ++c) {
     if (b->chain_ds[c] != ds) continue;
     b->h_consts[c].p_learned = p_learned;
-    int32_t v = on;
-    BMG_HIP(hipMemcpy((char*)(b->d_scal + c) + offsetof(ChainScal, learn_on), &v, sizeof(v),
-                      hipMemcpyHostToDevice));
+    BMG_HIP(hipMemcpyAsync((char*)(b->d_scal + c) + offsetof(ChainScal, learn_on), &v, sizeof(v),
+                           hipMemcpyHostToDevice, b->stream));
   }
-  BMG_HIP(hipMemcpy(b->d_consts, b->h_consts.data(), sizeof(ChainConst) * b->n_chains, hipMemcpyHostToDevice));
+  BMG_HIP(hipMemcpyAsync(b->d_consts, b->h_consts.data(), sizeof(ChainConst) * b->n_chains,
+                         hipMemcpyHostToDevice, b->stream));
+  BMG_HIP(hipStreamSynchronize(b->stream));
   return BMG_OK;
 }
 
@@ -812,6 +892,9 @@ extern "C" int bmg_batch_set_learned(bmg_batch* b, int ds, const float* w, int H
   if (!b || ds < 0 || ds >= (int)b->ctxs.size()) { set_error("bad batch or dataset"); return BMG_E_ARG; }
   if (H < 1 || H > BMG_LH_MAX || !(p_learned >= 0.0 && p_learned <= 1.0)) {
     set_error("learned swap: 1 <= H <= 32 and 0 <= p_learned <= 1"); return BMG_E_ARG;
+  }
+  if (ds < (int)b->learn.size() && b->learn[ds]) {
+    set_error("this dataset's table is learned online (proposal = 1)"); return BMG_E_STATE;
   }
   BMG_HIP(hipSetDevice(b->device));
   BMG_HIP(hipStreamSynchronize(b->stream));
@@ -827,6 +910,67 @@ extern "C" int bmg_batch_set_learned(bmg_batch* b, int ds, const float* w, int H
   D.lw = b->lw_dev[ds]; D.lH = H;
   BMG_HIP(hipMemcpy(b->d_data, b->h_data.data(), sizeof(DevData) * b->h_data.size(), hipMemcpyHostToDevice));
   return set_learn_on(b, ds, 1, p_learned);
+}
+
+static LearnDS* learner(bmg_batch* b, int ds) {
+  if (!b || ds < 0 || ds >= (int)b->learn.size() || !b->learn[ds]) {
+    set_error("no learned proposal for this dataset (proposal = 1 at create)");
+    return nullptr;
+  }
+  hipSetDevice(b->device);
+  return b->learn[ds];
+}
+
+extern "C" int bmg_batch_learn_state(bmg_batch* b, int ds, double out[4]) {
+  LearnDS* L = learner(b, ds);
+  if (!L || !out) return BMG_E_ARG;
+  int64_t cnt = 0;
+  int rc = learn_samples(L, nullptr, nullptr, nullptr, 0, &cnt, b->stream);
+  if (rc) return rc;
+  out[0] = (double)learn_rounds(L); out[1] = learn_frozen(L); out[2] = (double)cnt; out[3] = learn_H(L);
+  return BMG_OK;
+}
+
+extern "C" int bmg_batch_learn_table(bmg_batch* b, int ds, float* w) {
+  LearnDS* L = learner(b, ds);
+  if (!L || !w) return BMG_E_ARG;
+  BMG_HIP(hipStreamSynchronize(b->stream));
+  BMG_HIP(hipMemcpy(w, learn_table(L), sizeof(float) * b->h_data[ds].p * 2 * learn_H(L), hipMemcpyDeviceToHost));
+  return BMG_OK;
+}
+
+extern "C" int bmg_batch_learn_samples(bmg_batch* b, int ds, uint32_t* j, int32_t* off, float* label,
+                                       int64_t cap, int64_t* count) {
+  LearnDS* L = learner(b, ds);
+  if (!L || !count) return BMG_E_ARG;
+  return learn_samples(L, j, off, label, cap, count, b->stream);
+}
+
+extern "C" int bmg_batch_learn_params(bmg_batch* b, int ds, float* params) {
+  LearnDS* L = learner(b, ds);
+  if (!L || !params) return BMG_E_ARG;
+  return learn_params(L, params, b->stream);
+}
+
+extern "C" int bmg_batch_learn_static(bmg_batch* b, int ds, float* z, float* pbar, int32_t* band) {
+  LearnDS* L = learner(b, ds);
+  if (!L) return BMG_E_ARG;
+  return learn_static(L, z, pbar, band, b->stream);
+}
+
+extern "C" int bmg_batch_learn_features(bmg_batch* b, int ds, const uint32_t* j, const int32_t* off,
+                                        int64_t rows, float* out) {
+  LearnDS* L = learner(b, ds);
+  if (!L || !j || !off || !out || rows < 1) return BMG_E_ARG;
+  const uint64_t p = b->h_data[ds].p;
+  const int H = learn_H(L);
+  for (int64_t i = 0; i < rows; ++i) {
+    int64_t k = (int64_t)j[i] + off[i];
+    if (j[i] >= p || off[i] == 0 || off[i] < -H || off[i] > H || k < 0 || k >= (int64_t)p) {
+      set_error("feature rows need 0 < |offset| <= H with both SNPs inside the genome"); return BMG_E_ARG;
+    }
+  }
+  return learn_features(L, j, off, rows, out, b->stream);
 }
 
 extern "C" int bmg_batch_sync(bmg_batch* b) {

@@ -1,0 +1,431 @@
+// bmg_learn.hip -- online training of the learned swap proposal.
+//
+// Per dataset: the band X^T X over the window (int8 MFMA, exact), the
+// marginal score of every SNP, a ring of swap labels filled from the burn-in
+// RB scans, the network (bmg_mlp.hip) and the partner-weight table the chain
+// kernel draws from.  Everything here runs between chain segments, on the
+// batch's stream, and only during burn-in; the table is frozen before the
+// first kept sample.
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include <algorithm>
+#include <random>
+#include <vector>
+
+#include "bmg_device.h"
+#include "bmg_learn.h"
+#include "bmg_mlp.h"
+
+namespace bmg {
+
+#define LEARN_RING (1u << 19)
+#define LEARN_CHUNK (1 << 20)
+
+struct LearnView {
+  const double* pre;   // p x offset self moments
+  int off, ot;         // offset, offset of the sampler's type
+  uint64_t p;
+  uint32_t n;
+  int H;
+  const int32_t* band; // p x (H + 1): sum_i x_j x_{j+d}
+  const float* z;      // marginal scores
+  const float* pbar;   // mean p_proposal of the dataset's chains
+};
+
+struct LearnDS {
+  LearnCfg cfg{};
+  LearnView v{};
+  int type = 0;
+  int32_t* band = nullptr;
+  float* z = nullptr;
+  float* pbar = nullptr;
+  uint32_t* rj = nullptr;
+  int32_t* ro = nullptr;
+  float* rl = nullptr;
+  unsigned long long* counter = nullptr;
+  MlpArgs mlp{};
+  uint16_t* Xb = nullptr;
+  float* lb = nullptr;
+  uint16_t* Xc = nullptr;
+  float* oc = nullptr;
+  float* table = nullptr;
+  int* d_chains = nullptr;
+  int n_chains = 0;
+  int adam_t = 0;
+  uint64_t rounds = 0;
+  int frozen = 0;
+};
+
+__device__ __forceinline__ int learn_off(int H, int c) { return c < H ? c - H : c - H + 1; }
+
+// Features of the pair (j, k = j + o), k inside the genome.
+__device__ void pair_features(const LearnView& L, uint64_t j, int o, float* f) {
+  const uint64_t k = (uint64_t)((int64_t)j + o);
+  const double n = (double)L.n;
+  const double sj = L.pre[j * L.off + L.ot], cj = L.pre[j * L.off + L.ot + 1];
+  const double sk = L.pre[k * L.off + L.ot], ck = L.pre[k * L.off + L.ot + 1];
+  const uint64_t lo = j < k ? j : k;
+  const int d = o < 0 ? -o : o;
+  const double cov = (double)L.band[lo * (L.H + 1) + d] - sj * sk / n;
+  double r = (cj > 0.0 && ck > 0.0) ? cov / sqrt(cj * ck) : 0.0;
+  r = fmin(1.0, fmax(-1.0, r));
+  const float rf = (float)r;
+  const float zj = L.z[j], zk = L.z[k];
+  const float pj = fminf(fmaxf(L.pbar[j], 1e-6f), 1.0f - 1e-6f);
+  const float pk = fminf(fmaxf(L.pbar[k], 1e-6f), 1.0f - 1e-6f);
+  const float lj = fminf(fmaxf(logf(pj / (1.0f - pj)), -10.0f), 10.0f) / 5.0f;
+  const float lk = fminf(fmaxf(logf(pk / (1.0f - pk)), -10.0f), 10.0f) / 5.0f;
+  const float zc = (zk - rf * zj) / sqrtf(fmaxf(1.0f - rf * rf, 1e-3f));
+  const float zcs = fminf(fmaxf(zc / 10.0f, -5.0f), 5.0f);
+  f[0] = rf;
+  f[1] = rf * rf;
+  f[2] = zj / 10.0f;
+  f[3] = zk / 10.0f;
+  f[4] = fminf(zj * zj / 100.0f, 25.0f);
+  f[5] = fminf(zk * zk / 100.0f, 25.0f);
+  f[6] = fminf(fmaxf(rf * zj * zk / 100.0f, -25.0f), 25.0f);
+  f[7] = (float)d / (float)L.H;
+  f[8] = (float)(sj / n) * 0.5f;
+  f[9] = (float)(sk / n) * 0.5f;
+  f[10] = lj;
+  f[11] = lk;
+  f[12] = zcs;
+  f[13] = zcs * zcs;
+  f[14] = o > 0 ? 1.0f : -1.0f;
+  f[15] = 1.0f;
+}
+
+__device__ __forceinline__ uint16_t f2bf_l(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7F800000u) == 0x7F800000u && (u & 0x007FFFFFu)) return 0x7FC0u;
+  u = u + 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+__device__ __forceinline__ void store_row(uint16_t* X, int64_t row, const float* f) {
+  uint16_t h[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) h[i] = f2bf_l(f[i]);
+  uint4 a, b;
+  __builtin_memcpy(&a, h, 16);
+  __builtin_memcpy(&b, h + 8, 16);
+  *(uint4*)(X + row * 16) = a;
+  *(uint4*)(X + row * 16 + 8) = b;
+}
+
+// z_j = corr(x_j, y) sqrt(n), clipped to +-50; rx = x_j . (y - mean y)
+__global__ void k_learn_static(LearnView L, const double* __restrict__ rx, double syy, float* __restrict__ z) {
+  uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (j >= L.p) return;
+  double css = L.pre[j * L.off + L.ot + 1];
+  double c = (css > 0.0 && syy > 0.0) ? rx[j] / sqrt(css * syy) : 0.0;
+  double v = c * sqrt((double)L.n);
+  z[j] = (float)fmin(50.0, fmax(-50.0, v));
+}
+
+__device__ __forceinline__ double lbf(double rx, double css, double d, double s2) {
+  double q = css + d;
+  return rx * rx / (2.0 * s2 * q) - 0.5 * log(q / d);
+}
+
+// one workgroup per (model index, chain of the group); one lane per offset
+__global__ void k_learn_labels(LearnView L, int type, const DevData* __restrict__ data,
+                               const ChainConst* __restrict__ consts, const ChainScal* __restrict__ scal,
+                               const ChainArrays* __restrict__ arrays, const int* __restrict__ list, int kmax,
+                               const double* __restrict__ xa, const double* __restrict__ rsum,
+                               uint32_t* __restrict__ rj, int32_t* __restrict__ ro, float* __restrict__ rl,
+                               unsigned long long* __restrict__ counter) {
+  const int v = blockIdx.y;
+  const int c = list[v];
+  const ChainScal& CS = scal[c];
+  const ChainArrays& A = arrays[c];
+  const int m = blockIdx.x;
+  if (m >= CS.nloci || (int)threadIdx.x >= 2 * L.H) return;
+  const uint64_t j = (uint64_t)A.idx[m];
+  const int o = learn_off(L.H, threadIdx.x);
+  const int64_t k = (int64_t)j + o;
+  if (k < 0 || k >= (int64_t)L.p) return;
+  if (A.model_inds[k] >= 0) return;
+  const double n = (double)L.n;
+  const double mr = rsum[v] / n;
+  const double sj = L.pre[j * L.off + L.ot], cj = L.pre[j * L.off + L.ot + 1];
+  const double sk = L.pre[(uint64_t)k * L.off + L.ot], ck = L.pre[(uint64_t)k * L.off + L.ot + 1];
+  const int x1 = A.idx[kmax + m];
+  const double bj = A.vec[3 * kmax + x1];
+  const uint64_t lo = (uint64_t)k < j ? (uint64_t)k : j;
+  const double cov = (double)L.band[lo * (L.H + 1) + (o < 0 ? -o : o)] - sj * sk / n;
+  const double rxj = xa[(size_t)v * L.p + j] - sj * mr + bj * cj;
+  const double rxk = xa[(size_t)v * L.p + (uint64_t)k] - sk * mr + bj * cov;
+  const double d = CS.inv_tau2_alpha2[type];
+  const double s2 = CS.sigma2;
+  if (!(cj > 0.0) || !(ck > 0.0) || !(d > 0.0) || !(s2 > 0.0)) return;
+  double lab = lbf(rxk, ck, d, s2) - lbf(rxj, cj, d, s2);
+  if (!isfinite(lab)) return;
+  lab = fmin(30.0, fmax(-30.0, lab));
+  unsigned long long slot = atomicAdd(counter, 1ull) % LEARN_RING;
+  rj[slot] = (uint32_t)j;
+  ro[slot] = o;
+  rl[slot] = (float)lab;
+}
+
+__global__ void k_learn_pbar(const ChainArrays* __restrict__ arrays, const int* __restrict__ chains, int nc,
+                             uint64_t p, float* __restrict__ pbar) {
+  uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (j >= p) return;
+  double s = 0.0;
+  for (int i = 0; i < nc; ++i) s += arrays[chains[i]].p_proposal[j];
+  pbar[j] = (float)(s / (double)nc);
+}
+
+// minibatch: rows drawn uniformly from the filled part of the ring
+__global__ void k_learn_batch(LearnView L, const uint32_t* __restrict__ rj, const int32_t* __restrict__ ro,
+                              const float* __restrict__ rl, uint64_t count, uint32_t seed, uint32_t round,
+                              uint32_t step, int rows, uint16_t* __restrict__ X, float* __restrict__ lab) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= rows) return;
+  Rng r = rng_make(seed, 0x1EA2Du, round, step);
+  r.c0 = (uint32_t)b;
+  uint32_t o4[4];
+  rng_block(r, o4);
+  uint64_t u = ((uint64_t)o4[0] << 32) | o4[1];
+  uint64_t s = u % count;
+  float f[16];
+  pair_features(L, rj[s], ro[s], f);
+  store_row(X, b, f);
+  lab[b] = rl[s];
+}
+
+// features of table rows [r0, r0 + rows): row = j * 2H + c
+__global__ void k_learn_table_feat(LearnView L, uint64_t r0, int64_t rows, uint16_t* __restrict__ X) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= rows) return;
+  uint64_t row = r0 + (uint64_t)i;
+  uint64_t j = row / (uint64_t)(2 * L.H);
+  int o = learn_off(L.H, (int)(row % (uint64_t)(2 * L.H)));
+  int64_t k = (int64_t)j + o;
+  float f[16];
+  if (k >= 0 && k < (int64_t)L.p) {
+    pair_features(L, j, o, f);
+  } else {
+#pragma unroll
+    for (int q = 0; q < 16; ++q) f[q] = 0.0f;
+  }
+  store_row(X, i, f);
+}
+
+__global__ void k_learn_table_store(LearnView L, uint64_t r0, int64_t rows, const float* __restrict__ out,
+                                    float* __restrict__ table) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= rows) return;
+  uint64_t row = r0 + (uint64_t)i;
+  uint64_t j = row / (uint64_t)(2 * L.H);
+  int o = learn_off(L.H, (int)(row % (uint64_t)(2 * L.H)));
+  int64_t k = (int64_t)j + o;
+  float w = 0.0f;
+  if (k >= 0 && k < (int64_t)L.p) {
+    float v = out[i];
+    v = isfinite(v) ? fminf(30.0f, fmaxf(-30.0f, v)) : 0.0f;
+    w = expf(0.5f * v);
+  }
+  table[row] = w;
+}
+
+__global__ void k_learn_features_rows(LearnView L, const uint32_t* __restrict__ js, const int32_t* __restrict__ os,
+                                      int64_t rows, float* __restrict__ out) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= rows) return;
+  float f[16];
+  pair_features(L, js[i], os[i], f);
+  for (int q = 0; q < 16; ++q) out[i * 16 + q] = f[q];
+}
+
+static unsigned blocks(int64_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }
+
+int learn_create(LearnDS** out, bmg_ctx* ctx, const DevData& D, const LearnCfg& cfg, int n_chains,
+                 hipStream_t s) {
+  *out = nullptr;
+  if (cfg.H < 1 || cfg.H > BMG_LH_MAX) { set_error("learned proposal: window must be 1..32"); return BMG_E_ARG; }
+  if (D.type > 3) { set_error("learned proposal: one coding (A, H, D or R) only"); return BMG_E_UNSUPPORTED; }
+  if (cfg.batch < 64 || cfg.steps < 0 || !(cfg.lr > 0.0f)) { set_error("learned proposal: bad training settings"); return BMG_E_ARG; }
+  LearnDS* L = new LearnDS();
+  L->cfg = cfg;
+  L->type = D.type;
+  L->n_chains = n_chains;
+  const uint64_t p = D.p;
+  auto fail = [&](int rc) { learn_destroy(L); return rc; };
+  if (hipMalloc(&L->band, sizeof(int32_t) * p * (cfg.H + 1)) != hipSuccess ||
+      hipMalloc(&L->z, sizeof(float) * p) != hipSuccess || hipMalloc(&L->pbar, sizeof(float) * p) != hipSuccess ||
+      hipMalloc(&L->rj, sizeof(uint32_t) * LEARN_RING) != hipSuccess ||
+      hipMalloc(&L->ro, sizeof(int32_t) * LEARN_RING) != hipSuccess ||
+      hipMalloc(&L->rl, sizeof(float) * LEARN_RING) != hipSuccess ||
+      hipMalloc(&L->counter, sizeof(unsigned long long)) != hipSuccess ||
+      hipMalloc(&L->Xb, sizeof(uint16_t) * 16 * cfg.batch) != hipSuccess ||
+      hipMalloc(&L->lb, sizeof(float) * cfg.batch) != hipSuccess ||
+      hipMalloc(&L->Xc, sizeof(uint16_t) * 16 * LEARN_CHUNK) != hipSuccess ||
+      hipMalloc(&L->oc, sizeof(float) * LEARN_CHUNK) != hipSuccess ||
+      hipMalloc(&L->table, sizeof(float) * p * 2 * cfg.H) != hipSuccess ||
+      hipMalloc(&L->d_chains, sizeof(int) * std::max(n_chains, 1)) != hipSuccess) {
+    set_error("learned proposal: device allocation failed");
+    return fail(BMG_E_HIP);
+  }
+  int rc = mlp_alloc(&L->mlp);
+  if (rc) return fail(rc);
+  L->v = LearnView{D.pre, D.offset, D.ot[D.type], p, D.n, cfg.H, L->band, L->z, L->pbar};
+  // band X^T X over the window, exact int32 (int8 MFMA)
+  rc = launch_band(ctx, (uint32_t)(cfg.H + 1), D.type, D.type, L->band, s);
+  if (rc) return fail(rc);
+  // marginal scores from one scan of y - mean(y)
+  std::vector<double> r(ctx->n_pad, 0.0);
+  double my = 0.0, syy = 0.0;
+  for (uint32_t i = 0; i < ctx->n; ++i) my += ctx->y_host[i];
+  my /= (double)ctx->n;
+  for (uint32_t i = 0; i < ctx->n; ++i) { r[i] = ctx->y_host[i] - my; syy += r[i] * r[i]; }
+  double *rd = nullptr, *rx = nullptr;
+  if (hipMalloc(&rd, sizeof(double) * ctx->n_pad) != hipSuccess || hipMalloc(&rx, sizeof(double) * p) != hipSuccess) {
+    hipFree(rd); set_error("learned proposal: device allocation failed"); return fail(BMG_E_HIP);
+  }
+  hipMemcpyAsync(rd, r.data(), sizeof(double) * ctx->n_pad, hipMemcpyHostToDevice, s);
+  rc = launch_scan(ctx, rd, 1, D.type, 5, rx, s);
+  if (!rc) {
+    hipLaunchKernelGGL(k_learn_static, dim3(blocks((int64_t)p)), dim3(256), 0, s, L->v, rx, syy, L->z);
+    if (hipGetLastError() != hipSuccess) { set_error("learned proposal: static kernel"); rc = BMG_E_HIP; }
+  }
+  if (!rc && hipStreamSynchronize(s) != hipSuccess) { set_error("learned proposal: set-up failed"); rc = BMG_E_HIP; }
+  hipFree(rd); hipFree(rx);
+  if (rc) return fail(rc);
+  hipMemsetAsync(L->counter, 0, sizeof(unsigned long long), s);
+  std::vector<float> flat(p, 0.0f);
+  hipMemcpyAsync(L->pbar, flat.data(), sizeof(float) * p, hipMemcpyHostToDevice, s);
+  // He-initialised weights (fixed seed), small last layer
+  std::vector<float> P(MLP_NPAR, 0.0f);
+  std::mt19937_64 gen(0x5EEDull * 1000003ull + cfg.seed);
+  std::normal_distribution<float> nd(0.0f, 1.0f);
+  for (int i = 0; i < MLP_IN * MLP_HID; ++i) P[MLP_OFF_W1 + i] = nd(gen) * sqrtf(2.0f / MLP_IN);
+  for (int i = 0; i < MLP_HID * MLP_HID; ++i) P[MLP_OFF_W2 + i] = nd(gen) * sqrtf(2.0f / MLP_HID);
+  for (int i = 0; i < MLP_HID; ++i) P[MLP_OFF_W3 + i] = nd(gen) * 0.05f;
+  hipMemcpyAsync(L->mlp.P, P.data(), sizeof(float) * MLP_NPAR, hipMemcpyHostToDevice, s);
+  rc = mlp_launch_copies(L->mlp, s);
+  if (!rc && hipStreamSynchronize(s) != hipSuccess) { set_error("learned proposal: set-up failed"); rc = BMG_E_HIP; }
+  if (rc) return fail(rc);
+  *out = L;
+  return BMG_OK;
+}
+
+void learn_destroy(LearnDS* L) {
+  if (!L) return;
+  hipFree(L->band); hipFree(L->z); hipFree(L->pbar); hipFree(L->rj); hipFree(L->ro); hipFree(L->rl);
+  hipFree(L->counter); hipFree(L->Xb); hipFree(L->lb); hipFree(L->Xc); hipFree(L->oc); hipFree(L->table);
+  hipFree(L->d_chains);
+  mlp_free(&L->mlp);
+  delete L;
+}
+
+float* learn_table(LearnDS* L) { return L->table; }
+int learn_H(const LearnDS* L) { return L->cfg.H; }
+int learn_frozen(const LearnDS* L) { return L->frozen; }
+void learn_freeze(LearnDS* L) { L->frozen = 1; }
+uint64_t learn_rounds(const LearnDS* L) { return L->rounds; }
+
+int learn_labels(LearnDS* L, const DevData* d_data, const ChainConst* consts, const ChainScal* scal,
+                 const ChainArrays* arrays, const int* d_list, int nv, int kmax, const double* xa,
+                 const double* rsum, hipStream_t s) {
+  if (L->frozen || nv <= 0) return BMG_OK;
+  hipLaunchKernelGGL(k_learn_labels, dim3(kmax, nv), dim3(64), 0, s, L->v, L->type, d_data, consts, scal, arrays,
+                     d_list, kmax, xa, rsum, L->rj, L->ro, L->rl, L->counter);
+  BMG_HIP(hipGetLastError());
+  return BMG_OK;
+}
+
+int learn_round(LearnDS* L, const ChainArrays* d_arrays, const int* chains_host, int n, hipStream_t s) {
+  if (L->frozen) return BMG_OK;
+  const uint64_t p = L->v.p;
+  BMG_HIP(hipMemcpyAsync(L->d_chains, chains_host, sizeof(int) * n, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_learn_pbar, dim3(blocks((int64_t)p)), dim3(256), 0, s, d_arrays, L->d_chains, n, p, L->pbar);
+  BMG_HIP(hipGetLastError());
+  unsigned long long cnt = 0;
+  BMG_HIP(hipMemcpyAsync(&cnt, L->counter, sizeof(cnt), hipMemcpyDeviceToHost, s));
+  BMG_HIP(hipStreamSynchronize(s));
+  uint64_t count = std::min<unsigned long long>(cnt, LEARN_RING);
+  if (count >= 64) {
+    for (int st = 0; st < L->cfg.steps; ++st) {
+      hipLaunchKernelGGL(k_learn_batch, dim3(blocks(L->cfg.batch)), dim3(256), 0, s, L->v, L->rj, L->ro, L->rl,
+                         count, L->cfg.seed, (uint32_t)L->rounds, (uint32_t)st, L->cfg.batch, L->Xb, L->lb);
+      BMG_HIP(hipGetLastError());
+      int rc = mlp_launch_train(L->mlp, L->Xb, L->lb, L->cfg.batch, nullptr, s);
+      if (rc) return rc;
+      rc = mlp_launch_adam(L->mlp, L->cfg.lr, ++L->adam_t, s);
+      if (rc) return rc;
+    }
+  }
+  // the table over every (j, offset) pair
+  const uint64_t total = p * (uint64_t)(2 * L->cfg.H);
+  for (uint64_t r0 = 0; r0 < total; r0 += LEARN_CHUNK) {
+    int64_t rows = (int64_t)std::min<uint64_t>(LEARN_CHUNK, total - r0);
+    hipLaunchKernelGGL(k_learn_table_feat, dim3(blocks(rows)), dim3(256), 0, s, L->v, r0, rows, L->Xc);
+    BMG_HIP(hipGetLastError());
+    int rc = mlp_launch_forward(L->mlp, L->Xc, rows, L->oc, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_learn_table_store, dim3(blocks(rows)), dim3(256), 0, s, L->v, r0, rows, L->oc, L->table);
+    BMG_HIP(hipGetLastError());
+  }
+  L->rounds++;
+  return BMG_OK;
+}
+
+int learn_features(LearnDS* L, const uint32_t* j_host, const int32_t* off_host, int64_t rows, float* out,
+                   hipStream_t s) {
+  uint32_t* dj = nullptr;
+  int32_t* dof = nullptr;
+  float* dout = nullptr;
+  int rc = BMG_OK;
+  if (hipMalloc(&dj, sizeof(uint32_t) * rows) != hipSuccess || hipMalloc(&dof, sizeof(int32_t) * rows) != hipSuccess ||
+      hipMalloc(&dout, sizeof(float) * 16 * rows) != hipSuccess) {
+    set_error("allocation failed"); rc = BMG_E_HIP;
+  }
+  if (!rc) {
+    hipMemcpyAsync(dj, j_host, sizeof(uint32_t) * rows, hipMemcpyHostToDevice, s);
+    hipMemcpyAsync(dof, off_host, sizeof(int32_t) * rows, hipMemcpyHostToDevice, s);
+    hipLaunchKernelGGL(k_learn_features_rows, dim3(blocks(rows)), dim3(256), 0, s, L->v, dj, dof, rows, dout);
+    if (hipGetLastError() != hipSuccess) { set_error("features kernel"); rc = BMG_E_HIP; }
+  }
+  if (!rc) {
+    hipMemcpyAsync(out, dout, sizeof(float) * 16 * rows, hipMemcpyDeviceToHost, s);
+    if (hipStreamSynchronize(s) != hipSuccess) { set_error("features failed"); rc = BMG_E_HIP; }
+  }
+  hipFree(dj); hipFree(dof); hipFree(dout);
+  return rc;
+}
+
+int learn_samples(LearnDS* L, uint32_t* j, int32_t* off, float* label, int64_t cap, int64_t* count,
+                  hipStream_t s) {
+  unsigned long long cnt = 0;
+  BMG_HIP(hipMemcpyAsync(&cnt, L->counter, sizeof(cnt), hipMemcpyDeviceToHost, s));
+  BMG_HIP(hipStreamSynchronize(s));
+  int64_t k = (int64_t)std::min<unsigned long long>(cnt, LEARN_RING);
+  *count = (int64_t)cnt;
+  int64_t m = std::min(k, cap);
+  if (m > 0) {
+    if (j) BMG_HIP(hipMemcpy(j, L->rj, sizeof(uint32_t) * m, hipMemcpyDeviceToHost));
+    if (off) BMG_HIP(hipMemcpy(off, L->ro, sizeof(int32_t) * m, hipMemcpyDeviceToHost));
+    if (label) BMG_HIP(hipMemcpy(label, L->rl, sizeof(float) * m, hipMemcpyDeviceToHost));
+  }
+  return BMG_OK;
+}
+
+int learn_params(LearnDS* L, float* params, hipStream_t s) {
+  BMG_HIP(hipStreamSynchronize(s));
+  BMG_HIP(hipMemcpy(params, L->mlp.P, sizeof(float) * MLP_NPAR, hipMemcpyDeviceToHost));
+  return BMG_OK;
+}
+
+int learn_static(LearnDS* L, float* z, float* pbar, int32_t* band, hipStream_t s) {
+  BMG_HIP(hipStreamSynchronize(s));
+  const uint64_t p = L->v.p;
+  if (z) BMG_HIP(hipMemcpy(z, L->z, sizeof(float) * p, hipMemcpyDeviceToHost));
+  if (pbar) BMG_HIP(hipMemcpy(pbar, L->pbar, sizeof(float) * p, hipMemcpyDeviceToHost));
+  if (band) BMG_HIP(hipMemcpy(band, L->band, sizeof(int32_t) * p * (L->cfg.H + 1), hipMemcpyDeviceToHost));
+  return BMG_OK;
+}
+
+}  // namespace bmg

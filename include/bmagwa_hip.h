@@ -185,12 +185,20 @@ int bmg_model_set_d(bmg_model* m, int x_ind, double val);
  * multiple effect types, covariates, missing genotypes. */
 typedef struct {
   int32_t trace_every_iter;    /* keep per-iteration move/jd/size/loglik */
-  int32_t proposal;            /* 0 plain (RB running mean), 1 learned */
-  int32_t learn_every;         /* learned proposal: retrain period (iters) */
-  int32_t learn_freeze_iter;   /* learned proposal: frozen after this iter */
+  int32_t proposal;            /* 0 plain, 1 plain + learned swap (online) */
+  int32_t learn_every;         /* learned: training period in iterations
+                                  (rounded to RB blocks, burn-in only) */
+  int32_t learn_freeze_iter;   /* learned: freeze at the first RB block at or
+                                  after this iteration (0: end of burn-in) */
   int32_t scan_digits;         /* fixed-point digits of the RB scan (4..5) */
   int32_t use_graph;           /* replay segments as a captured hipGraph */
   uint64_t trace_capacity;     /* thinned loci entries kept per chain */
+  int32_t learn_window;        /* learned: partners within this many SNPs */
+  int32_t learn_steps;         /* learned: Adam steps per training round */
+  int32_t learn_batch;         /* learned: rows per Adam step */
+  int32_t learn_pad;
+  double  p_learned;           /* learned: probability of a learned swap */
+  double  learn_rate;          /* learned: Adam learning rate */
 } bmg_batch_cfg;
 
 int bmg_batch_create(bmg_ctx** ctxs, int n_ctx, int n_chains, const int32_t* chain_ctx,
@@ -210,6 +218,19 @@ int bmg_batch_sync(bmg_batch* b);
  * within H of j outside the model, accept with the exact Metropolis-Hastings
  * ratio.  w = NULL switches it off.  1 <= H <= 32. */
 int bmg_batch_set_learned(bmg_batch* b, int ds, const float* w, int H, double p_learned);
+/* The online learned proposal (proposal = 1): state of dataset ds's learner
+ * (out: training rounds, frozen flag, labels collected, window H), its
+ * partner-weight table (p x 2H), label ring, network parameters, static
+ * per-SNP inputs (marginal scores z, mean p_proposal, band X^T X p x (H+1))
+ * and the features of given (j, offset) pairs (rows x 16). */
+int bmg_batch_learn_state(bmg_batch* b, int ds, double out[4]);
+int bmg_batch_learn_table(bmg_batch* b, int ds, float* w);
+int bmg_batch_learn_samples(bmg_batch* b, int ds, uint32_t* j, int32_t* off, float* label,
+                            int64_t cap, int64_t* count);
+int bmg_batch_learn_params(bmg_batch* b, int ds, float* params);
+int bmg_batch_learn_static(bmg_batch* b, int ds, float* z, float* pbar, int32_t* band);
+int bmg_batch_learn_features(bmg_batch* b, int ds, const uint32_t* j, const int32_t* off,
+                             int64_t rows, float* out);
 /* per-chain state: loglik, size, sigma2, alpha, p_move_size, n_accepted,
  * n_iter, loci_len */
 int bmg_batch_state(bmg_batch* b, int chain, double out[8]);

@@ -131,6 +131,11 @@ def load():
     _sig(lib, "orc_mlp_gradients", C.c_double, _F, _F, _F, C.c_int64, _D)
     _sig(lib, "orc_mlp_loss_exact", C.c_double, _F, _F, _F, C.c_int64, _D)
     _sig(lib, "orc_mlp_adam", None, _F, _F, _F, _F, C.c_float, C.c_int)
+    _sig(lib, "orc_learn_pair_features", None, _D, C.c_int, C.c_int, C.c_uint32,
+         C.POINTER(C.c_int32), C.c_int, _F, _F, C.c_uint64, C.c_int, _F)
+    _sig(lib, "orc_sampler_marginal_z", None, _P, _F)
+    _sig(lib, "orc_sampler_swap_labels", C.c_int64, _P, C.c_int, C.POINTER(C.c_uint32),
+         C.POINTER(C.c_int32), _F, C.c_int64)
     _sig(lib, "orc_sampler_iterate", C.c_int, _P, C.c_uint64)
     _sig(lib, "orc_sampler_p_rao", _D, _P)
     _sig(lib, "orc_sampler_p_proposal", _D, _P)
@@ -355,6 +360,18 @@ class Sampler:
     def setup(self):
         return lib().orc_sampler_setup(self.h)
 
+    def marginal_z(self):
+        z = np.zeros(self.data.p, np.float32)
+        lib().orc_sampler_marginal_z(self.h, ptr(z, C.c_float))
+        return z
+
+    def swap_labels(self, H, cap=1 << 20):
+        j = np.zeros(cap, np.uint32); o = np.zeros(cap, np.int32); lab = np.zeros(cap, np.float32)
+        k = lib().orc_sampler_swap_labels(self.h, H, ptr(j, C.c_uint32), ptr(o, C.c_int32),
+                                          ptr(lab, C.c_float), cap)
+        k = min(k, cap)
+        return j[:k], o[:k], lab[:k]
+
     def set_move_mix(self, p):
         a = np.ascontiguousarray(p, dtype=np.float64)
         return lib().orc_sampler_set_move_mix(self.h, ptr(a))
@@ -498,6 +515,18 @@ def mlp_adam(params, m, v, grads, lr, t):
     g = _f32(grads)
     lib().orc_mlp_adam(ptr(params, C.c_float), ptr(m, C.c_float), ptr(v, C.c_float),
                        ptr(g, C.c_float), C.c_float(lr), t)
+
+
+def learn_pair_features(pre, ot, n, band, H, z, pbar, j, o):
+    """Features of pair (j, j + o); pre is p x offset self moments."""
+    pre = np.ascontiguousarray(pre, dtype=np.float64)
+    band = np.ascontiguousarray(band, dtype=np.int32)
+    z = _f32(z); pbar = _f32(pbar)
+    f = np.zeros(16, np.float32)
+    lib().orc_learn_pair_features(ptr(pre), pre.shape[1], ot, n, ptr(band, C.c_int32), H,
+                                  ptr(z, C.c_float), ptr(pbar, C.c_float), int(j), int(o),
+                                  ptr(f, C.c_float))
+    return f
 
 
 def mlp_init(seed, scale3=0.1):

@@ -243,6 +243,13 @@ double orc_ess_geyer(const double* x, int64_t n);
 #define ORC_MLP_OFF_B3 (ORC_MLP_OFF_W3 + ORC_MLP_HID)
 #define ORC_MLP_NPAR (ORC_MLP_OFF_B3 + 1)
 float  orc_bf16_round(float f);
+/* learned-proposal inputs (see orc_learn.c / orc_sampler.c) */
+void   orc_learn_pair_features(const double* pre, int off, int ot, uint32_t n,
+                               const int32_t* band, int H, const float* z, const float* pbar,
+                               uint64_t j, int o, float* f);
+void   orc_sampler_marginal_z(orc_sampler* s, float* z);
+int64_t orc_sampler_swap_labels(orc_sampler* s, int H, uint32_t* oj, int32_t* oo, float* olab,
+                                int64_t cap);
 void   orc_mlp_forward(const float* params, const float* X, int64_t B, float* out);
 /* gradients of the mean Huber loss with the device's bf16 rounding points;
  * returns the loss */

@@ -1083,6 +1083,87 @@ static int do_learned_swap(orc_sampler* s)
   return 0;
 }
 
+/* Learned-proposal inputs from the sampler's current state (no reference
+ * counterpart).  Marginal scores z_j = corr(x_j, y) sqrt(n), clipped to
+ * +-50, from dense double dot products. */
+void orc_sampler_marginal_z(orc_sampler* s, float* z)
+{
+  size_t n = s->cur->n;
+  const double* y = orc__data_y(s->d);
+  double my = 0, syy = 0;
+  for (size_t i = 0; i < n; ++i) my += y[i];
+  my /= (double)n;
+  for (size_t i = 0; i < n; ++i) syy += (y[i] - my) * (y[i] - my);
+  int type = orc__dm_type(s->dm, 0);
+  int ot = s->pre_ot[type];
+  double* x = (double*)malloc(sizeof(double) * n);
+  for (size_t j = 0; j < s->m_g; ++j) {
+    orc_datamodel_get(s->dm, type, j, x);
+    double rx = 0;
+    for (size_t i = 0; i < n; ++i) rx += x[i] * (y[i] - my);
+    double css = s->pre_xx[j * s->pre_offset + ot + 1];
+    double c = (css > 0 && syy > 0) ? rx / sqrt(css * syy) : 0.0;
+    double v = c * sqrt((double)n);
+    z[j] = (float)(v > 50 ? 50 : (v < -50 ? -50 : v));
+  }
+  free(x);
+}
+
+/* Swap labels of the current model: for each model SNP j (locus order) and
+ * each SNP k = j + o (o = -H..-1, 1..H) inside the genome and outside the
+ * model,
+ *   label = lbf(k | gamma - j) - lbf(j | gamma - j),
+ *   lbf(rx, css) = rx^2 / (2 sigma2 (css + d)) - log((css + d) / d) / 2,
+ * where rx is the centred dot product of the SNP with the residual that
+ * excludes j's effect (y - y_hat + beta_j x_j, formed densely here) and d is
+ * the shared prior precision of the type; clipped to +-30.  Single coding
+ * (not AH) only.  Returns the number of labels (at most cap are written). */
+int64_t orc_sampler_swap_labels(orc_sampler* s, int H, uint32_t* oj, int32_t* oo, float* olab,
+                                int64_t cap)
+{
+  const orc_model* cm = s->cur;
+  size_t n = cm->n;
+  const double* y = orc__data_y(s->d);
+  double* r = (double*)malloc(sizeof(double) * n);
+  double* xk = (double*)malloc(sizeof(double) * n);
+  int64_t cnt = 0;
+  for (int m = 0; m < cm->nloci; ++m) {
+    int type = orc__dm_type(s->dm, cm->type_inds[m]);
+    if (type == ORC_AH) continue;
+    int64_t j = cm->loci[m];
+    unsigned xi = cm->x_ind1[m];
+    double bj = cm->beta[xi];
+    const double* xj = cm->x + (size_t)xi * n;
+    double mr = 0;
+    for (size_t i = 0; i < n; ++i) { r[i] = y[i] - cm->y_hat[i] + bj * xj[i]; mr += r[i]; }
+    mr /= (double)n;
+    double rxj = 0;
+    for (size_t i = 0; i < n; ++i) rxj += xj[i] * (r[i] - mr);
+    int ot = s->pre_ot[type];
+    double cj = s->pre_xx[(size_t)j * s->pre_offset + ot + 1];
+    double d = s->pr.inv_tau2_alpha2[type], s2 = cm->sigma2;
+    for (int c = 0; c < 2 * H; ++c) {
+      int o = c < H ? c - H : c - H + 1;
+      int64_t k = j + o;
+      if (k < 0 || k >= (int64_t)s->m_g || cm->model_inds[k] >= 0) continue;
+      double ck = s->pre_xx[(size_t)k * s->pre_offset + ot + 1];
+      if (!(cj > 0) || !(ck > 0) || !(d > 0) || !(s2 > 0)) continue;
+      orc_datamodel_get(s->dm, type, (size_t)k, xk);
+      double rxk = 0;
+      for (size_t i = 0; i < n; ++i) rxk += xk[i] * (r[i] - mr);
+      double lk = rxk * rxk / (2 * s2 * (ck + d)) - 0.5 * log((ck + d) / d);
+      double lj = rxj * rxj / (2 * s2 * (cj + d)) - 0.5 * log((cj + d) / d);
+      double lab = lk - lj;
+      if (!isfinite(lab)) continue;
+      lab = lab > 30 ? 30 : (lab < -30 ? -30 : lab);
+      if (cnt < cap) { oj[cnt] = (uint32_t)j; oo[cnt] = o; olab[cnt] = (float)lab; }
+      ++cnt;
+    }
+  }
+  free(r); free(xk);
+  return cnt;
+}
+
 int orc_sampler_set_move_mix(orc_sampler* s, const double p[3])
 {
   if (!s || p[0] < 0 || p[1] < 0 || p[2] < 0 || p[0] + p[1] + p[2] <= 0) return -1;

@@ -1,0 +1,110 @@
+"""Online learned proposal on the GPU against the oracle: marginal scores,
+band X^T X, pair features and the swap labels of the first burn-in RB block
+(the device and oracle trajectories coincide until the first table exists);
+then a full burn-in: training rounds, freezing, a finite table, learned swaps
+accepted, PIPs close to a plain run."""
+import numpy as np
+import pytest
+
+import oracle
+from bmagwa_amd import synth
+from test_gpu_sampler import dataset
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def bmg():
+    import bmagwa_amd
+    bmagwa_amd.lib()
+    return bmagwa_amd
+
+
+H = 12
+LCFG = dict(proposal=1, learn_window=H, learn_steps=20, learn_batch=1024, p_learned=0.3,
+            learn_every=100, trace_every_iter=1)
+
+
+def setup_pair(bmg, n, p, n_iter, seed=4321, burnin=3):
+    body, y, causal = dataset(n, p, (0.9, 0.95, 10, 40), 6, seed)
+    cfg = synth.sampler_cfg(n, p, 6, seed=91, n_rao=100, n_rao_burnin=burnin, thin=10,
+                            max_move_size=10, use_individual_tau2=0, do_n_iter=n_iter)
+    d = oracle.Data(n, p, body, y)
+    s = oracle.Sampler(cfg, d)
+    s.init_flat()
+    assert s.setup() == 0
+    ctx = bmg.Ctx(n, p, body, y)
+    b = bmg.Batch([ctx], [0], [cfg], LCFG)
+    b.setup()
+    return s, d, b, body, y, causal, cfg
+
+
+def test_static_inputs_and_features(bmg):
+    n, p = 300, 500
+    s, d, b, body, y, _, _ = setup_pair(bmg, n, p, 200)
+    z, pbar, band = b.learn_static(0)
+    zo = s.marginal_z()
+    assert np.allclose(z, zo, rtol=1e-5, atol=1e-4)
+    bo = np.zeros(p * (H + 1), np.int64)
+    oracle.lib().orc_band_xtx(oracle.u8(body), (n + 3) // 4, n, p, H + 1, 0, 0,
+                              oracle.ptr(bo, oracle.C.c_int64))
+    bo = bo.reshape(p, H + 1)
+    for j in range(p):
+        w = min(H + 1, p - j)
+        assert np.array_equal(band[j, :w], bo[j, :w]), j
+    pre = oracle.DataModel(d, ["A"]).self_moments()
+    rng = np.random.default_rng(3)
+    js, os_ = [], []
+    while len(js) < 400:
+        j = int(rng.integers(0, p)); o = int(rng.integers(-H, H + 1))
+        if o != 0 and 0 <= j + o < p:
+            js.append(j); os_.append(o)
+    fd = b.learn_features(0, js, os_)
+    pb = rng.uniform(0, 1, p).astype(np.float32)  # pbar on the device is flat here
+    pb[:] = pbar
+    for i, (j, o) in enumerate(zip(js, os_)):
+        fo = oracle.learn_pair_features(pre, 0, n, band, H, z, pb, j, o)
+        assert np.allclose(fd[i], fo, rtol=1e-5, atol=1e-5), (j, o, fd[i], fo)
+
+
+def test_first_block_labels_match_oracle(bmg):
+    n, p = 300, 500
+    s, d, b, *_ = setup_pair(bmg, n, p, 100)
+    b.run(100)                      # one segment, then the first (burn-in) RB block
+    s.iterate(100)
+    jg, og, lg, cnt = b.learn_samples(0)
+    jo, oo, lo = s.swap_labels(H)
+    assert cnt == len(jo) > 0
+    kg = np.lexsort((og, jg)); ko = np.lexsort((oo, jo))
+    assert np.array_equal(jg[kg], jo[ko]) and np.array_equal(og[kg], oo[ko])
+    assert np.allclose(lg[kg], lo[ko], rtol=1e-5, atol=1e-4), np.max(np.abs(lg[kg] - lo[ko]))
+    st = b.learn_state(0)
+    assert st["rounds"] == 1 and not st["frozen"]
+
+
+def test_burn_in_trains_freezes_and_keeps_posterior(bmg):
+    n, p, n_iter = 400, 1500, 12000
+    body, y, causal = dataset(n, p, (0.9, 0.95, 10, 40), 6, 77)
+    runs = {}
+    for prop in [0, 1]:
+        cfgs = [synth.sampler_cfg(n, p, 6, seed=300 + c, chain=c, n_rao=100, n_rao_burnin=10,
+                                  thin=10, max_move_size=10, do_n_iter=n_iter) for c in range(4)]
+        ctx = bmg.Ctx(n, p, body, y)
+        bc = dict(LCFG, proposal=prop, trace_every_iter=1)
+        b = bmg.Batch([ctx], [0] * 4, cfgs, bc)
+        b.setup()
+        b.run(n_iter)
+        pips = np.mean([b.p_rao(c) for c in range(4)], axis=0)
+        runs[prop] = (b, pips)
+        if prop == 1:
+            st = b.learn_state(0)
+            assert st["frozen"] and st["rounds"] >= 2 and st["labels"] > 0
+            w = b.learn_table(0)
+            assert np.all(np.isfinite(w)) and np.all(w >= 0)
+            assert np.all(w[:H, :H][np.tril_indices(H)] >= 0)
+            tr = b.trace(0)
+            learned = tr["move"] == 3
+            assert learned.sum() > 0.1 * n_iter
+            assert (learned & (tr["jd"] > 0)).sum() > 0
+    diff = np.abs(runs[0][1] - runs[1][1])
+    assert diff.max() < 0.1, diff.max()
