@@ -174,6 +174,7 @@ SIGNATURES = {
     "bmg_batch_thinned": (C.c_int, _P, C.c_int, _U32, C.c_int64, _U32, C.c_int64, _D, _D, _D, _I64, _I64),
     "bmg_batch_rb": (C.c_int, _P, C.c_int, _D, _D),
     "bmg_batch_timing": (C.c_int, _P, _D),
+    "bmg_batch_profile": (C.c_int, _P, C.c_int, _D),
 }
 
 _lib = None
@@ -584,6 +585,13 @@ class Batch:
         out = np.zeros(self.p[c])
         check(lib().bmg_batch_rb(self.h, c, _p(yh), _p(out)))
         return out
+
+    def profile(self, c):
+        o = np.zeros(16)
+        check(lib().bmg_batch_profile(self.h, c, _p(o)))
+        keys = ["tau2", "addrem", "swap", "statechange", "learned", "beta_sigma2", "pve_thin", "total",
+                "add_xy", "add_xtx", "add_trsv", "add_rest", "n_add", "n_rem", "rem", "unused"]
+        return dict(zip(keys, o))
 
     def timing(self):
         o = np.zeros(4)

@@ -646,14 +646,18 @@ static int batch_create(bmg_ctx** ctxs, int n_ctx, int n_chains, const int32_t* 
   uint64_t tr_cap = b->cfg.trace_every_iter ? std::max<uint64_t>(opts[0].do_n_iter, 1) + 8 : 1;
   uint64_t th_cap = std::max<uint64_t>(opts[0].do_n_iter / opts[0].thin, 1) + 8;
   uint64_t th_loci_cap = b->cfg.trace_capacity ? b->cfg.trace_capacity : th_cap * 64;
-  uint64_t pmax = 0, npmax = 0, tmax = 0;
-  for (auto& D : b->h_data) { pmax = std::max(pmax, D.p); npmax = std::max<uint64_t>(npmax, D.n_pad); tmax = std::max(tmax, D.tree_size); }
+  uint64_t pmax = 0, npmax = 0, tmax = 0, wmax = 0;
+  for (auto& D : b->h_data) {
+    pmax = std::max(pmax, D.p); npmax = std::max<uint64_t>(npmax, D.n_pad); tmax = std::max(tmax, D.tree_size);
+    wmax = std::max<uint64_t>(wmax, D.W32);
+  }
   size_t sz = 0;
   auto add = [&](size_t bytes) { size_t o = sz; sz += align256(bytes); return o; };
   size_t o_mi = add(4 * pmax), o_qa = add(8 * pmax), o_qr = add(8 * pmax);
   size_t o_ta = add(8 * tmax), o_tr = add(8 * tmax);
   size_t o_pp = add(8 * pmax), o_pr = add(8 * pmax), o_prr = add(8 * pmax);
   size_t o_L = add(8 * (size_t)K * (K + 1) / 2), o_xx = add(8 * (size_t)K * K);
+  size_t o_xc = add(4 * (size_t)K * wmax);
   size_t o_yh = add(8 * npmax), o_vec = add(8 * 5 * K), o_idx = add(4 * 8 * K), o_ad = add(8 * 6 * 256);
   size_t o_trm = add(tr_cap), o_trj = add(tr_cap), o_trs = add(tr_cap), o_trz = add(4 * tr_cap),
          o_trl = add(8 * tr_cap);
@@ -676,6 +680,7 @@ static int batch_create(bmg_ctx** ctxs, int n_ctx, int n_chains, const int32_t* 
     A.tree_add = (double*)(base + o_ta); A.tree_rem = (double*)(base + o_tr);
     A.p_proposal = (double*)(base + o_pp); A.p_rao = (double*)(base + o_pr); A.p_r = (double*)(base + o_prr);
     A.Lcur = (double*)(base + o_L); A.xxs = (double*)(base + o_xx); A.y_hat = (double*)(base + o_yh);
+    A.xcol = (uint32_t*)(base + o_xc);
     A.vec = (double*)(base + o_vec); A.idx = (int32_t*)(base + o_idx); A.adapt = (double*)(base + o_ad);
     A.tr_move = base + o_trm; A.tr_jd = base + o_trj; A.tr_ms = base + o_trs;
     A.tr_size = (uint32_t*)(base + o_trz); A.tr_ll = (double*)(base + o_trl);
@@ -971,6 +976,19 @@ extern "C" int bmg_batch_learn_features(bmg_batch* b, int ds, const uint32_t* j,
     }
   }
   return learn_features(L, j, off, rows, out, b->stream);
+}
+
+extern "C" int bmg_batch_profile(bmg_batch* b, int chain, double out[16]) {
+  if (!b || chain < 0 || chain >= b->n_chains || !out) { set_error("bad argument"); return BMG_E_ARG; }
+  BMG_HIP(hipSetDevice(b->device));
+  BMG_HIP(hipStreamSynchronize(b->stream));
+  ChainScal cs;
+  BMG_HIP(hipMemcpy(&cs, b->d_scal + chain, sizeof(cs), hipMemcpyDeviceToHost));
+  int khz = 0;
+  BMG_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, b->device));
+  for (int q = 0; q < 16; ++q)
+    out[q] = (q == 12 || q == 13) ? (double)cs.prof[q] : (khz > 0 ? (double)cs.prof[q] / (double)khz : 0.0);
+  return BMG_OK;
 }
 
 extern "C" int bmg_batch_sync(bmg_batch* b) {

@@ -57,6 +57,11 @@ struct ChainScal {
   double p_ms;
   uint64_t n_acpt[4], n_moves[4];   // add/remove, swap, state change, learned swap
   int32_t learn_on;
+  // wall-clock ticks per phase: tau2/alpha refresh, add/remove, swap, state
+  // change, learned swap, beta/sigma2, pve + thinning, whole iterations;
+  // then inside additions: x.y, X^T x, triangular solve, the rest of the
+  // update, and the count of additions and removals, time in removals
+  uint64_t prof[16];
   uint32_t rb_calls;
   int32_t movesize;
   double pves[3];
@@ -81,6 +86,7 @@ struct ChainArrays {
   double* p_proposal; double* p_rao; double* p_r;  // p
   double* Lcur;              // KMAX (KMAX+1) / 2
   double* xxs;               // KMAX x KMAX slot matrix
+  uint32_t* xcol;            // KMAX x W32: packed genotypes of each slot's column
   double* y_hat;             // n_pad
   double* vec;               // 5 x KMAX: xy, v, d, beta, mu
   int32_t* idx;              // 8 x KMAX: loci, x_ind1, x_ind2, slot, snp, xtype, tind, -

@@ -24,6 +24,28 @@ using namespace bmg;
 
 __device__ __forceinline__ int P(int c) { return c * (c + 1) / 2; }
 
+// element i (runtime) of a small register array without dynamic indexing,
+// which would place the whole per-lane state in scratch memory.  Each
+// element passes through an empty register constraint, so the compiler
+// cannot fold the selects back into one load at a computed address.
+template <typename T, int N>
+__device__ __forceinline__ T sel(const T (&a)[N], int i) {
+  T r = a[0];
+  asm volatile("" : "+v"(r));
+#pragma unroll
+  for (int q = 1; q < N; ++q) {
+    T v = a[q];
+    asm volatile("" : "+v"(v));
+    r = (i == q) ? v : r;
+  }
+  return r;
+}
+template <typename T, int N>
+__device__ __forceinline__ void put(T (&a)[N], int i, T v) {
+#pragma unroll
+  for (int q = 0; q < N; ++q) a[q] = (i == q) ? v : a[q];
+}
+
 // ------------------------------------------------------------- state -----
 template <int KMAX>
 struct Smem {
@@ -76,6 +98,7 @@ struct Ctx {
   const DevData& D;
   const ChainConst& K;
   const ChainArrays& A;
+  uint64_t* prof;   // the chain's phase counters (thread 0 writes), or null
 };
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
@@ -83,7 +106,7 @@ __device__ __forceinline__ int wave_id() { return threadIdx.x >> 6; }
 
 // broadcast a value computed on wave 0 lane 0 to every thread
 template <int KMAX>
-__device__ double bcast_d(Smem<KMAX>& S, double v) {
+__device__ __forceinline__ double bcast_d(Smem<KMAX>& S, double v) {
   __syncthreads();
   if (threadIdx.x == 0) S.bcd[0] = v;
   __syncthreads();
@@ -92,7 +115,7 @@ __device__ double bcast_d(Smem<KMAX>& S, double v) {
   return r;
 }
 template <int KMAX>
-__device__ long long bcast_i(Smem<KMAX>& S, long long v) {
+__device__ __forceinline__ long long bcast_i(Smem<KMAX>& S, long long v) {
   __syncthreads();
   if (threadIdx.x == 0) S.bci[0] = v;
   __syncthreads();
@@ -109,12 +132,14 @@ __device__ __forceinline__ int coding_of_type(int type, int second) {
 
 // x_j^T y (FP64) for one SNP/coding
 template <int KMAX>
-__device__ double snp_dot_y(Ctx<KMAX>& C, uint64_t snp, int coding) {
-  const uint32_t* w = C.D.g + snp * C.D.W32;
+__device__ __forceinline__ double snp_dot_y(Ctx<KMAX>& C, uint64_t snp, int coding) {
+  const uint32_t W = C.D.W32;
+  const uint32_t* __restrict__ w = C.D.g + snp * W;
+  const double* __restrict__ y = C.D.y;
   double s = 0.0;
-  for (uint32_t k = threadIdx.x; k < C.D.W32; k += BMG_BLOCK) {
+  for (uint32_t k = threadIdx.x; k < W; k += BMG_BLOCK) {
     uint32_t x = w[k];
-    const double* yy = C.D.y + 16 * (size_t)k;
+    const double* yy = y + 16 * (size_t)k;
 #pragma unroll
     for (int f = 0; f < 16; ++f) {
       int iv = code_ival((x >> (2 * f)) & 3u, coding);
@@ -125,62 +150,100 @@ __device__ double snp_dot_y(Ctx<KMAX>& C, uint64_t snp, int coding) {
 }
 
 // S.work[c] = x_c^T x_new for c in [0, cols], column `cols` being the new
-// one itself; column 0 is the constant.  Exact integer popcount dots.
+// one itself; column 0 is the constant.  Exact integer popcount dots.  Each
+// wave takes groups of 4 columns; a lane loads 4 words of the new column and
+// of each of the 4 columns before using any, so 20 loads are in flight.
 template <int KMAX>
-__device__ void new_column_dots(Ctx<KMAX>& C, uint64_t snp_new, int coding_new, int cols) {
-  const uint32_t* wn = C.D.g + snp_new * C.D.W32;
-  for (int c = wave_id(); c <= cols; c += BMG_BLOCK / BMG_WAVE) {
-    long long s = 0;
-    if (c == 0) {
-      for (uint32_t k = lane_id(); k < C.D.W32; k += BMG_WAVE) {
-        uint32_t p1, p2;
-        planes(wn[k], coding_new, p1, p2);
-        s += __popc(p1) + __popc(p2);
-      }
-    } else {
-      uint64_t sc = (c == cols) ? snp_new : (uint64_t)C.S.snp[c];
-      int cc = (c == cols) ? coding_new : C.S.xtype[c];
-      const uint32_t* wc = C.D.g + sc * C.D.W32;
-      for (uint32_t k = lane_id(); k < C.D.W32; k += BMG_WAVE)
-        s += dot_planes(wc[k], wn[k], cc, coding_new);
+__device__ __forceinline__ void new_column_dots(Ctx<KMAX>& C, uint64_t snp_new, int coding_new, int cols) {
+  // pointers read once: loads of struct fields inside the loops would be
+  // repeated (and waited for) on every column group
+  const uint32_t W = C.D.W32;
+  const uint32_t* __restrict__ wn = C.D.g + snp_new * W;
+  const uint32_t* __restrict__ xcol = C.A.xcol;
+  const int lane = lane_id();
+  for (int c0 = 4 * wave_id(); c0 <= cols; c0 += 4 * (BMG_BLOCK / BMG_WAVE)) {
+    const uint32_t* wc[4];
+    int cc[4];
+    bool on[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int c = c0 + q;
+      on[q] = c <= cols;
+      cc[q] = (c >= cols) ? coding_new : (c == 0 ? coding_new : (int)C.S.xtype[c]);
+      // the model's columns from the chain's contiguous slot cache
+      wc[q] = (c >= cols || c == 0) ? wn : xcol + (size_t)C.S.slot[c] * W;
     }
-    s = wave_sum_i64(s);
-    if (lane_id() == 0) C.S.work[c] = (double)s;
+    int s[4] = {0, 0, 0, 0};   // per-lane partials: at most 4 * 16 per word
+    // straight-line body: every load is issued unconditionally at a valid
+    // address (index clamped, value masked), so nothing waits on a branch
+    for (uint32_t kb = lane; kb < W; kb += 4 * BMG_WAVE) {
+      uint32_t xn[4], xc[4][4], keep[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t k = kb + u * BMG_WAVE;
+        keep[u] = k < W ? 0xFFFFFFFFu : 0u;
+        const uint32_t kk = k < W ? k : W - 1;
+        xn[u] = wn[kk];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) xc[q][u] = wc[q][kk];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t x = xn[u] & keep[u];
+        uint32_t n1, n2;
+        planes(x, coding_new, n1, n2);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          uint32_t a1, a2;
+          planes(xc[q][u], cc[q], a1, a2);
+          // column 0 is the constant: its planes are all ones
+          if (c0 + q == 0) { a1 = BMG_EVEN; a2 = 0u; }
+          s[q] += (int)(__popc(a1 & n1) + __popc(a1 & n2) + __popc(a2 & n1) + __popc(a2 & n2));
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      int v = wave_sum_i32(s[q]);
+      if (lane == 0 && on[q]) C.S.work[c0 + q] = (double)v;
+    }
   }
   __syncthreads();
 }
 
 // ----------------------------------------------------- triangular (wave 0)
-template <int KMAX>
-__device__ __forceinline__ double reg_get(const double (&b)[(KMAX + 63) / 64], int t) {
-  double r = b[0];
-#pragma unroll
-  for (int q = 1; q < (KMAX + 63) / 64; ++q) if (t == q) r = b[q];
-  return r;
-}
-template <int KMAX>
-__device__ __forceinline__ void reg_set(double (&b)[(KMAX + 63) / 64], int t, double v) {
-#pragma unroll
-  for (int q = 0; q < (KMAX + 63) / 64; ++q) if (t == q) b[q] = v;
+// value of lane `src` (uniform) of a double held by every lane
+__device__ __forceinline__ double readlane_d(double v, int src) {
+  int2 h = *reinterpret_cast<int2*>(&v);
+  h.x = __builtin_amdgcn_readlane(h.x, src);
+  h.y = __builtin_amdgcn_readlane(h.y, src);
+  return *reinterpret_cast<double*>(&h);
 }
 
 // x := U^-T x for x[0..m) in LDS (dtrsv Upper/Trans order).  Wave 0 only.
+// Lane l holds x[l + 64 t] in b[t]; the outer loop over t is unrolled so the
+// register array is only ever indexed by constants.  Every lane forms
+// x_i = b_i / U_ii from the owner's b_i (a cross-lane read), so the serial
+// chain per row is one read and one division.
 template <int KMAX>
-__device__ void trsv_ut_w0(Smem<KMAX>& S, double* x, int m) {
+__device__ __forceinline__ void trsv_ut_w0(Smem<KMAX>& S, double* x, int m) {
   constexpr int NT = (KMAX + 63) / 64;
   const int lane = lane_id();
   double b[NT];
 #pragma unroll
   for (int t = 0; t < NT; ++t) { int j = lane + 64 * t; b[t] = j < m ? x[j] : 0.0; }
-  for (int i = 0; i < m; ++i) {
-    int owner = i & 63, ti = i >> 6;
-    double mine = 0.0;
-    if (lane == owner) { mine = reg_get<KMAX>(b, ti) / S.L[P(i) + i]; reg_set<KMAX>(b, ti, mine); }
-    double xi = __shfl(mine, owner, 64);
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
-      int j = lane + 64 * t;
-      if (j > i && j < m) b[t] = b[t] - S.L[P(j) + i] * xi;
+  for (int ti = 0; ti < NT; ++ti) {
+    const int iend = m < 64 * (ti + 1) ? m : 64 * (ti + 1);
+    for (int i = 64 * ti; i < iend; ++i) {
+      const int owner = i & 63;
+      const double xi = readlane_d(b[ti], owner) / S.L[P(i) + i];
+      if (lane == owner) b[ti] = xi;
+#pragma unroll
+      for (int t = ti; t < NT; ++t) {
+        int j = lane + 64 * t;
+        if (j > i && j < m) b[t] = b[t] - S.L[P(j) + i] * xi;
+      }
     }
   }
 #pragma unroll
@@ -189,26 +252,26 @@ __device__ void trsv_ut_w0(Smem<KMAX>& S, double* x, int m) {
 
 // x := U^-1 x (dtrsv Upper/NoTrans order, columns from the last).  Wave 0.
 template <int KMAX>
-__device__ void trsv_un_w0(Smem<KMAX>& S, double* x, int m) {
+__device__ __forceinline__ void trsv_un_w0(Smem<KMAX>& S, double* x, int m) {
   constexpr int NT = (KMAX + 63) / 64;
   const int lane = lane_id();
   double b[NT];
 #pragma unroll
   for (int t = 0; t < NT; ++t) { int j = lane + 64 * t; b[t] = j < m ? x[j] : 0.0; }
-  for (int j = m - 1; j >= 0; --j) {
-    int owner = j & 63, tj = j >> 6;
-    double mine = 0.0;
-    if (lane == owner) {
-      double xj = reg_get<KMAX>(b, tj);
-      if (xj != 0.0) { xj = xj / S.L[P(j) + j]; reg_set<KMAX>(b, tj, xj); }
-      mine = xj;
-    }
-    double t = __shfl(mine, owner, 64);
-    if (t != 0.0) {
 #pragma unroll
-      for (int q = 0; q < NT; ++q) {
-        int i = lane + 64 * q;
-        if (i < j) b[q] = b[q] - t * S.L[P(j) + i];
+  for (int tj = NT - 1; tj >= 0; --tj) {
+    const int jend = m < 64 * (tj + 1) ? m : 64 * (tj + 1);
+    for (int j = jend - 1; j >= 64 * tj; --j) {
+      const int owner = j & 63;
+      double t = readlane_d(b[tj], owner);
+      if (t != 0.0) {
+        t = t / S.L[P(j) + j];
+        if (lane == owner) b[tj] = t;
+#pragma unroll
+        for (int q = 0; q <= tj; ++q) {
+          int i = lane + 64 * q;
+          if (i < j) b[q] = b[q] - t * S.L[P(j) + i];
+        }
       }
     }
   }
@@ -241,20 +304,16 @@ __device__ __forceinline__ void drotg(double& a, double& b, double& c, double& s
 // is generated from logical column cr + t and applied to all later columns
 // at once (per element the same sequence as the sequential loop).
 template <int KMAX>
-__device__ void chol_downdate(Smem<KMAX>& S, int m, int cr) {
+__device__ __forceinline__ void chol_downdate(Smem<KMAX>& S, int m, int cr) {
   constexpr int NT = (KMAX + 63) / 64;
   if (wave_id() == 0) {
     const int lane = lane_id();
     for (int j = cr; j <= m - 2; ++j) {
-      int owner = j & 63;
-      double cc = 0.0, ss = 0.0;
-      if (lane == owner) {
-        double a = S.L[P(j + 1) + j], b = S.L[P(j + 1) + j + 1];
-        drotg(a, b, cc, ss);
-        S.L[P(j + 1) + j] = a;
-      }
-      cc = __shfl(cc, owner, 64);
-      ss = __shfl(ss, owner, 64);
+      // every lane generates the rotation from the same two LDS values
+      double a = S.L[P(j + 1) + j], b = S.L[P(j + 1) + j + 1];
+      double cc, ss;
+      drotg(a, b, cc, ss);
+      if (lane == (j & 63)) S.L[P(j + 1) + j] = a;
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
         int jl = lane + 64 * t;
@@ -305,7 +364,7 @@ __device__ void chol_downdate(Smem<KMAX>& S, int m, int cr) {
 // Full factorisation of S.L (m x m, in place), right-looking; returns 0 if
 // not positive definite.
 template <int KMAX>
-__device__ int chol_full(Smem<KMAX>& S, int m) {
+__device__ __forceinline__ int chol_full(Smem<KMAX>& S, int m) {
   int ok = 1;
   for (int k = 0; k < m; ++k) {
     __syncthreads();
@@ -350,7 +409,7 @@ __device__ __forceinline__ double wave_scan(double v) {
 
 // draw an index with probability w_i / total (wave 0 computes, all return)
 template <int KMAX>
-__device__ uint64_t tree_pick(Ctx<KMAX>& C, int tr, double u) {
+__device__ __forceinline__ uint64_t tree_pick(Ctx<KMAX>& C, int tr, double u) {
   if (wave_id() == 0) {
     const int lane = lane_id();
     double total = tr == 0 ? C.R.tot_add : C.R.tot_rem;
@@ -380,7 +439,7 @@ __device__ uint64_t tree_pick(Ctx<KMAX>& C, int tr, double u) {
 
 // set leaf i to its weight (unzero) or 0 (zero) and update the path
 template <int KMAX>
-__device__ void tree_set(Ctx<KMAX>& C, int tr, uint64_t i, int zero) {
+__device__ __forceinline__ void tree_set(Ctx<KMAX>& C, int tr, uint64_t i, int zero) {
   double* g0 = tr == 0 ? C.A.tree_add : C.A.tree_rem;
   const double* w = tr == 0 ? C.A.q_add : C.A.q_rem;
   double total;
@@ -426,7 +485,7 @@ __device__ void tree_set(Ctx<KMAX>& C, int tr, uint64_t i, int zero) {
 // -------------------------------------------------------------- model -----
 // Column bookkeeping helpers operate on LDS arrays with all threads.
 template <int KMAX>
-__device__ int alloc_slot(Ctx<KMAX>& C) {
+__device__ __forceinline__ int alloc_slot(Ctx<KMAX>& C) {
   // lowest slot free in both the current and proposed models
   int s = -1;
   for (int q = 0; q < (KMAX + 63) / 64 && s < 0; ++q) {
@@ -437,7 +496,7 @@ __device__ int alloc_slot(Ctx<KMAX>& C) {
 }
 
 template <int KMAX>
-__device__ void set_slot_bit(Ctx<KMAX>& C, int s, int on) {
+__device__ __forceinline__ void set_slot_bit(Ctx<KMAX>& C, int s, int on) {
   __syncthreads();
   if (threadIdx.x == 0) {
     if (on) C.S.slots_prop[s >> 6] |= (1ull << (s & 63));
@@ -448,7 +507,7 @@ __device__ void set_slot_bit(Ctx<KMAX>& C, int s, int on) {
 
 // src/model.hpp:432-506 for one column (type A, H, D or R)
 template <int KMAX>
-__device__ void update_on_add(Ctx<KMAX>& C, uint32_t ind, int model_ind, int coding,
+__device__ __forceinline__ void update_on_add(Ctx<KMAX>& C, uint32_t ind, int model_ind, int coding,
                               int is_x_ind2, double dval) {
   Smem<KMAX>& S = C.S;
   Reg& R = C.R;
@@ -474,8 +533,16 @@ __device__ void update_on_add(Ctx<KMAX>& C, uint32_t ind, int model_ind, int cod
   set_slot_bit(C, slot, 1);
   R.cols = col + 1;
   if (R.lmin > col) R.lmin = col;
+  {
+    const uint32_t* src = C.D.g + (uint64_t)ind * C.D.W32;
+    uint32_t* dst = C.A.xcol + (size_t)slot * C.D.W32;
+    for (uint32_t k = threadIdx.x; k < C.D.W32; k += BMG_BLOCK) dst[k] = src[k];
+  }
+  const uint64_t ta = wall_clock64();
   double xy = snp_dot_y(C, ind, coding);
+  const uint64_t tb = wall_clock64();
   new_column_dots(C, ind, coding, col);
+  const uint64_t tc = wall_clock64();
   // record the new X^T X row/column in the slot matrix
   for (int c = threadIdx.x; c <= col; c += BMG_BLOCK) {
     int sc = c == col ? slot : S.slot[c];
@@ -488,6 +555,7 @@ __device__ void update_on_add(Ctx<KMAX>& C, uint32_t ind, int model_ind, int cod
   __syncthreads();
   if (wave_id() == 0) trsv_ut_w0(S, S.L + P(col), col);
   __syncthreads();
+  const uint64_t td = wall_clock64();
   double ll2 = 0.0, lv = 0.0;
   for (int i = threadIdx.x; i < col; i += BMG_BLOCK) {
     double li = S.L[P(col) + i];
@@ -512,11 +580,16 @@ __device__ void update_on_add(Ctx<KMAX>& C, uint32_t ind, int model_ind, int cod
   R.ldq += 0.5 * log(dval);
   R.ldl += log(lcc);
   R.ll = R.ldq - R.ldl + C.K.minus_n_plus_nu_2 * log(R.syx);
+  if (threadIdx.x == 0 && C.prof) {
+    const uint64_t te = wall_clock64();
+    C.prof[8] += tb - ta; C.prof[9] += tc - tb; C.prof[10] += td - tc; C.prof[11] += te - td;
+    C.prof[12] += 1;
+  }
 }
 
 // src/model.hpp:239-268
 template <int KMAX>
-__device__ void add_term(Ctx<KMAX>& C, uint32_t ind, int t_ind, const double* d2) {
+__device__ __forceinline__ void add_term(Ctx<KMAX>& C, uint32_t ind, int t_ind, const double* d2) {
   Smem<KMAX>& S = C.S;
   Reg& R = C.R;
   int type = C.D.types[t_ind];
@@ -531,7 +604,7 @@ __device__ void add_term(Ctx<KMAX>& C, uint32_t ind, int t_ind, const double* d2
   if (threadIdx.x == 0) { S.loci[model_ind] = ind; S.tind[model_ind] = (int8_t)t_ind; }
   __syncthreads();
   R.nloci++;
-  R.Ns[type]++;
+  put(R.Ns, type, sel(R.Ns, type) + 1);
   R.mu_computed = 0;
   if (is_ah) {
     update_on_add(C, ind, model_ind, 0, 0, d2[0]);
@@ -555,7 +628,17 @@ __device__ void add_term(Ctx<KMAX>& C, uint32_t ind, int t_ind, const double* d2
 
 // src/model.hpp:508-556
 template <int KMAX>
-__device__ void update_on_remove(Ctx<KMAX>& C, int x_ind) {
+__device__ __forceinline__ void update_on_remove_impl(Ctx<KMAX>& C, int x_ind);
+
+template <int KMAX>
+__device__ __forceinline__ void update_on_remove(Ctx<KMAX>& C, int x_ind) {
+  const uint64_t t0 = wall_clock64();
+  update_on_remove_impl(C, x_ind);
+  if (threadIdx.x == 0 && C.prof) { C.prof[13] += 1; C.prof[14] += wall_clock64() - t0; }
+}
+
+template <int KMAX>
+__device__ __forceinline__ void update_on_remove_impl(Ctx<KMAX>& C, int x_ind) {
   Smem<KMAX>& S = C.S;
   Reg& R = C.R;
   int k = R.cols;
@@ -598,13 +681,13 @@ __device__ void update_on_remove(Ctx<KMAX>& C, int x_ind) {
 
 // src/model.hpp:270-312
 template <int KMAX>
-__device__ void remove_term(Ctx<KMAX>& C, int model_ind) {
+__device__ __forceinline__ void remove_term(Ctx<KMAX>& C, int model_ind) {
   Smem<KMAX>& S = C.S;
   Reg& R = C.R;
   int type = C.D.types[S.tind[model_ind]];
   int xi1 = S.xi1[model_ind], xi2 = S.xi2[model_ind];
   uint32_t ind = S.loci[model_ind];
-  R.Ns[type]--;
+  put(R.Ns, type, sel(R.Ns, type) - 1);
   int nl = R.nloci;
   int xmove = type == 4 ? 2 : 1;
   int t = threadIdx.x;
@@ -628,7 +711,7 @@ __device__ void remove_term(Ctx<KMAX>& C, int model_ind) {
 
 // index of SNP `ind` in the proposed model, or -1 (all threads)
 template <int KMAX>
-__device__ int find_locus(Ctx<KMAX>& C, uint64_t ind) {
+__device__ __forceinline__ int find_locus(Ctx<KMAX>& C, uint64_t ind) {
   __syncthreads();
   if (threadIdx.x == 0) C.S.bci[2] = -1;
   __syncthreads();
@@ -642,7 +725,7 @@ __device__ int find_locus(Ctx<KMAX>& C, uint64_t ind) {
 
 // src/model.hpp:199-237 on the proposed model
 template <int KMAX>
-__device__ void compute_log_likelihood(Ctx<KMAX>& C) {
+__device__ __forceinline__ void compute_log_likelihood(Ctx<KMAX>& C) {
   Smem<KMAX>& S = C.S;
   Reg& R = C.R;
   int k = R.cols;
@@ -686,7 +769,7 @@ __device__ void compute_log_likelihood(Ctx<KMAX>& C) {
 // working copy: only factor columns >= lmin differ from the stored current
 // factor.
 template <int KMAX>
-__device__ void commit(Ctx<KMAX>& C) {
+__device__ __forceinline__ void commit(Ctx<KMAX>& C) {
   Smem<KMAX>& S = C.S;
   Reg& R = C.R;
   __syncthreads();
@@ -713,7 +796,7 @@ __device__ void commit(Ctx<KMAX>& C) {
 }
 
 template <int KMAX>
-__device__ void restore(Ctx<KMAX>& C) {
+__device__ __forceinline__ void restore(Ctx<KMAX>& C) {
   Smem<KMAX>& S = C.S;
   Reg& R = C.R;
   __syncthreads();
@@ -736,24 +819,24 @@ __device__ void restore(Ctx<KMAX>& C) {
 
 // --------------------------------------------------------------- prior ----
 // src/prior.hpp:144-158 (single effect type, Ns[type] == n_loci)
-__device__ __forceinline__ double prior_add(const ChainConst& K, const int* Ns, int n_old, int type) {
+__device__ __forceinline__ double prior_add(const ChainConst& K, const int (&Ns)[5], int n_old, int type) {
   return (log(K.g_a + n_old) - log(K.g_b + K.m_g - n_old - 1)) +
-         (log(K.tp_type + Ns[type]) - log(K.tp_sum + n_old));
+         (log(K.tp_type + sel(Ns, type)) - log(K.tp_sum + n_old));
 }
-__device__ __forceinline__ double prior_rem(const ChainConst& K, const int* Ns, int n_old, int type) {
+__device__ __forceinline__ double prior_rem(const ChainConst& K, const int (&Ns)[5], int n_old, int type) {
   return (log(K.g_b + K.m_g - n_old) - log(K.g_a + n_old - 1)) +
-         (log(K.tp_sum + n_old - 1) - log(K.tp_type + Ns[type] - 1));
+         (log(K.tp_sum + n_old - 1) - log(K.tp_type + sel(Ns, type) - 1));
 }
 
 template <int KMAX>
-__device__ double draw_inv_tau2(Ctx<KMAX>& C, int term) {
+__device__ __forceinline__ double draw_inv_tau2(Ctx<KMAX>& C, int term) {
   if (C.K.use_ind)
     return 1.0 / (C.R.alpha2 * rng_sinvchi2(C.R.rng, C.K.nu_tau2[term], C.K.s2_tau2[term]));
-  return C.R.ita2[term];
+  return sel(C.R.ita2, term);
 }
 
 template <int KMAX>
-__device__ void prepare_add_new_term(Ctx<KMAX>& C, double* d2) {
+__device__ __forceinline__ void prepare_add_new_term(Ctx<KMAX>& C, double* d2) {
   int type = C.D.type;
   if (type == 4) { d2[0] = draw_inv_tau2(C, 0); d2[1] = draw_inv_tau2(C, 1); }
   else d2[0] = draw_inv_tau2(C, type);
@@ -761,7 +844,7 @@ __device__ void prepare_add_new_term(Ctx<KMAX>& C, double* d2) {
 
 // ------------------------------------------------- beta / sigma2 / pve ----
 template <int KMAX>
-__device__ void compute_mu_beta(Ctx<KMAX>& C) {
+__device__ __forceinline__ void compute_mu_beta(Ctx<KMAX>& C) {
   if (!C.R.mu_computed) {
     __syncthreads();
     for (int i = threadIdx.x; i < C.R.cols; i += BMG_BLOCK) C.S.mu[i] = C.S.v[i];
@@ -787,7 +870,7 @@ __device__ __forceinline__ void rng_skip(Rng& r, uint32_t k) {
 
 // src/model.hpp:326-342 with src/rand.hpp:61-78,144-168
 template <int KMAX>
-__device__ void sample_beta_sigma2(Ctx<KMAX>& C) {
+__device__ __forceinline__ void sample_beta_sigma2(Ctx<KMAX>& C) {
   Reg& R = C.R;
   double nu = C.K.n_plus_nu;
   double s2 = R.syx / C.K.n_plus_nu;
@@ -811,37 +894,73 @@ __device__ void sample_beta_sigma2(Ctx<KMAX>& C) {
 
 // y_hat = X_g beta_g (+ constant) into the chain's y_hat; returns sum x_b^2,
 // sum x_b (x_b = X_g beta_g) and sum of y_hat, y_hat^2 for the PVE
+// X_g beta_g (without the constant) for every individual, one packed word
+// (16 individuals) per lane: the columns' words are fetched eight at a time
+// (coalesced across lanes, several loads in flight) and each individual's
+// sum is formed in model-column order, as the per-individual loop does.
+// out5: sum xb, sum xb^2, sum y_hat, sum y_hat^2, sum (beta_0 - y) xb.
 template <int KMAX>
-__device__ void xbeta(Ctx<KMAX>& C, double* out4, int write_yhat) {
+__device__ __forceinline__ void xbeta(Ctx<KMAX>& C, double* out5, int write_yhat) {
   Smem<KMAX>& S = C.S;
   Reg& R = C.R;
-  double b0 = S.beta[0];
-  double sxb = 0, sxb2 = 0, sy = 0, sy2 = 0;
-  for (uint32_t i = threadIdx.x; i < C.D.n; i += BMG_BLOCK) {
-    double acc = 0.0;
-    for (int c = 1; c < R.beta_len; ++c) {
-      double t = S.beta[c];
-      if (t != 0.0) {
-        uint32_t w = C.D.g[(uint64_t)S.snp[c] * C.D.W32 + (i >> 4)];
-        int iv = code_ival((w >> (2 * (i & 15))) & 3u, S.xtype[c]);
-        acc = acc + t * (double)iv;
+  const double b0 = S.beta[0];
+  const uint32_t n = C.D.n, W = C.D.W32;
+  const uint32_t* __restrict__ xcol = C.A.xcol;
+  const double* __restrict__ y = C.D.y;
+  double* __restrict__ yhat = C.A.y_hat;
+  const int L = R.beta_len;
+  double sxb = 0, sxb2 = 0, sy = 0, sy2 = 0, syx = 0;
+  for (uint32_t w = threadIdx.x; w * 16 < n; w += BMG_BLOCK) {
+    double acc[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) acc[q] = 0.0;
+    for (int c0 = 1; c0 < L; c0 += 8) {
+      uint32_t wd[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int c = c0 + u;
+        wd[u] = (c < L && S.beta[c] != 0.0) ? xcol[(size_t)S.slot[c] * W + w] : 0u;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int c = c0 + u;
+        if (c < L) {
+          const double t = S.beta[c];
+          if (t != 0.0) {
+            const int xt = S.xtype[c];
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+              int iv = code_ival((wd[u] >> (2 * q)) & 3u, xt);
+              acc[q] = acc[q] + t * (double)iv;
+            }
+          }
+        }
       }
     }
-    sxb += acc; sxb2 += acc * acc;
-    double yh = R.nloci > 0 ? acc + b0 : b0;
-    sy += yh; sy2 += yh * yh;
-    if (write_yhat) C.A.y_hat[i] = yh;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const uint32_t i = w * 16 + q;
+      if (i < n) {
+        const double a = acc[q];
+        sxb += a; sxb2 += a * a;
+        const double yh = R.nloci > 0 ? a + b0 : b0;
+        sy += yh; sy2 += yh * yh;
+        syx += (b0 - y[i]) * a;
+        if (write_yhat) yhat[i] = yh;
+      }
+    }
   }
-  out4[0] = block_sum(sxb, S.red);
-  out4[1] = block_sum(sxb2, S.red);
-  out4[2] = block_sum(sy, S.red);
-  out4[3] = block_sum(sy2, S.red);
+  out5[0] = block_sum(sxb, S.red);
+  out5[1] = block_sum(sxb2, S.red);
+  out5[2] = block_sum(sy, S.red);
+  out5[3] = block_sum(sy2, S.red);
+  out5[4] = block_sum(syx, S.red);
 }
 
 // src/model.hpp:345-392 (no covariates: pves[2] = 0)
 template <int KMAX>
-__device__ void compute_pve(Ctx<KMAX>& C, double* pves) {
-  double s[4];
+__device__ __forceinline__ void compute_pve(Ctx<KMAX>& C, double* pves) {
+  double s[5];
   xbeta(C, s, 1);
   double n = (double)C.D.n;
   if (C.R.nloci > 0) {
@@ -857,7 +976,7 @@ __device__ void compute_pve(Ctx<KMAX>& C, double* pves) {
 
 // src/prior.cpp:30-141 on the working model
 template <int KMAX>
-__device__ void sample_alpha_and_tau2(Ctx<KMAX>& C) {
+__device__ __forceinline__ void sample_alpha_and_tau2(Ctx<KMAX>& C) {
   Smem<KMAX>& S = C.S;
   Reg& R = C.R;
   const ChainConst& K = C.K;
@@ -874,9 +993,11 @@ __device__ void sample_alpha_and_tau2(Ctx<KMAX>& C) {
   } else {
     double b2[4] = {0, 0, 0, 0}; double mt[4] = {0, 0, 0, 0};
     for (int i = 1; i < R.beta_len; ++i) {
-      b2[S.xtype[i]] += S.beta[i] * S.beta[i];
-      mt[S.xtype[i]] += 1.0;
+      const int xt = S.xtype[i];
+      put(b2, xt, sel(b2, xt) + S.beta[i] * S.beta[i]);
+      put(mt, xt, sel(mt, xt) + 1.0);
     }
+#pragma unroll
     for (int t = 0; t < 4; ++t) {
       if (!K.allow_terms[t]) continue;
       double nu = K.nu_tau2[t] + mt[t];
@@ -884,7 +1005,7 @@ __device__ void sample_alpha_and_tau2(Ctx<KMAX>& C) {
       R.ita2[t] = 1.0 / (R.alpha2 * rng_sinvchi2(R.rng, nu, s2));
     }
     __syncthreads();
-    for (int i = 1 + threadIdx.x; i < R.beta_len; i += BMG_BLOCK) S.d[i] = R.ita2[S.xtype[i]];
+    for (int i = 1 + threadIdx.x; i < R.beta_len; i += BMG_BLOCK) S.d[i] = sel(R.ita2, (int)S.xtype[i]);
   }
   R.mu_computed = 0;
   __syncthreads();
@@ -893,24 +1014,10 @@ __device__ void sample_alpha_and_tau2(Ctx<KMAX>& C) {
   double a = R.alpha;
   double xbxb = 0, ebxb = 0;
   if (nterms > 0) {
-    // x_b = X_g beta_g, e_b = beta_0 - y:  e_b . x_b = beta_0 sum x_b - y . x_b
-    double s[4];
+    // x_b = X_g beta_g, e_b = beta_0 - y
+    double s[5];
     xbeta(C, s, 0);
-    double yxb = 0.0;
-    Smem<KMAX>& SS = C.S;
-    for (uint32_t i = threadIdx.x; i < C.D.n; i += BMG_BLOCK) {
-      double acc = 0.0;
-      for (int c = 1; c < R.beta_len; ++c) {
-        double t = SS.beta[c];
-        if (t != 0.0) {
-          uint32_t w = C.D.g[(uint64_t)SS.snp[c] * C.D.W32 + (i >> 4)];
-          int iv = code_ival((w >> (2 * (i & 15))) & 3u, SS.xtype[c]);
-          acc = acc + t * (double)iv;
-        }
-      }
-      yxb += (SS.beta[0] - C.D.y[i]) * acc;
-    }
-    ebxb = block_sum(yxb, SS.red);
+    ebxb = s[4];
     xbxb = s[1];
   }
   int trials = 0;
@@ -933,8 +1040,9 @@ __device__ void sample_alpha_and_tau2(Ctx<KMAX>& C) {
   if (K.use_ind) {
     for (int i = 1 + threadIdx.x; i < R.beta_len; i += BMG_BLOCK) S.d[i] = S.d[i] * old / R.alpha2;
   } else {
+#pragma unroll
     for (int t = 0; t < 4; ++t) if (K.allow_terms[t]) R.ita2[t] = R.ita2[t] * old / R.alpha2;
-    for (int i = 1 + threadIdx.x; i < R.beta_len; i += BMG_BLOCK) S.d[i] = R.ita2[S.xtype[i]];
+    for (int i = 1 + threadIdx.x; i < R.beta_len; i += BMG_BLOCK) S.d[i] = sel(R.ita2, (int)S.xtype[i]);
   }
   __syncthreads();
 }
@@ -947,7 +1055,7 @@ __device__ __forceinline__ int sample_naive(const double* cumsum, int m, double 
 }
 
 template <int KMAX>
-__device__ void undo_dd(Ctx<KMAX>& C, int ms) {
+__device__ __forceinline__ void undo_dd(Ctx<KMAX>& C, int ms) {
   for (int i = 0; i < ms; ++i) {
     uint64_t ind = C.S.mi[i];
     if (C.S.mi_isadd[i]) { tree_set(C, 0, ind, 0); tree_set(C, 1, ind, 1); }
@@ -957,7 +1065,7 @@ __device__ void undo_dd(Ctx<KMAX>& C, int ms) {
 
 // src/sampler.hpp:899-1123 without delayed rejection
 template <int KMAX>
-__device__ int move_addrem(Ctx<KMAX>& C, double* adapt) {
+__device__ __forceinline__ int move_addrem(Ctx<KMAX>& C, double* adapt) {
   Smem<KMAX>& S = C.S;
   Reg& R = C.R;
   const ChainConst& K = C.K;
@@ -1056,7 +1164,7 @@ __device__ int move_addrem(Ctx<KMAX>& C, double* adapt) {
 
 // is SNP j in the current model (with the switch move's temporary marks)?
 template <int KMAX>
-__device__ int cur_in_model(Ctx<KMAX>& C, int64_t j, int64_t wbase) {
+__device__ __forceinline__ int cur_in_model(Ctx<KMAX>& C, int64_t j, int64_t wbase) {
   if (j < 0 || j >= (int64_t)C.D.p) return 0;   // outside the genome: stops a walk
   for (int q = 0; q < C.R.n_marks; ++q) if ((int64_t)C.S.marks[q] == j) return 1;
   if (j >= wbase && j < wbase + 512) return C.S.win[j - wbase] >= 0;
@@ -1064,7 +1172,7 @@ __device__ int cur_in_model(Ctx<KMAX>& C, int64_t j, int64_t wbase) {
 }
 
 template <int KMAX>
-__device__ void load_window(Ctx<KMAX>& C, int64_t center, int64_t* wbase) {
+__device__ __forceinline__ void load_window(Ctx<KMAX>& C, int64_t center, int64_t* wbase) {
   int64_t b = center - 256;
   if (b < 0) b = 0;
   __syncthreads();
@@ -1078,7 +1186,7 @@ __device__ void load_window(Ctx<KMAX>& C, int64_t center, int64_t* wbase) {
 
 // src/sampler.hpp:1271-1418
 template <int KMAX>
-__device__ int move_switch(Ctx<KMAX>& C, double* adapt) {
+__device__ __forceinline__ int move_switch(Ctx<KMAX>& C, double* adapt) {
   Smem<KMAX>& S = C.S;
   Reg& R = C.R;
   const ChainConst& K = C.K;
@@ -1183,7 +1291,7 @@ __device__ __forceinline__ int lw_off(int H, int c) { return c < H ? c - H : c -
 __device__ __forceinline__ int lw_col(int H, int o) { return o < 0 ? o + H : o + H - 1; }
 
 template <int KMAX>
-__device__ int move_learned_swap(Ctx<KMAX>& C) {
+__device__ __forceinline__ int move_learned_swap(Ctx<KMAX>& C) {
   Smem<KMAX>& S = C.S;
   Reg& R = C.R;
   const int H = C.D.lH;
@@ -1251,7 +1359,7 @@ __device__ int move_learned_swap(Ctx<KMAX>& C) {
 
 // src/sampler.hpp:1421-1669 without delayed rejection
 template <int KMAX>
-__device__ int move_statechange(Ctx<KMAX>& C, double* adapt) {
+__device__ __forceinline__ int move_statechange(Ctx<KMAX>& C, double* adapt) {
   Smem<KMAX>& S = C.S;
   Reg& R = C.R;
   const ChainConst& K = C.K;
@@ -1334,7 +1442,7 @@ __device__ int move_statechange(Ctx<KMAX>& C, double* adapt) {
 }
 
 // src/sampler.hpp:1728-1769
-__device__ void adapt_jd_mb(double* p_ms, double* adapt, int max_ms) {
+__device__ __forceinline__ void adapt_jd_mb(double* p_ms, double* adapt, int max_ms) {
   double* q_p0 = adapt + 5 * 256;
   const double* r_ms = adapt + 3 * 256;
   const double* r_n = adapt + 4 * 256;
@@ -1359,7 +1467,7 @@ __device__ void adapt_jd_mb(double* p_ms, double* adapt, int max_ms) {
   *p_ms = p_best;
 }
 
-__device__ void geometric_cdf(int maxsize, double p, double* values) {
+__device__ __forceinline__ void geometric_cdf(int maxsize, double p, double* values) {
   double q = 1 - p, qp = q;
   for (int i = 0; i < maxsize; ++i) { values[i] = 1 - q; q *= qp; }
 }
@@ -1367,7 +1475,7 @@ __device__ void geometric_cdf(int maxsize, double p, double* values) {
 // --------------------------------------------------------------- kernel ---
 
 template <int KMAX>
-__device__ void load_chain(Ctx<KMAX>& C, const ChainScal& CS) {
+__device__ __forceinline__ void load_chain(Ctx<KMAX>& C, const ChainScal& CS) {
   Smem<KMAX>& S = C.S;
   Reg& R = C.R;
   R.rng = rng_make(C.K.seed, C.K.chain_id, 0, 0);
@@ -1419,7 +1527,7 @@ __device__ void load_chain(Ctx<KMAX>& C, const ChainScal& CS) {
 }
 
 template <int KMAX>
-__device__ void store_chain(Ctx<KMAX>& C, ChainScal& CS) {
+__device__ __forceinline__ void store_chain(Ctx<KMAX>& C, ChainScal& CS) {
   Smem<KMAX>& S = C.S;
   Reg& R = C.R;
   __syncthreads();
@@ -1458,7 +1566,7 @@ __device__ void store_chain(Ctx<KMAX>& C, ChainScal& CS) {
 // acts on the current model; here the working copy equals it, and the
 // refreshed model is committed in full
 template <int KMAX>
-__device__ void refresh_tau2(Ctx<KMAX>& C) {
+__device__ __forceinline__ void refresh_tau2(Ctx<KMAX>& C) {
   sample_alpha_and_tau2(C);
   compute_log_likelihood(C);
   C.R.lmin = 0;
@@ -1466,14 +1574,20 @@ __device__ void refresh_tau2(Ctx<KMAX>& C) {
 }
 
 template <int KMAX>
-__device__ void run_iterations(Ctx<KMAX>& C, ChainScal& CS, uint64_t n_iter) {
+__device__ __forceinline__ void run_iterations(Ctx<KMAX>& C, ChainScal& CS, uint64_t n_iter) {
   Smem<KMAX>& S = C.S;
   Reg& R = C.R;
   const ChainConst& K = C.K;
   double* adapt = C.A.adapt;
   uint64_t it0 = CS.n_iter;
+  uint64_t pr[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   for (uint64_t iter = it0; iter < it0 + n_iter; ++iter) {
-    if ((iter + 1) % K.n_st == 0) refresh_tau2(C);
+    const uint64_t t_it = wall_clock64();
+    uint64_t t0 = t_it;
+    if ((iter + 1) % K.n_st == 0) {
+      refresh_tau2(C);
+      uint64_t t1 = wall_clock64(); pr[0] += t1 - t0; t0 = t1;
+    }
     // PMV move mix with one effect type (src/sampler.cpp:468-476); with a
     // learned table, a learned swap first with probability p_learned
     int move;
@@ -1494,12 +1608,23 @@ __device__ void run_iterations(Ctx<KMAX>& C, ChainScal& CS, uint64_t n_iter) {
     } else {
       if (R.c_nloci > 0 && C.D.p > 2) jd = move_statechange(C, adapt);
     }
+    {
+      const uint64_t t1 = wall_clock64(), dt = t1 - t0;
+      if (move == 0) pr[1] += dt;
+      else if (move == 1) pr[2] += dt;
+      else if (move == 2) pr[3] += dt;
+      else pr[4] += dt;
+      t0 = t1;
+    }
     if (threadIdx.x == 0) {
       CS.n_moves[move]++;
       CS.n_acpt[move] += jd > 0;
       CS.n_accepted += jd > 0;
     }
-    if ((iter + 1) % K.thin == 0 || (iter + 2) % K.n_st == 0) sample_beta_sigma2(C);
+    if ((iter + 1) % K.thin == 0 || (iter + 2) % K.n_st == 0) {
+      sample_beta_sigma2(C);
+      uint64_t t1 = wall_clock64(); pr[5] += t1 - t0; t0 = t1;
+    }
     if (K.trace_iter && threadIdx.x == 0) {
       uint64_t t = CS.tr_count;
       if (t < K.tr_cap) {
@@ -1527,6 +1652,16 @@ __device__ void run_iterations(Ctx<KMAX>& C, ChainScal& CS, uint64_t n_iter) {
         }
         CS.pves[0] = pves[0]; CS.pves[1] = pves[1]; CS.pves[2] = pves[2];
       }
+      uint64_t t1 = wall_clock64(); pr[6] += t1 - t0; t0 = t1;
+    }
+    pr[7] += wall_clock64() - t_it;
+    if ((iter + 1) % K.n_rao == 0 || iter + 1 == it0 + n_iter) {
+      if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) CS.prof[q] += pr[q];
+      }
+#pragma unroll
+      for (int q = 0; q < 8; ++q) pr[q] = 0;
     }
     if ((iter + 1) % K.n_rao == 0) {
       // decide the RB block now; the scan and proposal update run as
@@ -1578,7 +1713,7 @@ __device__ void run_iterations(Ctx<KMAX>& C, ChainScal& CS, uint64_t n_iter) {
 }
 
 template <int KMAX>
-__global__ __launch_bounds__(256) void k_chains(KernelArgs args) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_chains(KernelArgs args) {
   __shared__ Smem<KMAX> S;
   int c = blockIdx.x;
   if (c >= args.n_chains) return;
@@ -1587,7 +1722,7 @@ __global__ __launch_bounds__(256) void k_chains(KernelArgs args) {
   const DevData& D = args.data[K.ds];
   const ChainArrays& A = args.arrays[c];
   Reg R;
-  Ctx<KMAX> C{S, R, D, K, A};
+  Ctx<KMAX> C{S, R, D, K, A, CS.prof};
   if (args.mode == 1) {
     // setup (src/sampler.cpp:591-620): state was initialised on the host
     // with an empty model; compute its factor, then sample
@@ -1613,7 +1748,7 @@ template __global__ void k_chains<160>(KernelArgs);
 // ------------------------------------------------------- model op kernel --
 
 template <int KMAX>
-__device__ void swapadj_w0(Smem<KMAX>& S, int m, int col) {
+__device__ __forceinline__ void swapadj_w0(Smem<KMAX>& S, int m, int col) {
   if (threadIdx.x == 0) {
     // src/symmmatrix.cpp:220-265 on the packed factor and v
     for (int i = 0; i < col; ++i) {
@@ -1650,7 +1785,7 @@ __global__ __launch_bounds__(256) void k_model_op(ModelOpArgs a) {
   const DevData& D = a.data[K.ds];
   const ChainArrays& A = a.arrays[0];
   Reg R;
-  Ctx<KMAX> C{S, R, D, K, A};
+  Ctx<KMAX> C{S, R, D, K, A, nullptr};
   load_chain(C, CS);
   if (a.op == 0) {
     double d2[2] = {a.d0, a.d1};

@@ -43,12 +43,12 @@ __host__ __device__ inline void rng_block(Rng& r, uint32_t o[4]) {
   if (++r.c0 == 0) ++r.c1;
 }
 
-__device__ inline double rng_u01(Rng& r) {
+__device__ __forceinline__ double rng_u01(Rng& r) {
   uint32_t o[4]; rng_block(r, o);
   return words_u01(o[0], o[1]);
 }
 
-__device__ inline double rng_normal(Rng& r) {
+__device__ __forceinline__ double rng_normal(Rng& r) {
   uint32_t o[4]; rng_block(r, o);
   double ua = words_u01(o[0], o[1]), ub = words_u01(o[2], o[3]);
   return sqrt(-2.0 * log(1.0 - ua)) * cos(6.283185307179586 * ub);
@@ -57,7 +57,7 @@ __device__ inline double rng_normal(Rng& r) {
 // trials are capped so that a non-finite shape can never hang a wavefront;
 // at the acceptance rate of the method (> 95 %) the cap is never reached
 #define BMG_MAX_TRIALS 100000
-__device__ inline double rng_gamma_ge1(Rng& r, double a) {
+__device__ __forceinline__ double rng_gamma_ge1(Rng& r, double a) {
   double d = a - 1.0 / 3.0;
   double c = 1.0 / sqrt(9.0 * d);
   for (int trial = 0; trial < BMG_MAX_TRIALS; ++trial) {
@@ -73,7 +73,7 @@ __device__ inline double rng_gamma_ge1(Rng& r, double a) {
   return NAN;
 }
 
-__device__ inline double rng_gamma(Rng& r, double a) {
+__device__ __forceinline__ double rng_gamma_impl(Rng& r, double a) {
   if (a < 1.0) {
     double g = rng_gamma_ge1(r, a + 1.0);
     double u = rng_u01(r);
@@ -82,8 +82,22 @@ __device__ inline double rng_gamma(Rng& r, double a) {
   return rng_gamma_ge1(r, a);
 }
 
+// The gamma draw is one out-of-line function that takes the stream by value
+// and returns the advanced counter, so that a caller's stream (and the state
+// it sits in) stays in registers.
+struct GammaDraw { double v; uint32_t c0, c1; };
+static __device__ __noinline__ GammaDraw rng_gamma_by_value(Rng r, double a) {
+  double v = rng_gamma_impl(r, a);
+  return GammaDraw{v, r.c0, r.c1};
+}
+__device__ __forceinline__ double rng_gamma(Rng& r, double a) {
+  GammaDraw g = rng_gamma_by_value(r, a);
+  r.c0 = g.c0; r.c1 = g.c1;
+  return g.v;
+}
+
 // scaled inverse chi^2, redrawn until finite and positive (src/rand.hpp:85-97)
-__device__ inline double rng_sinvchi2(Rng& r, double nu, double s2) {
+__device__ __forceinline__ double rng_sinvchi2(Rng& r, double nu, double s2) {
   double val = -1.0;
   for (int trial = 0; trial < BMG_MAX_TRIALS && (val <= 0 || !isfinite(val)); ++trial)
     val = nu * s2 / (2.0 * rng_gamma(r, 0.5 * nu));
@@ -96,10 +110,40 @@ __device__ inline Rng rng_make(uint32_t seed, uint32_t chain, uint32_t c2, uint3
 }
 
 // ------------------------------------------------------------ reductions --
-__device__ inline double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+// Wave sums with DPP: row_shr 1, 2, 4, 8 leave each row's total in its
+// lane 15, row_bcast 15 and 31 carry them up to lane 63, whose value every
+// lane then reads.  Six dependent VALU steps instead of six LDS permutes.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ int dpp_i32(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, ROWS, 0xf, false);
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_f64(double v) {
+  int2 h = *reinterpret_cast<int2*>(&v);
+  h.x = __builtin_amdgcn_update_dpp(0, h.x, CTRL, ROWS, 0xf, false);
+  h.y = __builtin_amdgcn_update_dpp(0, h.y, CTRL, ROWS, 0xf, false);
+  return *reinterpret_cast<double*>(&h);
+}
+__device__ __forceinline__ int wave_sum_i32(int v) {
+  v += dpp_i32<0x111, 0xf>(v);
+  v += dpp_i32<0x112, 0xf>(v);
+  v += dpp_i32<0x114, 0xf>(v);
+  v += dpp_i32<0x118, 0xf>(v);
+  v += dpp_i32<0x142, 0xa>(v);
+  v += dpp_i32<0x143, 0xc>(v);
+  return __builtin_amdgcn_readlane(v, 63);
+}
+__device__ __forceinline__ double wave_sum(double v) {
+  v += dpp_f64<0x111, 0xf>(v);
+  v += dpp_f64<0x112, 0xf>(v);
+  v += dpp_f64<0x114, 0xf>(v);
+  v += dpp_f64<0x118, 0xf>(v);
+  v += dpp_f64<0x142, 0xa>(v);
+  v += dpp_f64<0x143, 0xc>(v);
+  int2 h = *reinterpret_cast<int2*>(&v);
+  h.x = __builtin_amdgcn_readlane(h.x, 63);
+  h.y = __builtin_amdgcn_readlane(h.y, 63);
+  return *reinterpret_cast<double*>(&h);
 }
 __device__ inline long long wave_sum_i64(long long v) {
 #pragma unroll

@@ -253,6 +253,12 @@ int bmg_batch_rb(bmg_batch* b, int chain, const double* y_hat, double* p_r);
 /* event timing summed over every bmg_batch_run since set-up: total ms, ms
  * in chain kernels, ms in RB blocks, number of scan launches */
 int bmg_batch_timing(bmg_batch* b, double out[4]);
+/* milliseconds one chain's workgroup spent, since set-up, in: tau2/alpha
+ * refreshes, add/remove moves, swaps, state changes, learned swaps,
+ * beta/sigma2 draws, PVE + thinning, whole iterations; then within term
+ * additions: x.y, X^T x, the triangular solve, the rest; then the number of
+ * additions, the number of removals, ms in removals, 0 (device clock) */
+int bmg_batch_profile(bmg_batch* b, int chain, double out[16]);
 
 /* --------------------------------------- learned-proposal network ----- */
 /* No counterpart in the reference.  The network of the learned proposal:
