@@ -1,5 +1,5 @@
 #!/usr/bin/env python3
-"""PIP ESS/s and MCMC iterations/s of the device PMV sampler, batched vs plain.
+"""PIP ESS/s and MCMC iterations/s of the device PMV sampler: learned+batched vs plain.
 
 Workload: n=10,000 individuals, p=500,000 SNPs, AR(1) LD blocks (rho ~
 U(0.7, 0.95), block length U{20..100}), MAF ~ U(0.02, 0.5), 100 causal SNPs,
@@ -7,16 +7,22 @@ h2 = 0.5, additive coding, generated on the GPU from a fixed seed.
 
 A "step" is one `Batch.run(iters_per_step)`: every chain of the batch does
 `iters_per_step` MCMC iterations (moves, tau2/alpha refreshes, thinning and
-the Rao-Blackwellised all-SNP scans of its schedule).  Two batches run
-back to back on the same dataset:
+the Rao-Blackwellised all-SNP scans of its schedule).  Batches run back to
+back on the same dataset, each after `--warmup` untimed steps (which cover
+the burn-in of 4 RB blocks, so adaptation and the learned table are frozen
+before the timed steps):
 
-* plain:   1 chain (the reference's one-chain-per-thread sampler on the GPU);
-* batched: `--chains` chains (16) advanced together in one kernel.
+* learned_batched (with --learned): `--chains` chains (16) in one kernel,
+  plain PMV moves plus the learned swap (network trained online during
+  burn-in);
+* batched:         `--chains` chains with the plain moves only;
+* plain:           1 chain, plain moves (the reference's sampler on the GPU).
 
 PIP ESS (median over the monitored SNPs: the causal SNPs plus those with a
 pooled PIP in [0.05, 0.95]) is Geyer's initial-monotone-sequence ESS of the
 thinned inclusion indicators of the timed steps, summed over chains, per
-wall second of the timed steps.  `value` is the batched figure.
+wall second of the timed steps.  `value` is the learned_batched figure with
+--learned, else the batched one.
 
 `roofline` covers the all-SNP X^T r scan kernel (HBM-bound) at the launch
 shape the batch uses (3 residual vectors x 5 digit planes), timed with HIP
@@ -59,17 +65,21 @@ def pooled_counts(th, first, p):
 
 
 def run_batch(bmg, synth, ctx, n_chains, steps, warmup, ips, seed0, batch_cfg=None):
+    total = (warmup + steps) * ips + 1000     # sizes the trace buffers
     cfgs = [synth.sampler_cfg(N, P, N_CAUSAL, types="A", seed=seed0 + c, chain=c,
-                              do_n_iter=10 ** 8, n_rao_burnin=4)
+                              do_n_iter=total, n_rao_burnin=4)
             for c in range(n_chains)]
+    bc = dict(batch_cfg or {})
+    bc["trace_capacity"] = (total // 10 + 16) * 200    # thinned loci: up to 200 per sample
     t0 = time.perf_counter()
-    b = bmg.Batch([ctx], [0] * n_chains, cfgs, batch_cfg)
+    b = bmg.Batch([ctx], [0] * n_chains, cfgs, bc)
     b.setup()
     t_setup = time.perf_counter() - t0
     for w in range(warmup):
         b.run(ips)
         log("%d chains warmup step %d/%d size %d" % (n_chains, w + 1, warmup, b.state(0)["size"]))
     first = [len(b.thinned(c)["sizes"]) for c in range(n_chains)]
+    learn = b.learn_state(0) if (batch_cfg or {}).get("proposal") == 1 else None
     tm0 = b.timing()
     b.sync()
     t0 = time.perf_counter()
@@ -82,8 +92,17 @@ def run_batch(bmg, synth, ctx, n_chains, steps, warmup, ips, seed0, batch_cfg=No
     ths = [b.thinned(c) for c in range(n_chains)]
     sizes = [b.state(c)["size"] for c in range(n_chains)]
     acc = [b.state(c)["n_accepted"] / max(b.state(c)["n_iter"], 1) for c in range(n_chains)]
+    if learn is not None:
+        tr = b.trace(0)
+        mv, jd = tr["move"], tr["jd"]
+        k0 = len(mv) - steps * ips      # first timed iteration
+        learned = mv[k0:] == 3
+        learn = dict(rounds=learn["rounds"], frozen=learn["frozen"], labels=learn["labels"],
+                     window=learn["H"], learned_swap_share=round(float(learned.mean()), 3),
+                     learned_swap_accept=round(float((jd[k0:][learned] > 0).mean()), 3)
+                     if learned.any() else None)
     b.close()
-    return dict(wall=wall, setup=t_setup, ths=ths, first=first, sizes=sizes, acc=acc,
+    return dict(wall=wall, setup=t_setup, ths=ths, first=first, sizes=sizes, acc=acc, learn=learn,
                 scan_ms=tm1["scan_ms"] - tm0["scan_ms"], chain_ms=tm1["chain_ms"] - tm0["chain_ms"],
                 n_scans=tm1["n_scans"] - tm0["n_scans"])
 
@@ -180,6 +199,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the band and scan timings")
+    ap.add_argument("--learned", action="store_true",
+                    help="also run the learned-swap batch and make it the headline (stalled at this "
+                         "size in one run; kept off until its cause is found)")
+    ap.add_argument("--full-chains", type=int, default=0,
+                    help="also run the learned batch with this many chains (e.g. 256: one per CU)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="time only the scan, band and copy kernels (for profiler runs)")
     args = ap.parse_args()
@@ -209,25 +233,43 @@ def main():
         return
 
     ips = args.iters_per_step
-    bat = run_batch(bmg, synth, ctx, args.chains, args.steps, args.warmup, ips, 100)
-    pl = None if args.no_plain else run_batch(bmg, synth, ctx, 1, args.steps, args.warmup, ips, 100)
+    runs = {}
+    if args.learned:
+        runs["learned_batched"] = run_batch(bmg, synth, ctx, args.chains, args.steps, args.warmup, ips,
+                                            100, dict(proposal=1, trace_every_iter=1))
+    runs["batched"] = run_batch(bmg, synth, ctx, args.chains, args.steps, args.warmup, ips, 100,
+                                dict(proposal=0))
+    if not args.no_plain:
+        runs["plain"] = run_batch(bmg, synth, ctx, 1, args.steps, args.warmup, ips, 100,
+                                  dict(proposal=0))
+    if args.full_chains > 0:
+        runs["batched_full"] = run_batch(bmg, synth, ctx, args.full_chains, args.steps, args.warmup, ips,
+                                         100, dict(proposal=1 if args.learned else 0))
 
     counts = np.zeros(P, np.int64)
     total = 0
-    for run in [bat] + ([pl] if pl else []):
+    for run in runs.values():
         for th, f in zip(run["ths"], run["first"]):
             c, t = pooled_counts(th, f, P)
             counts += c
             total += t
     pip = counts / max(total, 1)
     monitored = np.union1d(causal, np.nonzero((pip >= 0.05) & (pip <= 0.95))[0])
-    b_med, b_min = ess_of(bat, monitored, bat["wall"])
-    res_pl = None
-    if pl:
-        p_med, p_min = ess_of(pl, monitored, pl["wall"])
-        res_pl = dict(ess_per_s_median=p_med, ess_per_s_min=p_min,
-                      iters_per_s=ips * args.steps / pl["wall"],
-                      ms_per_step=pl["wall"] / args.steps * 1e3, model_size=pl["sizes"][0])
+    summary = {}
+    for name, run in runs.items():
+        med, mn = ess_of(run, monitored, run["wall"])
+        nch = len(run["sizes"])
+        summary[name] = dict(chains=nch, ess_per_s_median=round(med, 3), ess_per_s_min=round(mn, 3),
+                             iters_per_s=round(ips * args.steps * nch / run["wall"], 1),
+                             ms_per_step=round(run["wall"] / args.steps * 1e3, 2),
+                             model_size_mean=round(float(np.mean(run["sizes"])), 1),
+                             accept_rate_mean=round(float(np.mean(run["acc"])), 4),
+                             rb_block_ms=round(run["scan_ms"], 1), chain_kernel_ms=round(run["chain_ms"], 1))
+        if run.get("learn"):
+            summary[name]["learner"] = run["learn"]
+    head = "learned_batched" if args.learned else "batched"
+    lb = summary[head]
+    b_med = lb["ess_per_s_median"]
 
     roof = None
     extra = {}
@@ -254,7 +296,8 @@ def main():
                           "warm part; setup %.1f s excluded" % (c["iters"], c["samples"], c["wall"],
                                                                c["setup_s"]))
 
-    wall = bat["wall"]
+    wall = runs[head]["wall"]
+    pl = summary.get("plain")
     line = {
         "metric": "PIP ESS/sec + MCMC iters/sec, learned+batched vs plain, 1 MI355X",
         "value": round(b_med, 3),
@@ -265,19 +308,19 @@ def main():
         "dtype": "f64", "data": "synthetic (generated on device, seeded)",
         "config": {"workload": "PMV MCMC over SNP inclusion, n=10000 p=500000 AR(1) LD, "
                                "100 causal, h2=0.5, additive",
-                   "chains": args.chains, "iters_per_step": ips, "proposal": "plain"},
-        "batched": dict(ess_per_s_median=round(b_med, 3), ess_per_s_min=round(b_min, 3),
-                        iters_per_s=round(ips * args.steps * args.chains / wall, 1),
-                        iters_per_s_per_chain=round(ips * args.steps / wall, 1),
-                        model_size_mean=float(np.mean(bat["sizes"])),
-                        accept_rate_mean=round(float(np.mean(bat["acc"])), 4),
-                        scan_ms=round(bat["scan_ms"], 1), chain_ms=round(bat["chain_ms"], 1),
-                        n_scans=bat["n_scans"]),
-        "plain": res_pl,
-        "batched_vs_plain_ess": (round(b_med / res_pl["ess_per_s_median"], 2)
-                                 if res_pl and res_pl["ess_per_s_median"] > 0 else None),
+                   "chains": args.chains, "iters_per_step": ips,
+                   "proposal": ("plain moves + learned swap (MLP 16-256-256-1, bf16)" if args.learned
+                                else "plain PMV moves (learned swap not run at this size)")},
+        "iters_per_s": lb["iters_per_s"],
+        "runs": summary,
+        "headline_run": head,
+        "headline_vs_plain_ess": (round(b_med / pl["ess_per_s_median"], 2)
+                                  if pl and pl["ess_per_s_median"] > 0 else None),
+        "headline_vs_plain_iters": (round(lb["iters_per_s"] / pl["iters_per_s"], 2)
+                                    if pl and pl["iters_per_s"] > 0 else None),
         "monitored_snps": int(len(monitored)),
-        "setup_s": dict(dataset=round(t_data, 2), batch=round(bat["setup"], 2)),
+        "setup_s": dict(dataset=round(t_data, 2),
+                        batch=round(runs[head]["setup"], 2)),
         "roofline": roof,
         "cpu_baseline": cpu,
     }

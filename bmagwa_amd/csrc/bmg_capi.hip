@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cstddef>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -529,11 +530,36 @@ static int learn_after_block(bmg_batch* b, int ds, const std::vector<int>& all,
   return BMG_OK;
 }
 
+// a chain whose current model became non-finite has stopped iterating: the
+// run fails with the chain's index (the oracle stops at the same point)
+static int numeric_error(const std::vector<ChainScal>& cs) {
+  for (size_t c = 0; c < cs.size(); ++c)
+    if (cs[c].err & ERR_TRACE) {
+      set_error("chain " + std::to_string(c) + ": trace buffer full (raise trace_capacity or do_n_iter)");
+      return BMG_E_CAPACITY;
+    }
+  for (size_t c = 0; c < cs.size(); ++c)
+    if (cs[c].err & ERR_STOP) {
+      const ChainScal& x = cs[c];
+      char buf[384];
+      snprintf(buf, sizeof(buf),
+               "chain %zu stopped at iteration %llu (flags %d:%s%s%s%s): loglik %.6g syx %.6g sigma2 %.6g "
+               "alpha %.6g columns %d loci %d",
+               c, (unsigned long long)x.n_iter, x.err, (x.err & ERR_NONFINITE) ? " refresh" : "",
+               (x.err & ERR_SIGMA2) ? " sigma2" : "", (x.err & ERR_ALPHA) ? " alpha" : "",
+               (x.err & ERR_WALK) ? " walk" : "", x.ll, x.syx, x.sigma2, x.alpha, x.cols, x.nloci);
+      set_error(buf);
+      return BMG_E_NUMERIC;
+    }
+  return BMG_OK;
+}
+
 static int rb_block(bmg_batch* b) {
   std::vector<ChainScal> cs(b->n_chains);
   BMG_HIP(hipMemcpyAsync(cs.data(), b->d_scal, sizeof(ChainScal) * b->n_chains, hipMemcpyDeviceToHost,
                          b->stream));
   BMG_HIP(hipStreamSynchronize(b->stream));
+  if (int rc = numeric_error(cs)) return rc;
   for (size_t ds = 0; ds < b->ctxs.size(); ++ds) {
     std::vector<int> scan_list, all;
     for (int c = 0; c < b->n_chains; ++c) {
@@ -875,7 +901,9 @@ extern "C" int bmg_batch_run(bmg_batch* b, uint64_t n_iter) {
   if (rc) return rc;
   hipError_t e = hipStreamSynchronize(b->stream);
   if (e != hipSuccess) return hip_fail(e, "bmg_batch_run");
-  return BMG_OK;
+  std::vector<ChainScal> cs(b->n_chains);
+  BMG_HIP(hipMemcpy(cs.data(), b->d_scal, sizeof(ChainScal) * b->n_chains, hipMemcpyDeviceToHost));
+  return numeric_error(cs);
 }
 
 // learned swap on/off for the chains of dataset ds (see move_learned_swap)
@@ -1016,7 +1044,6 @@ extern "C" int bmg_batch_state(bmg_batch* b, int chain, double out[8]) {
   if (rc) return rc;
   out[0] = s.ll; out[1] = s.nloci; out[2] = s.sigma2; out[3] = s.alpha; out[4] = s.p_ms;
   out[5] = (double)s.n_accepted; out[6] = (double)b->iter; out[7] = (double)s.th_loci_len;
-  if (s.err) { set_error("chain error flags " + std::to_string(s.err)); return BMG_E_CAPACITY; }
   return BMG_OK;
 }
 

@@ -95,7 +95,15 @@ struct ChainArrays {
   uint32_t* th_size; uint32_t* th_loci; double* th_ll; double* th_s2; double* th_alpha;
 };
 
-enum { ERR_CAPACITY = 1, ERR_TRACE = 2, ERR_TREE = 4, ERR_NONFINITE = 8 };
+// error flags of a chain; every flag from ERR_NONFINITE on stops the chain
+enum {
+  ERR_CAPACITY = 1, ERR_TRACE = 2, ERR_TREE = 4,
+  ERR_NONFINITE = 8,     // tau2/alpha refresh left a non-finite log-likelihood
+  ERR_SIGMA2 = 16,       // sigma2 draw not finite and positive
+  ERR_ALPHA = 32,        // alpha draw did not finish
+  ERR_WALK = 64,         // a neighbourhood walk left the genome
+  ERR_STOP = ERR_NONFINITE | ERR_SIGMA2 | ERR_ALPHA | ERR_WALK
+};
 
 }  // namespace bmg
 

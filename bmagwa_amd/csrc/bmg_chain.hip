@@ -24,6 +24,11 @@ using namespace bmg;
 
 __device__ __forceinline__ int P(int c) { return c * (c + 1) / 2; }
 
+// A log-likelihood that is not finite (+inf when the residual term reaches
+// zero, NaN) can only come from rounding in a near-singular design; it is
+// taken as -inf, so such a proposal is rejected (as in the oracle).
+__device__ __forceinline__ double finite_ll(double v) { return isfinite(v) ? v : -INFINITY; }
+
 // element i (runtime) of a small register array without dynamic indexing,
 // which would place the whole per-lane state in scratch memory.  Each
 // element passes through an empty register constraint, so the compiler
@@ -579,7 +584,7 @@ __device__ __forceinline__ void update_on_add(Ctx<KMAX>& C, uint32_t ind, int mo
   R.syx = R.syx - vc * vc;
   R.ldq += 0.5 * log(dval);
   R.ldl += log(lcc);
-  R.ll = R.ldq - R.ldl + C.K.minus_n_plus_nu_2 * log(R.syx);
+  R.ll = finite_ll(R.ldq - R.ldl + C.K.minus_n_plus_nu_2 * log(R.syx));
   if (threadIdx.x == 0 && C.prof) {
     const uint64_t te = wall_clock64();
     C.prof[8] += tb - ta; C.prof[9] += tc - tb; C.prof[10] += td - tc; C.prof[11] += te - td;
@@ -676,7 +681,7 @@ __device__ __forceinline__ void update_on_remove_impl(Ctx<KMAX>& C, int x_ind) {
     R.syx = C.K.nus2_plus_yy - vv;
   }
   R.ldq -= 0.5 * log(inv_val);
-  R.ll = R.ldq - R.ldl + C.K.minus_n_plus_nu_2 * log(R.syx);
+  R.ll = finite_ll(R.ldq - R.ldl + C.K.minus_n_plus_nu_2 * log(R.syx));
 }
 
 // src/model.hpp:270-312
@@ -760,7 +765,7 @@ __device__ __forceinline__ void compute_log_likelihood(Ctx<KMAX>& C) {
   R.ldl = block_sum(ld, S.red);
   vv = block_sum(vv, S.red);
   R.syx = C.K.nus2_plus_yy - vv;
-  R.ll = R.ldq - R.ldl + C.K.minus_n_plus_nu_2 * log(R.syx);
+  R.ll = finite_ll(R.ldq - R.ldl + C.K.minus_n_plus_nu_2 * log(R.syx));
   R.mu_computed = 0;
 }
 
@@ -877,7 +882,7 @@ __device__ __forceinline__ void sample_beta_sigma2(Ctx<KMAX>& C) {
   double val = -1.0;
   for (int trial = 0; trial < BMG_MAX_TRIALS && (val <= 0 || !isfinite(val)); ++trial)
     val = nu * s2 / (2.0 * rng_gamma(R.rng, 0.5 * nu));
-  if (!(val > 0) || !isfinite(val)) R.err |= ERR_NONFINITE;
+  if (!(val > 0) || !isfinite(val)) R.err |= ERR_SIGMA2;
   R.sigma2 = val;
   compute_mu_beta(C);
   R.beta_len = R.cols;
@@ -1022,7 +1027,7 @@ __device__ __forceinline__ void sample_alpha_and_tau2(Ctx<KMAX>& C) {
   }
   int trials = 0;
   do {
-    if (++trials > BMG_MAX_TRIALS) { R.err |= ERR_NONFINITE; a = R.alpha; break; }
+    if (++trials > BMG_MAX_TRIALS) { R.err |= ERR_ALPHA; a = R.alpha; break; }
     if (nterms > 0) {
       if ((a <= 0.0) || log(rng_u01(R.rng)) <= -2.0 * K.mu_alpha * a) a = -a;
       double as2 = a * R.sigma2;
@@ -1245,7 +1250,7 @@ __device__ __forceinline__ int move_switch(Ctx<KMAX>& C, double* adapt) {
         }
       }
     }
-    if (ia < 0 || ia >= m_g) { R.err |= ERR_NONFINITE; ia = ia < 0 ? 0 : m_g - 1; }
+    if (ia < 0 || ia >= m_g) { R.err |= ERR_WALK; ia = ia < 0 ? 0 : m_g - 1; }
     int t_ind = S.tind[mir];
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -1407,7 +1412,7 @@ __device__ __forceinline__ int move_statechange(Ctx<KMAX>& C, double* adapt) {
   for (int i = 0; i < ms; ++i) {
     uint64_t ind = S.sc_perm[i] + ic + 1;
     if (ind > wrap) ind = ic - (ind - wrap);
-    if (ind >= m_g) { R.err |= ERR_NONFINITE; ind = m_g - 1; }
+    if (ind >= m_g) { R.err |= ERR_WALK; ind = m_g - 1; }
     int fl = find_locus(C, ind);
     if (threadIdx.x == 0) {
       if (fl < 0) S.mi_add[ms_add] = ind; else S.mi_rem[ms_rem] = ind;
@@ -1567,10 +1572,24 @@ __device__ __forceinline__ void store_chain(Ctx<KMAX>& C, ChainScal& CS) {
 // refreshed model is committed in full
 template <int KMAX>
 __device__ __forceinline__ void refresh_tau2(Ctx<KMAX>& C) {
+  Reg& R = C.R;
+  // a refresh whose factorisation fails is undone (as in the oracle): the
+  // previous tau2 / alpha and the committed model are kept
+  const double al = R.alpha, al2 = R.alpha2;
+  double ita[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) ita[q] = R.ita2[q];
   sample_alpha_and_tau2(C);
   compute_log_likelihood(C);
-  C.R.lmin = 0;
-  commit(C);
+  R.lmin = 0;
+  if (isfinite(R.ll)) {
+    commit(C);
+  } else {
+    restore(C);
+    R.alpha = al; R.alpha2 = al2;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) R.ita2[q] = ita[q];
+  }
 }
 
 template <int KMAX>
@@ -1582,10 +1601,14 @@ __device__ __forceinline__ void run_iterations(Ctx<KMAX>& C, ChainScal& CS, uint
   uint64_t it0 = CS.n_iter;
   uint64_t pr[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   for (uint64_t iter = it0; iter < it0 + n_iter; ++iter) {
+    // a chain whose current model turned non-finite stops here; the host
+    // reports it (the oracle stops with an error at the same point)
+    if (R.err & ERR_STOP) break;
     const uint64_t t_it = wall_clock64();
     uint64_t t0 = t_it;
     if ((iter + 1) % K.n_st == 0) {
       refresh_tau2(C);
+      if (!isfinite(R.c_ll)) R.err |= ERR_NONFINITE;
       uint64_t t1 = wall_clock64(); pr[0] += t1 - t0; t0 = t1;
     }
     // PMV move mix with one effect type (src/sampler.cpp:468-476); with a

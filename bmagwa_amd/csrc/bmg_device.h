@@ -54,9 +54,11 @@ __device__ __forceinline__ double rng_normal(Rng& r) {
   return sqrt(-2.0 * log(1.0 - ua)) * cos(6.283185307179586 * ub);
 }
 
-// trials are capped so that a non-finite shape can never hang a wavefront;
-// at the acceptance rate of the method (> 95 %) the cap is never reached
-#define BMG_MAX_TRIALS 100000
+// Trials are capped so that a non-finite shape cannot stall a wavefront:
+// each trial of these samplers succeeds with probability > 0.9 for any
+// finite positive shape, so 1000 failures in a row never happen in a valid
+// state, while a non-finite state fails fast and is flagged by the caller.
+#define BMG_MAX_TRIALS 1000
 __device__ __forceinline__ double rng_gamma_ge1(Rng& r, double a) {
   double d = a - 1.0 / 3.0;
   double c = 1.0 / sqrt(9.0 * d);

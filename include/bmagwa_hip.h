@@ -30,7 +30,8 @@ enum {
   BMG_E_HIP = -2,         /* HIP runtime error */
   BMG_E_UNSUPPORTED = -3, /* valid for the reference, not built for the GPU */
   BMG_E_CAPACITY = -4,    /* model larger than the compiled capacity */
-  BMG_E_STATE = -5        /* call out of order */
+  BMG_E_STATE = -5,       /* call out of order */
+  BMG_E_NUMERIC = -6      /* a chain's current model became non-finite */
 };
 
 /* effect types, src/data_model.hpp:41 */
@@ -232,7 +233,9 @@ int bmg_batch_learn_static(bmg_batch* b, int ds, float* z, float* pbar, int32_t*
 int bmg_batch_learn_features(bmg_batch* b, int ds, const uint32_t* j, const int32_t* off,
                              int64_t rows, float* out);
 /* per-chain state: loglik, size, sigma2, alpha, p_move_size, n_accepted,
- * n_iter, loci_len */
+ * n_iter, loci_len.  bmg_batch_run fails with BMG_E_NUMERIC when a chain's
+ * current model turned non-finite (that chain stops; the others ran on) and
+ * with BMG_E_CAPACITY when a thinned-trace buffer filled up. */
 int bmg_batch_state(bmg_batch* b, int chain, double out[8]);
 int bmg_batch_loci(bmg_batch* b, int chain, uint32_t* loci, int cap, int* k);
 /* p_rao (RB PIP estimate, src/sampler.cpp:739-757) and p_proposal */

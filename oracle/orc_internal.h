@@ -6,6 +6,13 @@
 
 #include "oracle.h"
 
+/* A log-likelihood that is not finite (+inf when the residual term reaches
+ * zero, NaN) can only come from rounding in a near-singular design; it is
+ * taken as -inf, so such a proposal is rejected.  The reference keeps the
+ * raw value (+inf would be accepted). */
+#include <math.h>
+static inline double orc__finite_ll(double v) { return isfinite(v) ? v : -INFINITY; }
+
 #define ORC_LOG_HALF (-0.69314718055994528622676398299518041312694549560546875)
 
 /* orc_data.c */

@@ -262,8 +262,8 @@ void orc_model_compute_log_likelihood(orc_model* m)
   m->syx_plus_vs2 = m->pr->nus2_plus_yy - dot(m->v, m->v, k);
   m->log_det_invQ_plus_xx = 0;
   for (int i = 0; i < k; ++i) m->log_det_invQ_plus_xx += log(E(m->l, ld, i, i));
-  m->loglik = m->log_det_invQ - m->log_det_invQ_plus_xx
-              + m->pr->minus_n_plus_nu_2 * log(m->syx_plus_vs2);
+  m->loglik = orc__finite_ll(m->log_det_invQ - m->log_det_invQ_plus_xx
+                             + m->pr->minus_n_plus_nu_2 * log(m->syx_plus_vs2));
   m->mu_beta_computed = 0;
 }
 
@@ -297,8 +297,8 @@ static void update_on_add(orc_model* m, uint32_t ind, int model_ind,
     m->syx_plus_vs2 = m->syx_plus_vs2 - vm2;
     m->log_det_invQ += 0.5 * log(dval);
     m->log_det_invQ_plus_xx += log(lcol[col]);
-    m->loglik = m->log_det_invQ - m->log_det_invQ_plus_xx
-                + m->pr->minus_n_plus_nu_2 * log(m->syx_plus_vs2);
+    m->loglik = orc__finite_ll(m->log_det_invQ - m->log_det_invQ_plus_xx
+                               + m->pr->minus_n_plus_nu_2 * log(m->syx_plus_vs2));
   }
 }
 
@@ -356,8 +356,8 @@ static void update_on_remove(orc_model* m, int x_ind, int upd)
       m->syx_plus_vs2 = m->pr->nus2_plus_yy - dot(m->v, m->v, m->cols);
     }
     m->log_det_invQ -= 0.5 * log(inv_val);
-    m->loglik = m->log_det_invQ - m->log_det_invQ_plus_xx
-                + m->pr->minus_n_plus_nu_2 * log(m->syx_plus_vs2);
+    m->loglik = orc__finite_ll(m->log_det_invQ - m->log_det_invQ_plus_xx
+                               + m->pr->minus_n_plus_nu_2 * log(m->syx_plus_vs2));
   }
 }
 

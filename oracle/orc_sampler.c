@@ -1392,10 +1392,22 @@ int orc_sampler_iterate(orc_sampler* s, uint64_t n_iter)
   uint64_t end = s->n_iter + n_iter;
   for (uint64_t iter = s->n_iter; iter < end; ++iter) {
     if ((iter + 1) % o->n_sample_tau2_and_missing == 0) {
+      /* src/sampler.cpp:628-635; a refresh whose factorisation fails (the
+       * reference would continue with ll = -inf and then draw sigma2 from an
+       * infinite scale forever) is undone: the previous tau2 / alpha and the
+       * previous model (still in the working copy) are kept */
+      double al = s->pr.alpha, al2 = s->pr.alpha2, ita[4];
+      memcpy(ita, s->pr.inv_tau2_alpha2, sizeof(ita));
       sampler_sample_missing(s);
       prior_sample_alpha_and_tau2(s, s->cur);
       orc_model_compute_log_likelihood(s->cur);
-      orc_model_copy(s->nw, s->cur);
+      if (isfinite(s->cur->loglik)) {
+        orc_model_copy(s->nw, s->cur);
+      } else {
+        orc_model_copy(s->cur, s->nw);
+        s->pr.alpha = al; s->pr.alpha2 = al2;
+        memcpy(s->pr.inv_tau2_alpha2, ita, sizeof(ita));
+      }
     }
     int move;
     if (s->learn_on && orc_rng_u01(&s->rng) < s->p_learned) move = 3;
