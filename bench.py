@@ -12,17 +12,18 @@ back on the same dataset, each after `--warmup` untimed steps (which cover
 the burn-in of 4 RB blocks, so adaptation and the learned table are frozen
 before the timed steps):
 
-* learned_batched (with --learned): `--chains` chains (16) in one kernel,
-  plain PMV moves plus the learned swap (network trained online during
-  burn-in);
+* learned_batched: `--chains` chains (16) in one kernel, plain PMV moves
+  plus the learned swap (network trained online during burn-in);
 * batched:         `--chains` chains with the plain moves only;
-* plain:           1 chain, plain moves (the reference's sampler on the GPU).
+* plain:           1 chain, plain moves (the reference's sampler on the GPU);
+* batched_full:    `--full-chains` chains (256: one per CU) with the learned
+  swap, to show the whole GPU (reported beside the headline).
 
 PIP ESS (median over the monitored SNPs: the causal SNPs plus those with a
 pooled PIP in [0.05, 0.95]) is Geyer's initial-monotone-sequence ESS of the
 thinned inclusion indicators of the timed steps, summed over chains, per
-wall second of the timed steps.  `value` is the learned_batched figure with
---learned, else the batched one.
+wall second of the timed steps.  `value` is the learned_batched figure
+(the batched one with --no-learned).
 
 `roofline` covers the all-SNP X^T r scan kernel (HBM-bound) at the launch
 shape the batch uses (3 residual vectors x 5 digit planes), timed with HIP
@@ -98,7 +99,8 @@ def run_batch(bmg, synth, ctx, n_chains, steps, warmup, ips, seed0, batch_cfg=No
         k0 = len(mv) - steps * ips      # first timed iteration
         learned = mv[k0:] == 3
         learn = dict(rounds=learn["rounds"], frozen=learn["frozen"], labels=learn["labels"],
-                     window=learn["H"], learned_swap_share=round(float(learned.mean()), 3),
+                     window=learn["H"],
+                     learned_swap_share=round(float(learned.mean()), 3) if len(learned) else None,
                      learned_swap_accept=round(float((jd[k0:][learned] > 0).mean()), 3)
                      if learned.any() else None)
     b.close()
@@ -199,10 +201,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the band and scan timings")
-    ap.add_argument("--learned", action="store_true",
-                    help="also run the learned-swap batch and make it the headline (stalled at this "
-                         "size in one run; kept off until its cause is found)")
-    ap.add_argument("--full-chains", type=int, default=0,
+    ap.add_argument("--no-learned", dest="learned", action="store_false",
+                    help="skip the learned-swap batch (the headline is then the plain batched run)")
+    ap.add_argument("--full-chains", type=int, default=256,
                     help="also run the learned batch with this many chains (e.g. 256: one per CU)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="time only the scan, band and copy kernels (for profiler runs)")
@@ -244,7 +245,7 @@ def main():
                                   dict(proposal=0))
     if args.full_chains > 0:
         runs["batched_full"] = run_batch(bmg, synth, ctx, args.full_chains, args.steps, args.warmup, ips,
-                                         100, dict(proposal=1 if args.learned else 0))
+                                         100, dict(proposal=1 if args.learned else 0, trace_every_iter=1))
 
     counts = np.zeros(P, np.int64)
     total = 0
