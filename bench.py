@@ -92,6 +92,8 @@ def run_batch(bmg, synth, ctx, n_chains, steps, warmup, ips, seed0, batch_cfg=No
     log("%d chains: %d timed steps in %.2f s" % (n_chains, steps, wall))
     ths = [b.thinned(c) for c in range(n_chains)]
     sizes = [b.state(c)["size"] for c in range(n_chains)]
+    # Rao-Blackwellised PIPs averaged over the post-burn-in RB scans, mean of chains
+    p_rao = np.mean([b.p_rao(c) for c in range(n_chains)], axis=0)
     acc = [b.state(c)["n_accepted"] / max(b.state(c)["n_iter"], 1) for c in range(n_chains)]
     if learn is not None:
         tr = b.trace(0)
@@ -105,6 +107,7 @@ def run_batch(bmg, synth, ctx, n_chains, steps, warmup, ips, seed0, batch_cfg=No
                      if learned.any() else None)
     b.close()
     return dict(wall=wall, setup=t_setup, ths=ths, first=first, sizes=sizes, acc=acc, learn=learn,
+                p_rao=p_rao,
                 scan_ms=tm1["scan_ms"] - tm0["scan_ms"], chain_ms=tm1["chain_ms"] - tm0["chain_ms"],
                 n_scans=tm1["n_scans"] - tm0["n_scans"])
 
@@ -134,7 +137,8 @@ def scan_roofline(bmg, ctx):
     out_dev.free()
     bytes_per_launch = P * ((N + 3) // 4) + nvec * (N * 8 + P * 8)
     gbs = bytes_per_launch / (ms * 1e-3) / 1e9
-    return dict(ms=ms, bytes=bytes_per_launch, gbs=gbs, nvec=nvec, digits=digits)
+    return dict(ms=ms, bytes=bytes_per_launch, gbs=gbs, nvec=nvec, digits=digits,
+                shape=ctx.scan_shape(nvec, digits))
 
 
 def band_mfma(bmg, ctx, W=500):
@@ -271,6 +275,13 @@ def main():
     head = "learned_batched" if args.learned else "batched"
     lb = summary[head]
     b_med = lb["ess_per_s_median"]
+    # PIPs of the headline run against the plain-move runs (RB estimates)
+    pip_check = {}
+    for ref in ["batched", "batched_full"]:
+        if ref in runs and ref != head:
+            dpip = np.abs(runs[head]["p_rao"] - runs[ref]["p_rao"])
+            pip_check["vs_" + ref] = dict(max_abs_diff=round(float(dpip.max()), 4),
+                                          max_abs_diff_monitored=round(float(dpip[monitored].max()), 4))
 
     roof = None
     extra = {}
@@ -278,7 +289,8 @@ def main():
         sc = scan_roofline(bmg, ctx)
         roof = dict(bound="hbm", achieved=round(sc["gbs"], 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(sc["gbs"] / HBM_PEAK_GBS, 4), traffic=None,
-                    kernel="k_scan_mfma<2> (3 vectors x 5 digit planes)",
+                    kernel="k_scan_mfma<%d> (3 vectors x 5 digit planes)" % sc["shape"]["tiles_per_wave"],
+                    autotuned_shape=sc["shape"],
                     ms_per_launch=round(sc["ms"], 4), algorithmic_bytes=sc["bytes"])
         bd = band_mfma(bmg, ctx)
         extra["band_xtx_int8"] = dict(W=bd["W"], ms=round(bd["ms"], 3), tops=round(bd["tops"], 1),
@@ -320,6 +332,7 @@ def main():
         "headline_vs_plain_iters": (round(lb["iters_per_s"] / pl["iters_per_s"], 2)
                                     if pl and pl["iters_per_s"] > 0 else None),
         "monitored_snps": int(len(monitored)),
+        "pip_check": pip_check,
         "setup_s": dict(dataset=round(t_data, 2),
                         batch=round(runs[head]["setup"], 2)),
         "roofline": roof,

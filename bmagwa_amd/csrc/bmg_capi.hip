@@ -266,6 +266,29 @@ extern "C" int bmg_scan_xtr_timed(bmg_ctx* c, const double* r_dev, int nvec, int
   return launch_scan_timed(c, r_dev, nvec, coding, digits, out_dev, reps, ms_per_launch, s);
 }
 
+extern "C" int bmg_scan_shape(bmg_ctx* c, int nvec, int digits, int32_t* out) {
+  if (!c || !out || nvec < 1 || digits < 1 || nvec * digits > 16) { set_error("bad argument"); return BMG_E_ARG; }
+  const int NC = nvec * digits;
+  out[0] = c->scan_nt[NC];
+  out[1] = (int32_t)c->scan_tw[NC];
+  out[2] = c->scan_tw[NC] ? (int32_t)((c->W32 + c->scan_tw[NC] - 1) / c->scan_tw[NC]) : 0;
+  out[3] = (int32_t)c->scan_gx[NC];
+  return BMG_OK;
+}
+
+extern "C" int bmg_scan_set_shape(bmg_ctx* c, int nvec, int digits, int nt, uint32_t tile_words,
+                                  uint32_t workgroups) {
+  if (!c || nvec < 1 || digits < 1 || nvec * digits > 16 || (nt != 2 && nt != 4) || tile_words < 16 ||
+      tile_words % 16 || workgroups < 1 || (size_t)tile_words * 16 * nvec * digits > 163840) {
+    set_error("bad scan shape"); return BMG_E_ARG;
+  }
+  const int NC = nvec * digits;
+  c->scan_nt[NC] = nt;
+  c->scan_tw[NC] = tile_words;
+  c->scan_gx[NC] = workgroups;
+  return BMG_OK;
+}
+
 extern "C" int bmg_band_xtx_timed(bmg_ctx* c, uint32_t W, int ca, int cb, int32_t* out_dev, int reps,
                                   double* ms_per_launch, void* stream) {
   if (!c || !out_dev || !ms_per_launch || ca < 0 || ca > 3 || cb < 0 || cb > 3 || reps < 1) {

@@ -121,6 +121,37 @@ def test_scan_xtr_is_exact_on_representable_residuals(bmg):
     assert np.array_equal(got2[1], 2 * got2[0])
 
 
+def test_scan_shapes_all_exact(bmg):
+    """Every shape the scan autotuner may pick (tiles per wave, LDS tile
+    width, so one or several individual tiles added atomically, and the
+    number of workgroups looping over the SNP blocks) gives the exact dot
+    products on representable residuals; the tuned shape is one of them."""
+    rng = np.random.default_rng(11)
+    n, p = 20011, 1500
+    G, body = random_body(rng, n, p)
+    ctx = bmg.Ctx(n, p, body, rng.normal(size=n))
+    k = rng.integers(-1000, 1000, size=(3, n)).astype(np.float64)
+    R = k / 1024.0
+    ref = np.stack([(G.astype(np.float64) @ R[v]) for v in range(3)])
+    tuned = ctx.scan_xtr(R, coding=0, digits=5)
+    assert np.array_equal(tuned, ref)
+    shape = ctx.scan_shape(3, 5)
+    assert shape["tiles_per_wave"] in (2, 4) and shape["workgroups"] >= 1
+    w32 = ((n + 15) // 16 + 3) // 4 * 4
+    for nt in (2, 4):
+        for tw in (16, 272, 672):
+            for gx in (1, 5, 4096):
+                ctx.scan_set_shape(3, 5, nt, tw, gx)
+                got = ctx.scan_xtr(R, coding=0, digits=5)
+                assert np.array_equal(got, ref), (nt, tw, gx)
+                sh = ctx.scan_shape(3, 5)
+                assert sh["n_tiles"] == -(-w32 // tw)
+    with pytest.raises(bmg.BmgError):
+        ctx.scan_set_shape(3, 5, 3, 272, 8)
+    with pytest.raises(bmg.BmgError):
+        ctx.scan_set_shape(3, 5, 2, 688, 8)
+
+
 @pytest.mark.parametrize("n,p,ld", [(37, 50, None), (203, 300, (0.9, 0.9, 10, 40)),
                                     (64, 500, (0.7, 0.95, 20, 100))])
 def test_synthetic_genotypes_match_oracle(bmg, n, p, ld):
