@@ -128,7 +128,7 @@ SIGNATURES = {
     "bmg_ctx_n_pad": (C.c_int, _P, _U32),
     "bmg_scan_xtr_timed": (C.c_int, _P, _P, C.c_int, C.c_int, C.c_int, _P, C.c_int, _D, _P),
     "bmg_scan_shape": (C.c_int, _P, C.c_int, C.c_int, _P),
-    "bmg_scan_set_shape": (C.c_int, _P, C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_uint32),
+    "bmg_scan_set_shape": (C.c_int, _P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_uint32),
     "bmg_band_xtx_timed": (C.c_int, _P, C.c_uint32, C.c_int, C.c_int, _P, C.c_int, _D, _P),
     "bmg_dev_alloc": (C.c_int, C.c_int, C.c_uint64, _PP),
     "bmg_dev_free": (C.c_int, C.c_int, _P),
@@ -268,15 +268,15 @@ class Ctx:
 
     def scan_shape(self, nvec=1, digits=5):
         """Autotuned scan shape for nvec * digits digit columns: dict with
-        tiles_per_wave, tile_words, n_tiles, workgroups (zeros before the
-        first scan of that width)."""
-        out = np.zeros(4, dtype=np.int32)
+        tiles_per_wave, tile_words, n_tiles, workgroups, waves (zeros before
+        the first scan of that width)."""
+        out = np.zeros(5, dtype=np.int32)
         check(lib().bmg_scan_shape(self.h, nvec, digits, _p(out, C.c_int32)))
-        return dict(zip(("tiles_per_wave", "tile_words", "n_tiles", "workgroups"), out.tolist()))
+        return dict(zip(("tiles_per_wave", "tile_words", "n_tiles", "workgroups", "waves"), out.tolist()))
 
-    def scan_set_shape(self, nvec, digits, tiles_per_wave, tile_words, workgroups):
+    def scan_set_shape(self, nvec, digits, tiles_per_wave, tile_words, workgroups, waves=4):
         """Pin the scan shape for nvec * digits columns (skips the tuner)."""
-        check(lib().bmg_scan_set_shape(self.h, nvec, digits, tiles_per_wave, tile_words, workgroups))
+        check(lib().bmg_scan_set_shape(self.h, nvec, digits, tiles_per_wave, waves, tile_words, workgroups))
 
     def band_timed(self, W, ca, cb, out_dev, reps=3, stream=None):
         """Mean duration (ms) of the band kernel over `reps` launches."""

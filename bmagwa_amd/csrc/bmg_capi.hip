@@ -273,17 +273,20 @@ extern "C" int bmg_scan_shape(bmg_ctx* c, int nvec, int digits, int32_t* out) {
   out[1] = (int32_t)c->scan_tw[NC];
   out[2] = c->scan_tw[NC] ? (int32_t)((c->W32 + c->scan_tw[NC] - 1) / c->scan_tw[NC]) : 0;
   out[3] = (int32_t)c->scan_gx[NC];
+  out[4] = c->scan_wv[NC];
   return BMG_OK;
 }
 
-extern "C" int bmg_scan_set_shape(bmg_ctx* c, int nvec, int digits, int nt, uint32_t tile_words,
+extern "C" int bmg_scan_set_shape(bmg_ctx* c, int nvec, int digits, int nt, int waves, uint32_t tile_words,
                                   uint32_t workgroups) {
-  if (!c || nvec < 1 || digits < 1 || nvec * digits > 16 || (nt != 2 && nt != 4) || tile_words < 16 ||
-      tile_words % 16 || workgroups < 1 || (size_t)tile_words * 16 * nvec * digits > 163840) {
+  if (!c || nvec < 1 || digits < 1 || nvec * digits > 16 || (nt != 2 && nt != 4) ||
+      (waves != 4 && waves != 8 && waves != 16) || tile_words < 16 || tile_words % 16 || workgroups < 1 ||
+      ((size_t)std::min<uint32_t>(tile_words, (c->W32 + 15) / 16 * 16) * 16 + 16) * nvec * digits > 163840) {
     set_error("bad scan shape"); return BMG_E_ARG;
   }
   const int NC = nvec * digits;
   c->scan_nt[NC] = nt;
+  c->scan_wv[NC] = waves;
   c->scan_tw[NC] = tile_words;
   c->scan_gx[NC] = workgroups;
   return BMG_OK;

@@ -2,6 +2,8 @@
 // moments, synthetic genotypes, the X^T r scan and band X^T X (int8 MFMA).
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <vector>
 
 #include "bmg_device.h"
@@ -193,83 +195,142 @@ __global__ void k_digits(const double* __restrict__ r, uint32_t n_pad, int nvec,
   int v = blockIdx.y;
   if (i >= n_pad) return;
   long long R = llrint(r[v * (size_t)n_pad + i] / scale[v]);
+  // within each group of 16 individuals, individual 4j + s sits at byte
+  // 4s + j: the order in which the scan kernel splits a genotype word
+  const uint32_t l = i & 15u, pos = (i & ~15u) | (4u * (l & 3u) + (l >> 2));
   for (int d = 0; d < digits; ++d) {
     int dg = (int)(((R + 128) & 0xFF)) - 128;
-    dig[(size_t)(v * digits + d) * n_pad + i] = (int8_t)dg;
+    dig[(size_t)(v * digits + d) * n_pad + pos] = (int8_t)dg;
     R = (R - dg) >> 8;
   }
 }
 
-// grid.x: workgroups, each looping over blocks of 4 waves x NT tiles x 16
-// SNPs (so the digit tile it stages in LDS serves many blocks); grid.y:
-// individual tiles.
-template <int NT>
-__global__ __launch_bounds__(256) void k_scan_mfma(const uint32_t* __restrict__ g, uint64_t p,
-                                                   uint32_t W32, int coding,
-                                                   const int8_t* __restrict__ dig, int NC,
-                                                   uint32_t n_pad, uint32_t tile_words,
-                                                   int32_t* __restrict__ acc, int atomic) {
-  extern __shared__ int8_t sdig[];  // NC x (tile_words * 16)
+// Genotype values of a word (16 individuals) as 16 int8 in the split order:
+// out[s] byte j = value of individual 4j + s (field s of byte j), so each
+// output word is one shift and one mask of the value word.
+template <int CODING>
+__device__ __forceinline__ void word_split_i8x16(uint32_t w, uint32_t out[4]) {
+  const uint32_t lo = w & BMG_EVEN, hi = (w >> 1) & BMG_EVEN;
+  uint32_t v;
+  if (CODING == 0) v = hi + (hi & lo);     // 0, 1, 2 per field (no carry)
+  else if (CODING == 1) v = hi & ~lo;
+  else if (CODING == 2) v = hi;
+  else v = hi & lo;
+#pragma unroll
+  for (int b = 0; b < 4; ++b) out[b] = (v >> (2 * b)) & 0x03030303u;
+}
+
+// grid.x: workgroups, each looping over blocks of WAVES waves x NT tiles x
+// 16 SNPs (so the digit tile it stages in LDS serves many blocks); grid.y:
+// individual tiles.  Each LDS column holds the tile's digits rounded up to a
+// whole 16-word step with zeros, plus 16 bytes of padding so the 16 columns
+// a lane group reads fall in distinct banks.
+//
+// A wave walks its steps (block, 16-word slice) in order with two register
+// buffers of genotype words: while one step's MFMAs run, the loads of the
+// step after next are in flight in the other buffer.  Every load address is
+// clamped into the SNP's row and to the last SNP; what a clamped load brings
+// in meets a zero digit or lands in a row that is never stored.
+template <int NT, int CODING, int WAVES, bool ATOMIC>
+__global__ __launch_bounds__(64 * WAVES) void k_scan_mfma(const uint32_t* __restrict__ g, uint64_t p,
+                                                          uint32_t W32, const int8_t* __restrict__ dig,
+                                                          int NC, uint32_t n_pad, uint32_t tile_words,
+                                                          int32_t* __restrict__ acc) {
+  extern __shared__ int8_t sdig[];  // NC x (tile_words * 16 + 16)
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint32_t w0 = blockIdx.y * tile_words;
   const uint32_t w1 = min(W32, w0 + tile_words);
-  const uint32_t ind0 = w0 * 16, nind = (w1 - w0) * 16;
-  // stage the digit columns of this individual tile
-  for (uint32_t e = threadIdx.x * 16; e < (uint32_t)NC * nind; e += blockDim.x * 16) {
-    uint32_t col = e / nind, off = e % nind;
-    *(uint4*)(sdig + (size_t)col * nind + off) =
-        *(const uint4*)(dig + (size_t)col * n_pad + ind0 + off);
+  const uint32_t nind = (w1 - w0) * 16;
+  const uint32_t steps = (w1 - w0 + 15) / 16;
+  const uint32_t span = steps * 256;  // digit bytes the steps cover
+  const uint32_t ld = span + 16;
+  // stage the digit columns of this individual tile, zero past its end
+  for (uint32_t e = threadIdx.x * 16; e < (uint32_t)NC * span; e += 64 * WAVES * 16) {
+    uint32_t col = e / span, off = e % span;
+    uint4 v = make_uint4(0, 0, 0, 0);
+    if (off < nind) v = *(const uint4*)(dig + (size_t)col * n_pad + w0 * 16 + off);
+    *(uint4*)(sdig + (size_t)col * ld + off) = v;
   }
   __syncthreads();
   const int row = lane & 15, grp = lane >> 4;
   const int col = row;
-  const uint64_t per_blk = 4ull * NT * 16;
+  const int8_t* bcol = sdig + (size_t)min(col, NC - 1) * ld + grp * 64;
+  const uint64_t per_blk = (uint64_t)WAVES * NT * 16;
   const uint64_t n_blk = (p + per_blk - 1) / per_blk;
-  for (uint64_t blk = blockIdx.x; blk < n_blk; blk += gridDim.x) {
-    const uint64_t base = (blk * 4 + wave) * (NT * 16);
-    v4i c[NT];
-#pragma unroll
-    for (int t = 0; t < NT; ++t) c[t] = v4i{0, 0, 0, 0};
-    for (uint32_t wb = w0; wb < w1; wb += 16) {
-      uint4 a4[NT];
-#pragma unroll
-      for (int t = 0; t < NT; ++t) {
-        uint64_t snp = base + t * 16 + row;
-        uint32_t wi = wb + 4 * grp;
-        if (snp < p && wi < w1) a4[t] = *(const uint4*)(g + snp * W32 + wi);
-        else a4[t] = make_uint4(0, 0, 0, 0);
-      }
-#pragma unroll
-      for (int s = 0; s < 4; ++s) {
-        uint32_t wi = wb + 4 * grp + s;
-        v4i b = {0, 0, 0, 0};
-        if (col < NC && wi < w1) {
-          uint4 bb = *(const uint4*)(sdig + (size_t)col * nind + (wi - w0) * 16);
-          b = v4i{(int)bb.x, (int)bb.y, (int)bb.z, (int)bb.w};
-        }
-#pragma unroll
-        for (int t = 0; t < NT; ++t) {
-          uint32_t w = s == 0 ? a4[t].x : (s == 1 ? a4[t].y : (s == 2 ? a4[t].z : a4[t].w));
-          uint32_t o[4];
-          word_to_i8x16(w, coding, o);
-          v4i a = {(int)o[0], (int)o[1], (int)o[2], (int)o[3]};
-          c[t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, c[t], 0, 0, 0);
-        }
-      }
-    }
-    // lane holds C[4 * grp + r][col]
+  if ((uint64_t)blockIdx.x >= n_blk) return;
+  const uint64_t total = (n_blk - blockIdx.x + gridDim.x - 1) / gridDim.x * steps;
+
+  // load cursor (block lb, slice lw) and compute cursor (cb, cw)
+  uint64_t lb = blockIdx.x, cb = blockIdx.x;
+  uint32_t lw = 0, cw = 0;
+  auto load = [&](uint4 (&buf)[NT]) {
+    const uint32_t wi = min(w0 + lw * 16 + 4 * grp, W32 - 4);
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
+      uint64_t snp = min((lb * WAVES + wave) * (NT * 16) + t * 16 + row, p - 1);
+      buf[t] = *(const uint4*)(g + snp * W32 + wi);
+    }
+    if (++lw == steps) { lw = 0; lb += gridDim.x; }
+  };
+  v4i c[NT];
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        uint64_t snp = base + t * 16 + 4 * grp + r;
-        if (snp < p && col < NC) {
-          if (atomic) atomicAdd(&acc[snp * 16 + col], c[t][r]);
-          else acc[snp * 16 + col] = c[t][r];
-        }
+  for (int t = 0; t < NT; ++t) c[t] = v4i{0, 0, 0, 0};
+  auto compute = [&](const uint4 (&a4)[NT]) {
+    v4i b[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      // columns past NC read column NC - 1; their results are never stored
+      uint4 bb = *(const uint4*)(bcol + (cw * 16 + s) * 16);
+      b[s] = v4i{(int)bb.x, (int)bb.y, (int)bb.z, (int)bb.w};
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        uint32_t w = s == 0 ? a4[t].x : (s == 1 ? a4[t].y : (s == 2 ? a4[t].z : a4[t].w));
+        uint32_t o[4];
+        word_split_i8x16<CODING>(w, o);
+        v4i a = {(int)o[0], (int)o[1], (int)o[2], (int)o[3]};
+        c[t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b[s], c[t], 0, 0, 0);
       }
     }
+    if (++cw == steps) {
+      // lane holds C[4 * grp + r][col] of this block
+      const uint64_t base = (cb * WAVES + wave) * (NT * 16);
+      // rows past p land in the accumulator's padding, columns past NC in
+      // slots the combine never reads: the stores need no guard
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          int32_t* dst = acc + (base + t * 16 + 4 * grp + r) * 16 + col;
+          if (ATOMIC) atomicAdd(dst, c[t][r]);
+          else *dst = c[t][r];
+        }
+        c[t] = v4i{0, 0, 0, 0};
+      }
+      cw = 0;
+      cb += gridDim.x;
+    }
+  };
+  uint4 bufA[NT], bufB[NT];
+  load(bufA);
+  load(bufB);
+  // every trip of the loop issues both buffers' loads, so the wait for a
+  // buffer never has to cover the other's
+  // (the scheduling barriers keep each buffer's new loads after the last
+  // read of its old words, so the buffers keep their registers round the
+  // loop instead of being copied -- a copy would wait for the loads)
+  uint64_t k = 0;
+  for (; k + 1 < total; k += 2) {
+    compute(bufA);
+    __builtin_amdgcn_sched_barrier(0);
+    load(bufA);
+    compute(bufB);
+    __builtin_amdgcn_sched_barrier(0);
+    load(bufB);
   }
+  if (k < total) compute(bufA);
 }
 
 __global__ void k_fill_i32(int32_t* __restrict__ p, uint64_t n, int32_t v) {
@@ -450,7 +511,9 @@ static int scan_prepare(bmg_ctx* c, const double* r_dev, int nvec, int digits, h
   }
   int NC = nvec * digits;
   if (ensure((void**)&c->dig, &c->dig_bytes, (size_t)NC * c->n_pad)) return BMG_E_HIP;
-  if (ensure((void**)&c->acc, &c->acc_bytes, (size_t)c->p * 16 * sizeof(int32_t))) return BMG_E_HIP;
+  // p rows plus one block of the largest scan shape (16 waves x 4 tiles x 16
+  // SNPs) for the unguarded stores of the last block
+  if (ensure((void**)&c->acc, &c->acc_bytes, (size_t)(c->p + 1024) * 16 * sizeof(int32_t))) return BMG_E_HIP;
   if (!c->scale) BMG_HIP(hipMalloc(&c->scale, sizeof(double) * 16));
   hipLaunchKernelGGL(k_absmax, dim3(nvec), dim3(256), 0, s, r_dev, c->n_pad, nvec, c->scale, digits);
   hipLaunchKernelGGL(k_digits, dim3((c->n_pad + 255) / 256, nvec), dim3(256), 0, s, r_dev,
@@ -460,7 +523,7 @@ static int scan_prepare(bmg_ctx* c, const double* r_dev, int nvec, int digits, h
 }
 
 struct ScanShape {
-  int nt;
+  int nt, waves;
   uint32_t tile_words, n_tiles;
   unsigned gx;
   int atomic;
@@ -469,22 +532,23 @@ struct ScanShape {
 
 // a shape: NT SNP tiles per wave, individual tiles of tile_words words,
 // gx workgroups looping over the SNP blocks
-static ScanShape make_shape(const bmg_ctx* c, int NC, int nt, uint32_t tile_words, unsigned gx) {
+static ScanShape make_shape(const bmg_ctx* c, int NC, int nt, int waves, uint32_t tile_words, unsigned gx) {
   ScanShape sh;
   sh.nt = nt;
+  sh.waves = waves;
   sh.tile_words = std::min<uint32_t>(tile_words, (c->W32 + 15) / 16 * 16);
   sh.n_tiles = (c->W32 + sh.tile_words - 1) / sh.tile_words;
-  const uint64_t n_blk = (c->p + 4ull * nt * 16 - 1) / (4ull * nt * 16);
+  const uint64_t per_blk = (uint64_t)waves * nt * 16;
+  const uint64_t n_blk = (c->p + per_blk - 1) / per_blk;
   sh.gx = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(n_blk, gx));
   sh.atomic = sh.n_tiles > 1;
-  sh.lds = (size_t)NC * sh.tile_words * 16;
+  sh.lds = (size_t)NC * (sh.tile_words * 16 + 16);
   return sh;
 }
 
-// digit tile widths that fit 64 KiB (two workgroups per CU) or all 160 KiB
-// of a CU's LDS (one)
+// widest digit tile (a multiple of 16 words) whose padded columns fit `bytes`
 static uint32_t tile_for(int NC, size_t bytes) {
-  return (uint32_t)std::max<size_t>(16, (bytes / NC) / 16 / 16 * 16);
+  return (uint32_t)std::max<size_t>(16, (bytes / NC - 16) / 16 / 16 * 16);
 }
 
 // zero the accumulator when several individual tiles add into it (a fill
@@ -494,13 +558,42 @@ static void scan_clear(bmg_ctx* c, const ScanShape& sh, hipStream_t s) {
     hipLaunchKernelGGL(k_fill_i32, dim3(2048), dim3(256), 0, s, c->acc, (uint64_t)c->p * 16, 0);
 }
 
+template <int NT, int CODING, bool ATOMIC>
+static void scan_mfma_w(bmg_ctx* c, int NC, const ScanShape& sh, hipStream_t s) {
+  dim3 grid(sh.gx, sh.n_tiles);
+  switch (sh.waves) {
+    case 16:
+      hipLaunchKernelGGL((k_scan_mfma<NT, CODING, 16, ATOMIC>), grid, dim3(1024), sh.lds, s, c->g, c->p, c->W32,
+                         c->dig, NC, c->n_pad, sh.tile_words, c->acc);
+      break;
+    case 8:
+      hipLaunchKernelGGL((k_scan_mfma<NT, CODING, 8, ATOMIC>), grid, dim3(512), sh.lds, s, c->g, c->p, c->W32,
+                         c->dig, NC, c->n_pad, sh.tile_words, c->acc);
+      break;
+    default:
+      hipLaunchKernelGGL((k_scan_mfma<NT, CODING, 4, ATOMIC>), grid, dim3(256), sh.lds, s, c->g, c->p, c->W32,
+                         c->dig, NC, c->n_pad, sh.tile_words, c->acc);
+  }
+}
+
+template <int CODING>
+static void scan_mfma_c(bmg_ctx* c, int NC, const ScanShape& sh, hipStream_t s) {
+  if (sh.atomic) {
+    if (sh.nt == 4) scan_mfma_w<4, CODING, true>(c, NC, sh, s);
+    else scan_mfma_w<2, CODING, true>(c, NC, sh, s);
+  } else {
+    if (sh.nt == 4) scan_mfma_w<4, CODING, false>(c, NC, sh, s);
+    else scan_mfma_w<2, CODING, false>(c, NC, sh, s);
+  }
+}
+
 static void scan_mfma(bmg_ctx* c, int NC, int coding, const ScanShape& sh, hipStream_t s) {
-  if (sh.nt == 4)
-    hipLaunchKernelGGL(k_scan_mfma<4>, dim3(sh.gx, sh.n_tiles), dim3(256), sh.lds, s, c->g, c->p, c->W32,
-                       coding, c->dig, NC, c->n_pad, sh.tile_words, c->acc, sh.atomic);
-  else
-    hipLaunchKernelGGL(k_scan_mfma<2>, dim3(sh.gx, sh.n_tiles), dim3(256), sh.lds, s, c->g, c->p, c->W32,
-                       coding, c->dig, NC, c->n_pad, sh.tile_words, c->acc, sh.atomic);
+  switch (coding) {
+    case 0: scan_mfma_c<0>(c, NC, sh, s); break;
+    case 1: scan_mfma_c<1>(c, NC, sh, s); break;
+    case 2: scan_mfma_c<2>(c, NC, sh, s); break;
+    default: scan_mfma_c<3>(c, NC, sh, s); break;
+  }
 }
 
 static int scan_kernel(bmg_ctx* c, int nvec, int coding, int digits, const ScanShape& sh, hipStream_t s) {
@@ -518,18 +611,23 @@ static int scan_kernel(bmg_ctx* c, int nvec, int coding, int digits, const ScanS
 static int scan_tuned_shape(bmg_ctx* c, int nvec, int coding, int digits, hipStream_t s, ScanShape* out) {
   const int NC = nvec * digits;
   if (c->scan_tw[NC]) {
-    *out = make_shape(c, NC, c->scan_nt[NC], c->scan_tw[NC], c->scan_gx[NC]);
+    *out = make_shape(c, NC, c->scan_nt[NC], c->scan_wv[NC], c->scan_tw[NC], c->scan_gx[NC]);
     return BMG_OK;
   }
+  // 256 CUs: grids of 1 to 8 workgroups per CU, over the individual tiles
   std::vector<ScanShape> cand;
   const uint32_t tws[2] = {tile_for(NC, 65536), tile_for(NC, 163840)};
-  for (int nt : {2, 4})
-    for (int ti = 0; ti < 2; ++ti) {
-      ScanShape probe = make_shape(c, NC, nt, tws[ti], 1);
-      if (ti == 1 && probe.tile_words == make_shape(c, NC, nt, tws[0], 1).tile_words) continue;
-      for (unsigned per_cu : {2u, 4u, 8u})
-        cand.push_back(make_shape(c, NC, nt, tws[ti], std::max(1u, 256u * per_cu / probe.n_tiles)));
-    }
+  for (int waves : {4, 8, 16})
+    for (int nt : {2, 4})
+      for (int ti = 0; ti < 2; ++ti) {
+        ScanShape probe = make_shape(c, NC, nt, waves, tws[ti], 1);
+        if (ti == 1 && probe.tile_words == make_shape(c, NC, nt, waves, tws[0], 1).tile_words) continue;
+        for (unsigned per_cu : {1u, 2u, 4u, 8u})
+          cand.push_back(make_shape(c, NC, nt, waves, tws[ti], std::max(1u, 256u * per_cu / probe.n_tiles)));
+      }
+  // BMG_SCAN_TUNE_LOG=1 prints every candidate's time to stderr
+  const char* log_env = getenv("BMG_SCAN_TUNE_LOG");
+  const bool log = log_env && log_env[0] == '1';
   hipEvent_t e0, e1;
   BMG_HIP(hipEventCreate(&e0));
   BMG_HIP(hipEventCreate(&e1));
@@ -539,22 +637,28 @@ static int scan_tuned_shape(bmg_ctx* c, int nvec, int coding, int digits, hipStr
     float t_min = 1e30f;
     bool ok = true;
     for (int r = 0; r < 2 && ok; ++r) {
-      hipEventRecord(e0, s);
+      (void)hipEventRecord(e0, s);
       scan_clear(c, cand[i], s);
       scan_mfma(c, NC, coding, cand[i], s);
       if (hipGetLastError() != hipSuccess) { ok = false; break; }
-      hipEventRecord(e1, s);
+      (void)hipEventRecord(e1, s);
       if (hipEventSynchronize(e1) != hipSuccess) { ok = false; break; }
       float t = 0;
-      hipEventElapsedTime(&t, e0, e1);
+      (void)hipEventElapsedTime(&t, e0, e1);
       t_min = std::min(t_min, t);
     }
     if (ok && t_min < best) { best = t_min; picked = (int)i; }
+    if (log)
+      fprintf(stderr, "scan tune NC=%d waves=%d nt=%d tile_words=%u tiles=%u gx=%u: %s %.4f ms\n", NC,
+              cand[i].waves, cand[i].nt, cand[i].tile_words, cand[i].n_tiles, cand[i].gx, ok ? "ok" : "failed",
+              ok ? t_min : 0.0f);
   }
-  hipEventDestroy(e0); hipEventDestroy(e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   if (picked < 0) { set_error("scan: no candidate shape could be launched"); return BMG_E_HIP; }
   const ScanShape& pick = cand[picked];
-  c->scan_nt[NC] = pick.nt; c->scan_tw[NC] = pick.tile_words; c->scan_gx[NC] = pick.gx;
+  c->scan_nt[NC] = pick.nt; c->scan_wv[NC] = pick.waves;
+  c->scan_tw[NC] = pick.tile_words; c->scan_gx[NC] = pick.gx;
   *out = pick;
   return BMG_OK;
 }

@@ -131,16 +131,17 @@ int bmg_scan_xtr_timed(bmg_ctx* ctx, const double* r_dev, int nvec, int coding, 
                        double* out_dev, int reps, double* ms_per_launch, void* stream);
 /* The scan shape the autotuner picked for nvec * digits digit columns (no
  * counterpart in the reference): out[0] SNP tiles per wave, out[1] LDS tile
- * width in 32-bit words, out[2] individual tiles, out[3] workgroups.  The
- * first scan of each width on a context times every candidate shape and
- * keeps the fastest; all zeros before that. */
+ * width in 32-bit words, out[2] individual tiles, out[3] workgroups, out[4]
+ * waves per workgroup.  The first scan of each width on a context times
+ * every candidate shape and keeps the fastest; all zeros before that. */
 int bmg_scan_shape(bmg_ctx* ctx, int nvec, int digits, int32_t* out);
 /* Pin the scan shape for nvec * digits columns instead of tuning it:
- * tiles_per_wave 2 or 4, tile_words a multiple of 16 whose digit tile fits
- * the LDS (tile_words * 16 * nvec * digits <= 160 KiB), workgroups >= 1
+ * tiles_per_wave 2 or 4, waves 4, 8 or 16, tile_words a multiple of 16
+ * whose padded digit tile fits the LDS ((min(tile_words, words per SNP
+ * rounded up to 16) * 16 + 16) * nvec * digits <= 160 KiB), workgroups >= 1
  * (clamped to the number of SNP blocks). */
-int bmg_scan_set_shape(bmg_ctx* ctx, int nvec, int digits, int tiles_per_wave, uint32_t tile_words,
-                       uint32_t workgroups);
+int bmg_scan_set_shape(bmg_ctx* ctx, int nvec, int digits, int tiles_per_wave, int waves,
+                       uint32_t tile_words, uint32_t workgroups);
 /* Timing form of bmg_band_xtx_dev: `reps` launches, each bracketed by HIP
  * events on `stream`; *ms_per_launch is the mean kernel time. */
 int bmg_band_xtx_timed(bmg_ctx* ctx, uint32_t W, int coding_a, int coding_b, int32_t* out_dev,

@@ -137,19 +137,39 @@ def test_scan_shapes_all_exact(bmg):
     assert np.array_equal(tuned, ref)
     shape = ctx.scan_shape(3, 5)
     assert shape["tiles_per_wave"] in (2, 4) and shape["workgroups"] >= 1
+    assert shape["waves"] in (4, 8, 16)
     w32 = ((n + 15) // 16 + 3) // 4 * 4
-    for nt in (2, 4):
-        for tw in (16, 272, 672):
-            for gx in (1, 5, 4096):
-                ctx.scan_set_shape(3, 5, nt, tw, gx)
-                got = ctx.scan_xtr(R, coding=0, digits=5)
-                assert np.array_equal(got, ref), (nt, tw, gx)
-                sh = ctx.scan_shape(3, 5)
-                assert sh["n_tiles"] == -(-w32 // tw)
+    for waves in (4, 8, 16):
+        for nt in (2, 4):
+            for tw in (16, 272, 672):
+                for gx in (1, 5, 4096):
+                    ctx.scan_set_shape(3, 5, nt, tw, gx, waves)
+                    got = ctx.scan_xtr(R, coding=0, digits=5)
+                    assert np.array_equal(got, ref), (waves, nt, tw, gx)
+                    sh = ctx.scan_shape(3, 5)
+                    assert sh["n_tiles"] == -(-w32 // tw)
     with pytest.raises(bmg.BmgError):
         ctx.scan_set_shape(3, 5, 3, 272, 8)
     with pytest.raises(bmg.BmgError):
         ctx.scan_set_shape(3, 5, 2, 688, 8)
+    with pytest.raises(bmg.BmgError):
+        ctx.scan_set_shape(3, 5, 2, 272, 8, waves=6)
+
+
+@pytest.mark.parametrize("coding", [0, 1, 2, 3])
+def test_scan_codings_exact(bmg, coding):
+    """The scan decodes each genotype coding (additive, heterozygous,
+    dominant, recessive) to the values of the exact decode (itself
+    bit-exact against the oracle above), missing genotypes as 0."""
+    rng = np.random.default_rng(5 + coding)
+    n, p = 3001, 700
+    G, body = random_body(rng, n, p, with_missing=True)
+    ctx = bmg.Ctx(n, p, body, rng.normal(size=n))
+    X = np.concatenate([ctx.decode(np.arange(j, min(p, j + 100)), coding) for j in range(0, p, 100)])
+    X[X < 0] = 0.0  # missing genotypes enter the scan as 0
+    r = rng.integers(-500, 500, size=(2, n)).astype(np.float64) / 64.0
+    got = ctx.scan_xtr(r, coding=coding, digits=4)
+    assert np.array_equal(got, r @ X.T)
 
 
 @pytest.mark.parametrize("n,p,ld", [(37, 50, None), (203, 300, (0.9, 0.9, 10, 40)),
