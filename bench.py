@@ -160,6 +160,15 @@ def copy_calibration(bmg, nbytes=1 << 30):
     return dict(bytes=2 * nbytes, ms=ms, gbs=2 * nbytes / (ms * 1e-3) / 1e9)
 
 
+def read_calibration(bmg, nbytes=1 << 31):
+    """Streaming read of 2 GiB: the read rate the device reaches."""
+    a, sink = bmg.DevBuf(nbytes), bmg.DevBuf(256)
+    ms = bmg.read_timed(a, nbytes, sink, reps=5)
+    a.free()
+    sink.free()
+    return dict(bytes=nbytes, ms=ms, gbs=nbytes / (ms * 1e-3) / 1e9)
+
+
 def cpu_baseline(ctx, y, monitored, seconds):
     import oracle
     oracle.load()
@@ -234,7 +243,8 @@ def main():
         sc = scan_roofline(bmg, ctx)
         bd = band_mfma(bmg, ctx)
         cp = copy_calibration(bmg)
-        print(json.dumps(dict(scan=sc, band=bd, copy=cp)), flush=True)
+        rd = read_calibration(bmg)
+        print(json.dumps(dict(scan=sc, band=bd, copy=cp, read=rd)), flush=True)
         return
 
     ips = args.iters_per_step
@@ -298,6 +308,9 @@ def main():
         cp = copy_calibration(bmg)
         extra["copy_calibration"] = dict(bytes=cp["bytes"], ms=round(cp["ms"], 4),
                                          gbs=round(cp["gbs"], 1))
+        rd = read_calibration(bmg)
+        extra["read_calibration"] = dict(bytes=rd["bytes"], ms=round(rd["ms"], 4), gbs=round(rd["gbs"], 1),
+                                         scan_frac_of_read=round(sc["gbs"] / rd["gbs"], 4))
 
     cpu = None
     if not args.no_cpu:

@@ -5,8 +5,8 @@ include/bmagwa_hip.h).  This package binds it with ctypes; importing it does
 not need a GPU, but every call that computes does, and there is no CPU
 fallback: a missing library raises at load time.
 """
-from ._lib import (BmgError, Ctx, Batch, Model, Mlp, DevBuf, build, copy_timed, lib, opts_from_cfg,
-                   lib_path, BATCH_CFG_DEFAULT)
+from ._lib import (BmgError, Ctx, Batch, Model, Mlp, DevBuf, build, copy_timed, read_timed, lib,
+                   opts_from_cfg, lib_path, BATCH_CFG_DEFAULT)
 
-__all__ = ["BmgError", "Ctx", "Batch", "Model", "Mlp", "DevBuf", "build", "copy_timed", "lib",
-           "opts_from_cfg", "lib_path", "BATCH_CFG_DEFAULT"]
+__all__ = ["BmgError", "Ctx", "Batch", "Model", "Mlp", "DevBuf", "build", "copy_timed", "read_timed",
+           "lib", "opts_from_cfg", "lib_path", "BATCH_CFG_DEFAULT"]

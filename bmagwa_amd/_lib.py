@@ -134,6 +134,7 @@ SIGNATURES = {
     "bmg_dev_free": (C.c_int, C.c_int, _P),
     "bmg_memcpy_h2d": (C.c_int, C.c_int, _P, _P, C.c_uint64),
     "bmg_copy_timed": (C.c_int, C.c_int, _P, _P, C.c_uint64, C.c_int, _D),
+    "bmg_read_timed": (C.c_int, C.c_int, _P, C.c_uint64, _P, C.c_int, _D),
     "bmg_mlp_nparams": (C.c_int,),
     "bmg_mlp_create": (C.c_int, C.c_int, _PP),
     "bmg_mlp_destroy": (C.c_int, _P),
@@ -405,6 +406,15 @@ def copy_timed(src, dst, nbytes, reps=10, device=0):
     """Mean duration (ms) of a plain device copy of nbytes."""
     ms = C.c_double()
     check(lib().bmg_copy_timed(device, C.c_void_p(src.ptr), C.c_void_p(dst.ptr), nbytes, reps,
+                               C.byref(ms)))
+    return ms.value
+
+
+def read_timed(src, nbytes, sink, reps=5, device=0):
+    """Best mean duration (ms) of a streaming read of nbytes over a few
+    grid sizes; sink: a device buffer of at least 4 bytes."""
+    ms = C.c_double()
+    check(lib().bmg_read_timed(device, C.c_void_p(src.ptr), nbytes, C.c_void_p(sink.ptr), reps,
                                C.byref(ms)))
     return ms.value
 

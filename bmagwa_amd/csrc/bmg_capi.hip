@@ -68,7 +68,9 @@ static int ctx_alloc(int device, uint32_t n, uint64_t p, bmg_ctx** out, bmg_ctx*
   bmg_ctx* c = new bmg_ctx();
   c->device = device;
   c->n = n; c->p = p;
-  c->W32 = ((n + 15) / 16 + 3) / 4 * 4;
+  // rows of whole 128-byte lines: a scan step reads each SNP's line in one
+  // piece instead of straddling two
+  c->W32 = ((n + 15) / 16 + 31) / 32 * 32;
   c->n_pad = c->W32 * 16;
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e != hipSuccess) { delete c; return hip_fail(e, "hipStreamCreate"); }
@@ -280,8 +282,8 @@ extern "C" int bmg_scan_shape(bmg_ctx* c, int nvec, int digits, int32_t* out) {
 extern "C" int bmg_scan_set_shape(bmg_ctx* c, int nvec, int digits, int nt, int waves, uint32_t tile_words,
                                   uint32_t workgroups) {
   if (!c || nvec < 1 || digits < 1 || nvec * digits > 16 || (nt != 2 && nt != 4) ||
-      (waves != 4 && waves != 8 && waves != 16) || tile_words < 16 || tile_words % 16 || workgroups < 1 ||
-      ((size_t)std::min<uint32_t>(tile_words, (c->W32 + 15) / 16 * 16) * 16 + 16) * nvec * digits > 163840) {
+      (waves != 4 && waves != 8 && waves != 16) || tile_words < 32 || tile_words % 32 || workgroups < 1 ||
+      ((size_t)std::min<uint32_t>(tile_words, (c->W32 + 31) / 32 * 32) * 16 + 16) * nvec * digits > 163840) {
     set_error("bad scan shape"); return BMG_E_ARG;
   }
   const int NC = nvec * digits;
@@ -330,6 +332,18 @@ extern "C" int bmg_copy_timed(int device, const void* src_dev, void* dst_dev, ui
   hipStream_t s = nullptr;
   BMG_HIP(hipStreamCreate(&s));
   int rc = launch_copy_timed(src_dev, dst_dev, bytes, reps, ms_per_launch, s);
+  hipStreamSynchronize(s);
+  hipStreamDestroy(s);
+  return rc;
+}
+
+extern "C" int bmg_read_timed(int device, const void* src_dev, uint64_t bytes, void* sink_dev, int reps,
+                              double* ms_per_launch) {
+  if (!src_dev || !sink_dev || !ms_per_launch || reps < 1) { set_error("bad argument"); return BMG_E_ARG; }
+  BMG_HIP(hipSetDevice(device));
+  hipStream_t s = nullptr;
+  BMG_HIP(hipStreamCreate(&s));
+  int rc = launch_read_timed(src_dev, bytes, sink_dev, reps, ms_per_launch, s);
   hipStreamSynchronize(s);
   hipStreamDestroy(s);
   return rc;

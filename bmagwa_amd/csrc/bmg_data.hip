@@ -222,27 +222,30 @@ __device__ __forceinline__ void word_split_i8x16(uint32_t w, uint32_t out[4]) {
 
 // grid.x: workgroups, each looping over blocks of WAVES waves x NT tiles x
 // 16 SNPs (so the digit tile it stages in LDS serves many blocks); grid.y:
-// individual tiles.  Each LDS column holds the tile's digits rounded up to a
-// whole 16-word step with zeros, plus 16 bytes of padding so the 16 columns
-// a lane group reads fall in distinct banks.
+// individual tiles.  A step covers 32 words (512 individuals) of 16 SNPs:
+// each SNP row's 128-byte line is read whole within the step, the first
+// half by one load of the lane groups and the second by the next.  Each LDS
+// column holds the tile's digits rounded up to whole steps with zeros, plus
+// 16 bytes of padding so the 16 columns a lane group reads fall in distinct
+// banks.
 //
-// A wave walks its steps (block, 16-word slice) in order with two register
-// buffers of genotype words: while one step's MFMAs run, the loads of the
-// step after next are in flight in the other buffer.  Every load address is
-// clamped into the SNP's row and to the last SNP; what a clamped load brings
-// in meets a zero digit or lands in a row that is never stored.
+// A wave walks its steps (block, slice) in order with two register buffers
+// of genotype words: while one step's MFMAs run, the loads of the step
+// after next are in flight in the other buffer.  Every load address is
+// clamped into the SNP's row and to the last SNP; what a clamped load
+// brings in meets a zero digit or lands in a row that is never stored.
 template <int NT, int CODING, int WAVES, bool ATOMIC>
 __global__ __launch_bounds__(64 * WAVES) void k_scan_mfma(const uint32_t* __restrict__ g, uint64_t p,
                                                           uint32_t W32, const int8_t* __restrict__ dig,
                                                           int NC, uint32_t n_pad, uint32_t tile_words,
                                                           int32_t* __restrict__ acc) {
-  extern __shared__ int8_t sdig[];  // NC x (tile_words * 16 + 16)
+  extern __shared__ int8_t sdig[];  // NC x (tile_words * 16 + 16), tile_words a multiple of 32
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint32_t w0 = blockIdx.y * tile_words;
   const uint32_t w1 = min(W32, w0 + tile_words);
   const uint32_t nind = (w1 - w0) * 16;
-  const uint32_t steps = (w1 - w0 + 15) / 16;
-  const uint32_t span = steps * 256;  // digit bytes the steps cover
+  const uint32_t steps = (w1 - w0 + 31) / 32;
+  const uint32_t span = steps * 512;  // digit bytes the steps cover
   const uint32_t ld = span + 16;
   // stage the digit columns of this individual tile, zero past its end
   for (uint32_t e = threadIdx.x * 16; e < (uint32_t)NC * span; e += 64 * WAVES * 16) {
@@ -260,45 +263,52 @@ __global__ __launch_bounds__(64 * WAVES) void k_scan_mfma(const uint32_t* __rest
   if ((uint64_t)blockIdx.x >= n_blk) return;
   const uint64_t total = (n_blk - blockIdx.x + gridDim.x - 1) / gridDim.x * steps;
 
-  // load cursor (block lb, slice lw) and compute cursor (cb, cw)
+  // load cursor (block lb, step lw) and compute cursor (cb, cw)
   uint64_t lb = blockIdx.x, cb = blockIdx.x;
   uint32_t lw = 0, cw = 0;
-  auto load = [&](uint4 (&buf)[NT]) {
-    const uint32_t wi = min(w0 + lw * 16 + 4 * grp, W32 - 4);
+  auto load = [&](uint4 (&buf)[NT][2]) {
+    const uint32_t wa = min(w0 + lw * 32 + 4 * grp, W32 - 4);
+    const uint32_t wb = min(w0 + lw * 32 + 16 + 4 * grp, W32 - 4);
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
       uint64_t snp = min((lb * WAVES + wave) * (NT * 16) + t * 16 + row, p - 1);
-      buf[t] = *(const uint4*)(g + snp * W32 + wi);
+      const uint32_t* r = g + snp * W32;
+      buf[t][0] = *(const uint4*)(r + wa);
+      buf[t][1] = *(const uint4*)(r + wb);
     }
     if (++lw == steps) { lw = 0; lb += gridDim.x; }
   };
   v4i c[NT];
 #pragma unroll
   for (int t = 0; t < NT; ++t) c[t] = v4i{0, 0, 0, 0};
-  auto compute = [&](const uint4 (&a4)[NT]) {
-    v4i b[4];
+  auto compute = [&](const uint4 (&a4)[NT][2]) {
 #pragma unroll
-    for (int s = 0; s < 4; ++s) {
-      // columns past NC read column NC - 1; their results are never stored
-      uint4 bb = *(const uint4*)(bcol + (cw * 16 + s) * 16);
-      b[s] = v4i{(int)bb.x, (int)bb.y, (int)bb.z, (int)bb.w};
-    }
+    for (int h = 0; h < 2; ++h) {
+      v4i b[4];
 #pragma unroll
-    for (int s = 0; s < 4; ++s) {
+      for (int s = 0; s < 4; ++s) {
+        // columns past NC read column NC - 1; their results are never stored
+        uint4 bb = *(const uint4*)(bcol + (cw * 32 + h * 16 + s) * 16);
+        b[s] = v4i{(int)bb.x, (int)bb.y, (int)bb.z, (int)bb.w};
+      }
 #pragma unroll
-      for (int t = 0; t < NT; ++t) {
-        uint32_t w = s == 0 ? a4[t].x : (s == 1 ? a4[t].y : (s == 2 ? a4[t].z : a4[t].w));
-        uint32_t o[4];
-        word_split_i8x16<CODING>(w, o);
-        v4i a = {(int)o[0], (int)o[1], (int)o[2], (int)o[3]};
-        c[t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b[s], c[t], 0, 0, 0);
+      for (int s = 0; s < 4; ++s) {
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+          const uint4& q = a4[t][h];
+          uint32_t w = s == 0 ? q.x : (s == 1 ? q.y : (s == 2 ? q.z : q.w));
+          uint32_t o[4];
+          word_split_i8x16<CODING>(w, o);
+          v4i a = {(int)o[0], (int)o[1], (int)o[2], (int)o[3]};
+          c[t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b[s], c[t], 0, 0, 0);
+        }
       }
     }
     if (++cw == steps) {
-      // lane holds C[4 * grp + r][col] of this block
+      // lane holds C[4 * grp + r][col] of this block; rows past p land in
+      // the accumulator's padding, columns past NC in slots the combine
+      // never reads: the stores need no guard
       const uint64_t base = (cb * WAVES + wave) * (NT * 16);
-      // rows past p land in the accumulator's padding, columns past NC in
-      // slots the combine never reads: the stores need no guard
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
 #pragma unroll
@@ -313,14 +323,13 @@ __global__ __launch_bounds__(64 * WAVES) void k_scan_mfma(const uint32_t* __rest
       cb += gridDim.x;
     }
   };
-  uint4 bufA[NT], bufB[NT];
+  uint4 bufA[NT][2], bufB[NT][2];
   load(bufA);
   load(bufB);
   // every trip of the loop issues both buffers' loads, so the wait for a
-  // buffer never has to cover the other's
-  // (the scheduling barriers keep each buffer's new loads after the last
-  // read of its old words, so the buffers keep their registers round the
-  // loop instead of being copied -- a copy would wait for the loads)
+  // buffer never has to cover the other's (the scheduling barriers keep
+  // each buffer's new loads after the last read of its old words, so the
+  // buffers keep their registers round the loop instead of being copied)
   uint64_t k = 0;
   for (; k + 1 < total; k += 2) {
     compute(bufA);
@@ -536,7 +545,7 @@ static ScanShape make_shape(const bmg_ctx* c, int NC, int nt, int waves, uint32_
   ScanShape sh;
   sh.nt = nt;
   sh.waves = waves;
-  sh.tile_words = std::min<uint32_t>(tile_words, (c->W32 + 15) / 16 * 16);
+  sh.tile_words = std::min<uint32_t>(tile_words, (c->W32 + 31) / 32 * 32);
   sh.n_tiles = (c->W32 + sh.tile_words - 1) / sh.tile_words;
   const uint64_t per_blk = (uint64_t)waves * nt * 16;
   const uint64_t n_blk = (c->p + per_blk - 1) / per_blk;
@@ -546,9 +555,9 @@ static ScanShape make_shape(const bmg_ctx* c, int NC, int nt, int waves, uint32_
   return sh;
 }
 
-// widest digit tile (a multiple of 16 words) whose padded columns fit `bytes`
+// widest digit tile (a multiple of 32 words) whose padded columns fit `bytes`
 static uint32_t tile_for(int NC, size_t bytes) {
-  return (uint32_t)std::max<size_t>(16, (bytes / NC - 16) / 16 / 16 * 16);
+  return (uint32_t)std::max<size_t>(32, (bytes / NC - 16) / 16 / 32 * 32);
 }
 
 // zero the accumulator when several individual tiles add into it (a fill
@@ -739,6 +748,53 @@ int launch_copy_timed(const void* src, void* dst, uint64_t bytes, int reps, doub
   }
   hipEventDestroy(e0); hipEventDestroy(e1);
   *ms = total / (reps > 0 ? reps : 1);
+  return BMG_OK;
+}
+
+// Streaming read of n16 16-byte words, four loads in flight per thread; the
+// XOR of what a thread read is stored only if it equals a value the host
+// never plants, so the loads cannot be dropped and nothing is written.
+__global__ void k_read16(const uint4* __restrict__ src, uint64_t n16, uint32_t* __restrict__ sink) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  uint32_t x = 0;
+  for (; i + 3 * stride < n16; i += 4 * stride) {
+    uint4 a = src[i], b = src[i + stride], c = src[i + 2 * stride], d = src[i + 3 * stride];
+    x ^= a.x ^ a.y ^ a.z ^ a.w ^ b.x ^ b.y ^ b.z ^ b.w ^ c.x ^ c.y ^ c.z ^ c.w ^ d.x ^ d.y ^ d.z ^ d.w;
+  }
+  for (; i < n16; i += stride) {
+    uint4 a = src[i];
+    x ^= a.x ^ a.y ^ a.z ^ a.w;
+  }
+  if (x == 0x9E3779B9u) sink[0] = x;
+}
+
+// Best mean time over a few grid sizes: the streaming-read rate the device
+// reaches, the ceiling for the read-bound scan.
+int launch_read_timed(const void* src, uint64_t bytes, void* sink, int reps, double* ms, hipStream_t s) {
+  if (bytes % 16 || (uintptr_t)src % 16) { set_error("read: 16-byte aligned size and pointer only"); return BMG_E_ARG; }
+  const uint64_t n16 = bytes / 16;
+  hipEvent_t e0, e1;
+  BMG_HIP(hipEventCreate(&e0));
+  BMG_HIP(hipEventCreate(&e1));
+  double best = 1e30;
+  for (unsigned grid : {1024u, 2048u, 4096u, 8192u}) {
+    double total = 0.0;
+    for (int r = 0; r <= reps; ++r) {  // the first launch is a warm-up
+      BMG_HIP(hipEventRecord(e0, s));
+      hipLaunchKernelGGL(k_read16, dim3(grid), dim3(256), 0, s, (const uint4*)src, n16, (uint32_t*)sink);
+      BMG_HIP(hipGetLastError());
+      BMG_HIP(hipEventRecord(e1, s));
+      BMG_HIP(hipEventSynchronize(e1));
+      float t = 0;
+      BMG_HIP(hipEventElapsedTime(&t, e0, e1));
+      if (r > 0) total += t;
+    }
+    best = std::min(best, total / (reps > 0 ? reps : 1));
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *ms = best;
   return BMG_OK;
 }
 

@@ -12,7 +12,7 @@ struct bmg_ctx {
   int device = 0;
   uint32_t n = 0;
   uint64_t p = 0;
-  uint32_t W32 = 0;      // 32-bit words per SNP (multiple of 4)
+  uint32_t W32 = 0;      // 32-bit words per SNP (multiple of 32: 128-byte rows)
   uint32_t n_pad = 0;    // W32 * 16 individuals
   uint32_t* g = nullptr; // p * W32 packed words, SNP-major
   double* y = nullptr;   // n_pad, zero padded
@@ -58,6 +58,7 @@ int launch_band(bmg_ctx* c, uint32_t W, int ca, int cb, int32_t* out_dev, hipStr
 int launch_band_timed(bmg_ctx* c, uint32_t W, int ca, int cb, int32_t* out_dev, int reps, double* ms,
                       hipStream_t s);
 int launch_copy_timed(const void* src, void* dst, uint64_t bytes, int reps, double* ms, hipStream_t s);
+int launch_read_timed(const void* src, uint64_t bytes, void* sink, int reps, double* ms, hipStream_t s);
 int ensure(void** ptr, size_t* have, size_t need);
 }  // namespace bmg
 

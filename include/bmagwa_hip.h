@@ -136,9 +136,9 @@ int bmg_scan_xtr_timed(bmg_ctx* ctx, const double* r_dev, int nvec, int coding, 
  * every candidate shape and keeps the fastest; all zeros before that. */
 int bmg_scan_shape(bmg_ctx* ctx, int nvec, int digits, int32_t* out);
 /* Pin the scan shape for nvec * digits columns instead of tuning it:
- * tiles_per_wave 2 or 4, waves 4, 8 or 16, tile_words a multiple of 16
+ * tiles_per_wave 2 or 4, waves 4, 8 or 16, tile_words a multiple of 32
  * whose padded digit tile fits the LDS ((min(tile_words, words per SNP
- * rounded up to 16) * 16 + 16) * nvec * digits <= 160 KiB), workgroups >= 1
+ * rounded up to 32) * 16 + 16) * nvec * digits <= 160 KiB), workgroups >= 1
  * (clamped to the number of SNP blocks). */
 int bmg_scan_set_shape(bmg_ctx* ctx, int nvec, int digits, int tiles_per_wave, int waves,
                        uint32_t tile_words, uint32_t workgroups);
@@ -155,6 +155,12 @@ int bmg_dev_alloc(int device, uint64_t bytes, void** ptr);
 int bmg_dev_free(int device, void* ptr);
 int bmg_memcpy_h2d(int device, void* dst, const void* src, uint64_t bytes);
 int bmg_copy_timed(int device, const void* src_dev, void* dst_dev, uint64_t bytes, int reps,
+                   double* ms_per_launch);
+/* Streaming read of `bytes` (no counterpart in the reference): the best
+ * mean launch time over a few grid sizes, i.e. the read rate the device
+ * reaches, the practical ceiling of a read-bound kernel.  sink_dev: 4
+ * writable bytes that are written only in a case that does not occur. */
+int bmg_read_timed(int device, const void* src_dev, uint64_t bytes, void* sink_dev, int reps,
                    double* ms_per_launch);
 
 /* -------------------------------------------------------------- model --- */

@@ -138,10 +138,10 @@ def test_scan_shapes_all_exact(bmg):
     shape = ctx.scan_shape(3, 5)
     assert shape["tiles_per_wave"] in (2, 4) and shape["workgroups"] >= 1
     assert shape["waves"] in (4, 8, 16)
-    w32 = ((n + 15) // 16 + 3) // 4 * 4
+    w32 = ((n + 15) // 16 + 31) // 32 * 32
     for waves in (4, 8, 16):
         for nt in (2, 4):
-            for tw in (16, 272, 672):
+            for tw in (32, 288, 640):
                 for gx in (1, 5, 4096):
                     ctx.scan_set_shape(3, 5, nt, tw, gx, waves)
                     got = ctx.scan_xtr(R, coding=0, digits=5)
@@ -151,7 +151,9 @@ def test_scan_shapes_all_exact(bmg):
     with pytest.raises(bmg.BmgError):
         ctx.scan_set_shape(3, 5, 3, 272, 8)
     with pytest.raises(bmg.BmgError):
-        ctx.scan_set_shape(3, 5, 2, 688, 8)
+        ctx.scan_set_shape(3, 5, 2, 704, 8)
+    with pytest.raises(bmg.BmgError):
+        ctx.scan_set_shape(3, 5, 2, 272, 8)  # not a multiple of 32
     with pytest.raises(bmg.BmgError):
         ctx.scan_set_shape(3, 5, 2, 272, 8, waves=6)
 
