@@ -359,76 +359,103 @@ __global__ void k_scan_combine(const int32_t* __restrict__ acc, uint64_t p, int 
 }
 
 // ------------------------------------------------------ band X^T X ------
-// out[j * W + d] = sum_i x_j(i) x_{j+d}(i).  A workgroup owns 64 row SNPs
-// [r0, r0 + 64) and the column SNPs [r0, r0 + 64 + W); per chunk of 64
-// individuals it expands those SNPs to int8 in LDS once and then each wave
-// runs 16x16x64 int8 MFMAs over its share of the (row tile, column tile)
-// pairs that intersect the band.
-#define BAND_ROWS 64
-#define BAND_MAXP 36   // (row tile, column tile) pairs per wave; W <= 560
-__global__ __launch_bounds__(256) void k_band(const uint32_t* __restrict__ g, uint64_t p,
-                                              uint32_t W32, uint32_t W, int ca, int cb,
-                                              int32_t* __restrict__ out, uint64_t n_row_blocks) {
-  extern __shared__ uint32_t sx[];  // [2 codings][ncol SNPs][16 words]: int8 of 64 individuals
+// out[j * W + d] = sum_i x_j(i) x_{j+d}(i).  A workgroup of 8 waves owns
+// 128 row SNPs [r0, r0 + 128) and the column SNPs [r0, r0 + 128 + W); wave w
+// owns row tile w and every column tile within the band of it.  Per chunk
+// of 4 words (64 individuals) the workgroup expands those SNPs to int8 in
+// LDS once (two buffers: the chunk after is loaded from HBM into registers
+// while this chunk's MFMAs run, so one barrier per chunk suffices); the
+// wave then reads its row tile once and runs one 16x16x64 int8 MFMA per
+// column tile.  Both operands use the split byte order of word_split_i8x16,
+// so they agree on which individual each k is.  SNP slots are 80 bytes (64
+// used), so the 16 rows a lane group reads fall in distinct banks.
+#define BAND_ROWS 128
+#define BAND_WAVES 8
+#define BAND_MAXP 36   // column tiles per row tile: (W + 31) / 16 <= 36, W <= 560
+#define BAND_SLOT 20   // words per SNP slot in LDS
+template <int CA, int CB, int MAXP>
+__global__ __launch_bounds__(64 * BAND_WAVES) void k_band(const uint32_t* __restrict__ g, uint64_t p,
+                                                          uint32_t W32, uint32_t W,
+                                                          int32_t* __restrict__ out, uint64_t n_row_blocks) {
+  constexpr bool SAME = CA == CB;
+  extern __shared__ uint32_t sx[];  // 2 buffers x [ncol (+ 128 if !SAME) slots][BAND_SLOT words]
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   // XCD-aware ordering: workgroups are dealt round-robin over the 8 XCDs, so
   // row block rb goes to XCD rb / per; neighbouring row blocks (whose column
   // windows overlap) then share that XCD's L2.
-  uint64_t nb = n_row_blocks;
-  uint64_t per = (nb + 7) / 8;
-  uint64_t bid = blockIdx.x;
-  uint64_t rb = (bid % 8) * per + bid / 8;
+  const uint64_t nb = n_row_blocks;
+  const uint64_t per = (nb + 7) / 8;
+  const uint64_t bid = blockIdx.x;
+  const uint64_t rb = (bid % 8) * per + bid / 8;
   if (rb >= nb) return;
-  uint64_t r0 = rb * BAND_ROWS;
-  const uint32_t ncol_tiles = (BAND_ROWS + W + 15) / 16;
-  const uint32_t ncol = ncol_tiles * 16;
-  // column-tile offsets d per row tile: (ct - rt) * 16 < W + 16
-  const int ND = (int)((W + 16 + 15) / 16);
-  const int npairs = 4 * ND;
-  v4i c[BAND_MAXP];
+  const uint64_t r0 = rb * BAND_ROWS;
+  const uint32_t ncol = ((BAND_ROWS + W + 15) / 16) * 16;
+  const uint32_t nslot = ncol + (SAME ? 0 : BAND_ROWS);
+  const int ND = (int)((W + 31) / 16);  // column tiles ct - rt in [0, ND)
+  // the raw words of this thread: SNP slot sn, 16 bytes per chunk
+  constexpr int LPT = (BAND_ROWS + 560 + 15) / 16 * 16 / (64 * BAND_WAVES) + 1;  // loads per thread
+  uint4 raw[LPT];
+  auto load = [&](uint32_t wb) {
 #pragma unroll
-  for (int i = 0; i < BAND_MAXP; ++i) c[i] = v4i{0, 0, 0, 0};
-  uint32_t* xa = sx;
-  uint32_t* xb = sx + (size_t)ncol * 16;
+    for (int i = 0; i < LPT; ++i) {
+      const uint32_t sn = threadIdx.x + i * 64 * BAND_WAVES;
+      const uint64_t snp = min(r0 + sn, p - 1);
+      raw[i] = (sn < ncol && r0 + sn < p) ? *(const uint4*)(g + snp * W32 + wb) : make_uint4(0, 0, 0, 0);
+    }
+  };
+  v4i c[MAXP];
+#pragma unroll
+  for (int i = 0; i < MAXP; ++i) c[i] = v4i{0, 0, 0, 0};
   const int row = lane & 15, grp = lane >> 4;
-  for (uint32_t wb = 0; wb < W32; wb += 4) {
-    __syncthreads();
-    for (uint32_t e = threadIdx.x; e < ncol * 4; e += blockDim.x) {
-      uint32_t sn = e / 4, q = e % 4;
-      uint64_t snp = r0 + sn;
-      uint32_t w = (snp < p && wb + q < W32) ? g[snp * W32 + wb + q] : 0u;
-      uint32_t oa[4], ob[4];
-      word_to_i8x16(w, ca, oa);
-      word_to_i8x16(w, cb, ob);
-      *(uint4*)(xa + (size_t)sn * 16 + q * 4) = make_uint4(oa[0], oa[1], oa[2], oa[3]);
-      *(uint4*)(xb + (size_t)sn * 16 + q * 4) = make_uint4(ob[0], ob[1], ob[2], ob[3]);
+  const uint32_t nch = W32 / 4;
+  const uint32_t last_tile = ncol / 16 - 1;
+  load(0);
+  for (uint32_t ch = 0; ch < nch; ++ch) {
+    uint32_t* buf = sx + (size_t)(ch & 1) * nslot * BAND_SLOT;
+#pragma unroll
+    for (int i = 0; i < LPT; ++i) {
+      const uint32_t sn = threadIdx.x + i * 64 * BAND_WAVES;
+      if (sn < ncol) {
+        uint32_t o[4];
+        const uint32_t w4[4] = {raw[i].x, raw[i].y, raw[i].z, raw[i].w};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          word_split_i8x16<CB>(w4[q], o);
+          *(uint4*)(buf + sn * BAND_SLOT + q * 4) = make_uint4(o[0], o[1], o[2], o[3]);
+        }
+        if (!SAME && sn < BAND_ROWS) {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            word_split_i8x16<CA>(w4[q], o);
+            *(uint4*)(buf + (ncol + sn) * BAND_SLOT + q * 4) = make_uint4(o[0], o[1], o[2], o[3]);
+          }
+        }
+      }
     }
     __syncthreads();
+    if (ch + 1 < nch) load((ch + 1) * 4);
+    const uint32_t* xa = buf + (SAME ? 0 : ncol * BAND_SLOT);
+    const uint4 av = *(const uint4*)(xa + (wave * 16 + row) * BAND_SLOT + grp * 4);
+    const v4i a = {(int)av.x, (int)av.y, (int)av.z, (int)av.w};
+    // MAXP >= ND column tiles, unconditionally (so the LDS reads batch); the
+    // tiles past ND read a clamped slot and are never stored
 #pragma unroll
-    for (int i = 0; i < BAND_MAXP; ++i) {
-      int k = wave + 4 * i;
-      if (k < npairs) {
-        int rt = k / ND, ct = rt + k % ND;
-        uint32_t sa = rt * 16 + row, sb = ct * 16 + row;
-        uint4 av = *(const uint4*)(xa + (size_t)sa * 16 + grp * 4);
-        uint4 bv = *(const uint4*)(xb + (size_t)sb * 16 + grp * 4);
-        v4i a = {(int)av.x, (int)av.y, (int)av.z, (int)av.w};
-        v4i b = {(int)bv.x, (int)bv.y, (int)bv.z, (int)bv.w};
-        c[i] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, c[i], 0, 0, 0);
-      }
+    for (int d = 0; d < MAXP; ++d) {
+      const uint32_t sb = min((uint32_t)(wave + d), last_tile) * 16 + row;
+      const uint4 bv = *(const uint4*)(buf + sb * BAND_SLOT + grp * 4);
+      const v4i b = {(int)bv.x, (int)bv.y, (int)bv.z, (int)bv.w};
+      c[d] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, c[d], 0, 0, 0);
     }
   }
   const int col = lane & 15;
 #pragma unroll
-  for (int i = 0; i < BAND_MAXP; ++i) {
-    int k = wave + 4 * i;
-    if (k < npairs) {
-      int rt = k / ND, ct = rt + k % ND;
+  for (int d = 0; d < MAXP; ++d) {
+    if (d < ND) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        uint64_t j = r0 + rt * 16 + 4 * grp + r;   // row SNP
-        uint64_t jj = r0 + ct * 16 + col;          // column SNP
-        if (j < p && jj >= j && jj - j < W && jj < p) out[j * W + (jj - j)] = c[i][r];
+        const uint64_t j = r0 + wave * 16 + 4 * grp + r;   // row SNP
+        const uint64_t jj = r0 + (wave + d) * 16 + col;    // column SNP
+        if (j < p && jj >= j && jj - j < W && jj < p) out[j * W + (jj - j)] = c[d][r];
       }
     }
   }
@@ -798,18 +825,44 @@ int launch_read_timed(const void* src, uint64_t bytes, void* sink, int reps, dou
   return BMG_OK;
 }
 
+template <int CA, int CB>
+static void band_p(bmg_ctx* c, uint32_t W, int32_t* out_dev, uint64_t nrb, unsigned grid, hipStream_t s) {
+  const uint32_t ncol = ((BAND_ROWS + W + 15) / 16) * 16;
+  const size_t lds = (size_t)2 * (ncol + (CA == CB ? 0 : BAND_ROWS)) * BAND_SLOT * sizeof(uint32_t);
+  const dim3 blk(64 * BAND_WAVES);
+  const int nd = (int)((W + 31) / 16);
+  if (nd <= 4)
+    hipLaunchKernelGGL((k_band<CA, CB, 4>), dim3(grid), blk, lds, s, c->g, c->p, c->W32, W, out_dev, nrb);
+  else if (nd <= 12)
+    hipLaunchKernelGGL((k_band<CA, CB, 12>), dim3(grid), blk, lds, s, c->g, c->p, c->W32, W, out_dev, nrb);
+  else
+    hipLaunchKernelGGL((k_band<CA, CB, BAND_MAXP>), dim3(grid), blk, lds, s, c->g, c->p, c->W32, W, out_dev, nrb);
+}
+
+template <int CA>
+static void band_cb(bmg_ctx* c, uint32_t W, int cb, int32_t* out_dev, uint64_t nrb, unsigned grid, hipStream_t s) {
+  switch (cb) {
+    case 0: band_p<CA, 0>(c, W, out_dev, nrb, grid, s); break;
+    case 1: band_p<CA, 1>(c, W, out_dev, nrb, grid, s); break;
+    case 2: band_p<CA, 2>(c, W, out_dev, nrb, grid, s); break;
+    default: band_p<CA, 3>(c, W, out_dev, nrb, grid, s); break;
+  }
+}
+
 int launch_band(bmg_ctx* c, uint32_t W, int ca, int cb, int32_t* out_dev, hipStream_t s) {
   if (W < 1 || W > 560) { set_error("band: 1 <= W <= 560"); return BMG_E_ARG; }
   if ((W + 31) / 16 > BAND_MAXP) { set_error("band: W too large"); return BMG_E_ARG; }
   uint64_t nrb = (c->p + BAND_ROWS - 1) / BAND_ROWS;
-  uint32_t ncol = ((BAND_ROWS + W + 15) / 16) * 16;
-  size_t lds = (size_t)2 * ncol * 16 * sizeof(uint32_t);
   unsigned grid = (unsigned)(((nrb + 7) / 8) * 8);
-  hipLaunchKernelGGL(k_band, dim3(grid), dim3(256), lds, s, c->g, c->p, c->W32, W, ca, cb, out_dev, nrb);
+  switch (ca) {
+    case 0: band_cb<0>(c, W, cb, out_dev, nrb, grid, s); break;
+    case 1: band_cb<1>(c, W, cb, out_dev, nrb, grid, s); break;
+    case 2: band_cb<2>(c, W, cb, out_dev, nrb, grid, s); break;
+    default: band_cb<3>(c, W, cb, out_dev, nrb, grid, s); break;
+  }
   BMG_HIP(hipGetLastError());
   return BMG_OK;
 }
-
 int launch_band_timed(bmg_ctx* c, uint32_t W, int ca, int cb, int32_t* out_dev, int reps, double* ms,
                       hipStream_t s) {
   hipEvent_t e0, e1;

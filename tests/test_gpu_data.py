@@ -75,11 +75,15 @@ def test_recode_matches_oracle(bmg):
     assert np.array_equal(ctx.read_body(), d.body_now())
 
 
-@pytest.mark.parametrize("W,ca,cb", [(8, 0, 0), (40, 0, 0), (17, 0, 1), (100, 3, 2)])
-def test_band_xtx_exact(bmg, W, ca, cb):
-    rng = np.random.default_rng(W)
-    n, p = 517, 300
-    G, body = random_body(rng, n, p)
+@pytest.mark.parametrize("W,ca,cb,p", [(8, 0, 0, 300), (40, 0, 0, 300), (17, 0, 1, 300), (100, 3, 2, 300),
+                                        (1, 0, 0, 300), (21, 0, 0, 389), (64, 1, 1, 300), (200, 2, 0, 700),
+                                        (500, 0, 0, 700), (560, 3, 3, 900), (560, 1, 2, 700)])
+def test_band_xtx_exact(bmg, W, ca, cb, p):
+    """Every column-tile bucket (W up to 560), same and different codings,
+    row blocks cut by p, missing genotypes."""
+    rng = np.random.default_rng(W + p)
+    n = 517
+    G, body = random_body(rng, n, p, with_missing=True)
     ctx = bmg.Ctx(n, p, body, rng.normal(size=n))
     got = ctx.band_xtx(W, ca, cb)
     ref = np.zeros(p * W, dtype=np.int64)
