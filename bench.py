@@ -19,11 +19,16 @@ before the timed steps):
 * batched_full:    `--full-chains` chains (256: one per CU) with the learned
   swap, to show the whole GPU (reported beside the headline).
 
-PIP ESS (median over the monitored SNPs: the causal SNPs plus those with a
-pooled PIP in [0.05, 0.95]) is Geyer's initial-monotone-sequence ESS of the
-thinned inclusion indicators of the timed steps, summed over chains, per
-wall second of the timed steps.  `value` is the learned_batched figure
-(the batched one with --no-learned).
+PIP ESS (median over the monitored SNPs: those whose PIP in the long
+plain-method run lies in [0.05, 0.95]) is the multi-chain ESS
+(`bmagwa_amd.ess.ess_multichain`: pooled within/between-chain variance,
+Geyer's initial monotone sequence) of the thinned inclusion indicators of
+the timed steps of all chains of a run, per wall second of the timed steps.
+`value` is the learned_batched figure (the batched one with --no-learned).
+
+`pip_check` compares each run's Rao-Blackwellised PIPs with a long
+plain-method run (`--pip-ref-chains` chains, 256 by default, plain moves,
+`--pip-ref-steps` timed steps, its own seeds).
 
 `roofline` covers the all-SNP X^T r scan kernel (HBM-bound) at the launch
 shape the batch uses (3 residual vectors x 5 digit planes), timed with HIP
@@ -65,13 +70,14 @@ def pooled_counts(th, first, p):
     return np.bincount(th["loci"][off:].astype(np.int64), minlength=p), len(sizes) - first
 
 
-def run_batch(bmg, synth, ctx, n_chains, steps, warmup, ips, seed0, batch_cfg=None):
+def run_batch(bmg, synth, ctx, n_chains, steps, warmup, ips, seed0, batch_cfg=None, thin=1):
     total = (warmup + steps) * ips + 1000     # sizes the trace buffers
     cfgs = [synth.sampler_cfg(N, P, N_CAUSAL, types="A", seed=seed0 + c, chain=c,
-                              do_n_iter=total, n_rao_burnin=4)
+                              do_n_iter=total, n_rao_burnin=4, thin=thin)
             for c in range(n_chains)]
     bc = dict(batch_cfg or {})
-    bc["trace_capacity"] = (total // 10 + 16) * 200    # thinned loci: up to 200 per sample
+    bc["trace_capacity"] = (total // thin + 16) * 200    # thinned loci: up to 200 per sample
+    bc.setdefault("use_graph", 1)     # after burn-in: segments replayed from a captured hipGraph
     t0 = time.perf_counter()
     b = bmg.Batch([ctx], [0] * n_chains, cfgs, bc)
     b.setup()
@@ -89,7 +95,8 @@ def run_batch(bmg, synth, ctx, n_chains, steps, warmup, ips, seed0, batch_cfg=No
     b.sync()
     wall = time.perf_counter() - t0
     tm1 = b.timing()
-    log("%d chains: %d timed steps in %.2f s" % (n_chains, steps, wall))
+    gs = b.graph_stats()
+    log("%d chains: %d timed steps in %.2f s (%d graph replays)" % (n_chains, steps, wall, gs["replays"]))
     ths = [b.thinned(c) for c in range(n_chains)]
     sizes = [b.state(c)["size"] for c in range(n_chains)]
     # Rao-Blackwellised PIPs averaged over the post-burn-in RB scans, mean of chains
@@ -106,20 +113,22 @@ def run_batch(bmg, synth, ctx, n_chains, steps, warmup, ips, seed0, batch_cfg=No
                      learned_swap_accept=round(float((jd[k0:][learned] > 0).mean()), 3)
                      if learned.any() else None)
     b.close()
-    return dict(wall=wall, setup=t_setup, ths=ths, first=first, sizes=sizes, acc=acc, learn=learn,
+    return dict(wall=wall, steps=steps, setup=t_setup, ths=ths, first=first, sizes=sizes, acc=acc, learn=learn,
+                graph_replays=gs["replays"], graph_ms=gs["graph_ms"],
                 p_rao=p_rao,
                 scan_ms=tm1["scan_ms"] - tm0["scan_ms"], chain_ms=tm1["chain_ms"] - tm0["chain_ms"],
                 n_scans=tm1["n_scans"] - tm0["n_scans"])
 
 
 def ess_of(run, monitored, wall):
+    """Multi-chain PIP ESS per wall second: (median, min) over monitored SNPs."""
     from bmagwa_amd import ess
-    tot = np.zeros(len(monitored))
-    for th, f in zip(run["ths"], run["first"]):
-        M = indicator_rows(th, monitored, f)
-        if M.shape[1] >= 4:
-            tot += ess.ess_geyer(M)
-    return float(np.median(tot)) / wall, float(np.min(tot)) / wall
+    mats = [indicator_rows(th, monitored, f) for th, f in zip(run["ths"], run["first"])]
+    T = min(M.shape[1] for M in mats)
+    if T < 4:
+        return 0.0, 0.0
+    e = ess.ess_multichain(np.stack([M[:, :T] for M in mats]))
+    return float(np.median(e)) / wall, float(np.min(e)) / wall
 
 
 def scan_roofline(bmg, ctx):
@@ -169,7 +178,7 @@ def read_calibration(bmg, nbytes=1 << 31):
     return dict(bytes=nbytes, ms=ms, gbs=nbytes / (ms * 1e-3) / 1e9)
 
 
-def cpu_baseline(ctx, y, monitored, seconds):
+def cpu_baseline(ctx, y, monitored, seconds, thin):
     import oracle
     oracle.load()
     body = ctx.read_body()
@@ -178,7 +187,7 @@ def cpu_baseline(ctx, y, monitored, seconds):
     del body
     from bmagwa_amd import synth
     cfg = synth.sampler_cfg(N, P, N_CAUSAL, types="A", seed=9001, chain=0, do_n_iter=10 ** 8,
-                            n_rao_burnin=4)
+                            n_rao_burnin=4, thin=thin)
     s = oracle.Sampler(cfg, d)
     s.init_flat()
     s.setup()
@@ -197,7 +206,7 @@ def cpu_baseline(ctx, y, monitored, seconds):
     from bmagwa_amd import ess
     th = s.thinned()
     M = ess.indicator_matrix(th["sizes"], th["loci"], monitored, burn=first)
-    e = ess.ess_geyer(M) if M.shape[1] >= 4 else np.zeros(len(monitored))
+    e = ess.ess_multichain(M[None]) if M.shape[1] >= 4 else np.zeros(len(monitored))
     log("cpu oracle: %d iterations in %.1f s" % (its, wall))
     return dict(value=float(np.median(e)) / wall, iters_per_s=its / wall, iters=int(its),
                 wall=wall, samples=int(M.shape[1]), setup_s=t_setup)
@@ -220,6 +229,15 @@ def main():
                     help="also run the learned batch with this many chains (e.g. 256: one per CU)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="time only the scan, band and copy kernels (for profiler runs)")
+    ap.add_argument("--thin", type=int, default=10,
+                    help="thinning (part of the sampler's schedule: beta and sigma2 are drawn at "
+                         "every thinned iteration, as in the reference)")
+    ap.add_argument("--full-steps", type=int, default=0,
+                    help="timed steps of the --full-chains run (default: --steps)")
+    ap.add_argument("--pip-ref-chains", type=int, default=256,
+                    help="chains of the long plain-moves reference run for the PIP check (0: skip)")
+    ap.add_argument("--pip-ref-steps", type=int, default=30,
+                    help="timed steps of the PIP reference run")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -249,50 +267,70 @@ def main():
 
     ips = args.iters_per_step
     runs = {}
+    th = args.thin
     if args.learned:
         runs["learned_batched"] = run_batch(bmg, synth, ctx, args.chains, args.steps, args.warmup, ips,
-                                            100, dict(proposal=1, trace_every_iter=1))
+                                            100, dict(proposal=1, trace_every_iter=1), th)
     runs["batched"] = run_batch(bmg, synth, ctx, args.chains, args.steps, args.warmup, ips, 100,
-                                dict(proposal=0))
+                                dict(proposal=0), th)
     if not args.no_plain:
         runs["plain"] = run_batch(bmg, synth, ctx, 1, args.steps, args.warmup, ips, 100,
-                                  dict(proposal=0))
+                                  dict(proposal=0), th)
     if args.full_chains > 0:
-        runs["batched_full"] = run_batch(bmg, synth, ctx, args.full_chains, args.steps, args.warmup, ips,
-                                         100, dict(proposal=1 if args.learned else 0, trace_every_iter=1))
+        runs["batched_full"] = run_batch(bmg, synth, ctx, args.full_chains, args.full_steps or args.steps,
+                                         args.warmup, ips,
+                                         100, dict(proposal=1 if args.learned else 0, trace_every_iter=1),
+                                         th)
+    # the long plain-method run the PIPs are checked against (its own seeds;
+    # RB PIPs only, so its samples are thinned)
+    pip_ref = None
+    if args.pip_ref_chains > 0:
+        pip_ref = run_batch(bmg, synth, ctx, args.pip_ref_chains, args.pip_ref_steps, args.warmup, ips,
+                            50000, dict(proposal=0), 10)
+        pip_ref["ths"] = None
 
-    counts = np.zeros(P, np.int64)
-    total = 0
-    for run in runs.values():
-        for th, f in zip(run["ths"], run["first"]):
-            c, t = pooled_counts(th, f, P)
-            counts += c
-            total += t
-    pip = counts / max(total, 1)
-    monitored = np.union1d(causal, np.nonzero((pip >= 0.05) & (pip <= 0.95))[0])
+    if pip_ref is not None:
+        # SNPs whose PIP is uncertain according to the long plain-method run
+        monitored = np.nonzero((pip_ref["p_rao"] >= 0.05) & (pip_ref["p_rao"] <= 0.95))[0]
+    else:
+        counts = np.zeros(P, np.int64)
+        total = 0
+        for run in runs.values():
+            for th, f in zip(run["ths"], run["first"]):
+                c, t = pooled_counts(th, f, P)
+                counts += c
+                total += t
+        pip = counts / max(total, 1)
+        monitored = np.nonzero((pip >= 0.05) & (pip <= 0.95))[0]
     summary = {}
     for name, run in runs.items():
         med, mn = ess_of(run, monitored, run["wall"])
         nch = len(run["sizes"])
         summary[name] = dict(chains=nch, ess_per_s_median=round(med, 3), ess_per_s_min=round(mn, 3),
-                             iters_per_s=round(ips * args.steps * nch / run["wall"], 1),
-                             ms_per_step=round(run["wall"] / args.steps * 1e3, 2),
+                             iters_per_s=round(ips * run["steps"] * nch / run["wall"], 1),
+                             ms_per_step=round(run["wall"] / run["steps"] * 1e3, 2),
                              model_size_mean=round(float(np.mean(run["sizes"])), 1),
                              accept_rate_mean=round(float(np.mean(run["acc"])), 4),
-                             rb_block_ms=round(run["scan_ms"], 1), chain_kernel_ms=round(run["chain_ms"], 1))
+                             rb_block_ms=round(run["scan_ms"], 1), chain_kernel_ms=round(run["chain_ms"], 1),
+                             graph_segments=run["graph_replays"], graph_ms=round(run["graph_ms"], 1))
         if run.get("learn"):
             summary[name]["learner"] = run["learn"]
     head = "learned_batched" if args.learned else "batched"
     lb = summary[head]
     b_med = lb["ess_per_s_median"]
-    # PIPs of the headline run against the plain-move runs (RB estimates)
+    # Rao-Blackwellised PIPs of each run against the long plain-method run
     pip_check = {}
-    for ref in ["batched", "batched_full"]:
-        if ref in runs and ref != head:
-            dpip = np.abs(runs[head]["p_rao"] - runs[ref]["p_rao"])
-            pip_check["vs_" + ref] = dict(max_abs_diff=round(float(dpip.max()), 4),
-                                          max_abs_diff_monitored=round(float(dpip[monitored].max()), 4))
-
+    if pip_ref is not None:
+        n_ref = args.pip_ref_chains * (args.warmup + args.pip_ref_steps) * ips
+        pip_check["reference"] = ("plain moves, %d chains x %d iterations (%d after burn-in), RB PIPs"
+                                  % (args.pip_ref_chains, (args.warmup + args.pip_ref_steps) * ips,
+                                     args.pip_ref_steps * ips))
+        pip_check["reference_iterations"] = n_ref
+        for name, run in runs.items():
+            dpip = np.abs(run["p_rao"] - pip_ref["p_rao"])
+            pip_check[name] = dict(max_abs_diff=round(float(dpip.max()), 4),
+                                   max_abs_diff_monitored=round(float(dpip[monitored].max()), 4),
+                                   argmax_snp=int(np.argmax(dpip)))
     roof = None
     extra = {}
     if not args.no_extra:
@@ -314,7 +352,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        c = cpu_baseline(ctx, y, monitored, args.cpu_seconds)
+        c = cpu_baseline(ctx, y, monitored, args.cpu_seconds, args.thin)
         cpu = dict(value=c["value"], unit="PIP ESS/s (median)", cores=1, kind="port",
                    iters_per_s=round(c["iters_per_s"], 1),
                    sample="oracle plain PMV, 1 chain on 1 core, same dataset and settings: "
@@ -327,7 +365,7 @@ def main():
     line = {
         "metric": "PIP ESS/sec + MCMC iters/sec, learned+batched vs plain, 1 MI355X",
         "value": round(b_med, 3),
-        "unit": "PIP ESS/s (median over monitored SNPs, summed over chains)",
+        "unit": "PIP ESS/s (median over monitored SNPs; multi-chain ESS of all chains of the run)",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(wall / args.steps * 1e3, 2),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,

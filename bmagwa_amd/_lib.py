@@ -177,6 +177,7 @@ SIGNATURES = {
     "bmg_batch_thinned": (C.c_int, _P, C.c_int, _U32, C.c_int64, _U32, C.c_int64, _D, _D, _D, _I64, _I64),
     "bmg_batch_rb": (C.c_int, _P, C.c_int, _D, _D),
     "bmg_batch_timing": (C.c_int, _P, _D),
+    "bmg_batch_graph_stats": (C.c_int, _P, _D),
     "bmg_batch_profile": (C.c_int, _P, C.c_int, _D),
 }
 
@@ -621,6 +622,12 @@ class Batch:
         o = np.zeros(4)
         check(lib().bmg_batch_timing(self.h, _p(o)))
         return dict(total_ms=o[0], chain_ms=o[1], scan_ms=o[2], n_scans=int(o[3]))
+
+    def graph_stats(self):
+        """Steady-state graph: segments replayed, ms in replays, builds."""
+        o = np.zeros(3)
+        check(lib().bmg_batch_graph_stats(self.h, _p(o)))
+        return dict(replays=int(o[0]), graph_ms=o[1], builds=int(o[2]))
 
     def close(self):
         if getattr(self, "h", None):

@@ -395,9 +395,13 @@ struct bmg_batch {
   // timing of the last run
   double t_total = 0, t_chain = 0, t_scan = 0;
   int n_scans = 0;
-  // graph of one full segment
+  // graph of one steady-state segment (n_rao iterations + its RB block)
   hipGraphExec_t graph_exec = nullptr;
   hipGraph_t graph = nullptr;
+  int* d_glist = nullptr;                 // every dataset's chains, in dataset order
+  std::vector<const void*> graph_sig;     // buffers the graph's nodes point at
+  int graph_scans = 0;                    // scan launches per segment
+  double n_replays = 0, t_graph = 0, n_builds = 0;
   // learned-swap weights per dataset (p x 2H floats)
   std::vector<float*> lw_dev;
   // online learned proposal per dataset (proposal = 1)
@@ -647,6 +651,7 @@ extern "C" int bmg_batch_destroy(bmg_batch* b) {
   hipFree(b->r); hipFree(b->xa); hipFree(b->xh); hipFree(b->rsum); hipFree(b->d_list);
   if (b->graph_exec) hipGraphExecDestroy(b->graph_exec);
   if (b->graph) hipGraphDestroy(b->graph);
+  hipFree(b->d_glist);
   if (b->stream) hipStreamDestroy(b->stream);
   delete b;
   return BMG_OK;
@@ -664,7 +669,7 @@ static int batch_create(bmg_ctx** ctxs, int n_ctx, int n_chains, const int32_t* 
   b->ctxs.assign(ctxs, ctxs + n_ctx);
   b->n_chains = n_chains;
   if (cfg) b->cfg = *cfg;
-  int kmax = b->cfg.use_graph >= 0 ? 128 : 128;
+  int kmax = 128;
   b->device = ctxs[0]->device;
   hipSetDevice(b->device);
   // capacity: from the prior's model size (e_qg + 6 sd, AH doubles the columns)
@@ -905,6 +910,120 @@ extern "C" int bmg_batch_setup(bmg_batch* b) {
   return BMG_OK;
 }
 
+// --- steady-state graph ---------------------------------------------------
+// After burn-in (every chain's n_rao_burnin at 0, every learner frozen) each
+// RB point triggers the same device work for every chain: a scan of every
+// chain of every dataset, the element-wise update and the tree rebuild.  A
+// segment -- the chain kernel over n_rao iterations plus that RB block -- is
+// then captured once into a hipGraph and replayed; the host decides nothing
+// between segments.  The captured sequence is the eager one with the chain
+// lists read from a device array and without host round trips.
+static int steady_state(bmg_batch* b) {
+  std::vector<ChainScal> cs(b->n_chains);
+  BMG_HIP(hipStreamSynchronize(b->stream));
+  BMG_HIP(hipMemcpy(cs.data(), b->d_scal, sizeof(ChainScal) * b->n_chains, hipMemcpyDeviceToHost));
+  for (const ChainScal& c : cs)
+    if (c.n_rao_burnin > 0 || c.rb_pending || c.err) return 0;
+  for (LearnDS* L : b->learn)
+    if (L && !learn_frozen(L)) return 0;
+  return 1;
+}
+
+// the buffers a captured segment points at; a change means a rebuild
+static std::vector<const void*> graph_signature(bmg_batch* b) {
+  std::vector<const void*> v = {b->r, b->xa, b->xh, b->rsum, b->d_glist};
+  for (bmg_ctx* c : b->ctxs) { v.push_back(c->dig); v.push_back(c->acc); v.push_back(c->scale); }
+  return v;
+}
+
+static int build_graph(bmg_batch* b) {
+  if (b->graph_exec) { hipGraphExecDestroy(b->graph_exec); b->graph_exec = nullptr; }
+  if (b->graph) { hipGraphDestroy(b->graph); b->graph = nullptr; }
+  const int digits = b->cfg.scan_digits > 0 ? b->cfg.scan_digits : 5;
+  const int per = 16 / digits;
+  // chain lists and scratch at their final sizes before the capture
+  std::vector<int> order;
+  std::vector<size_t> ds_off;
+  for (size_t ds = 0; ds < b->ctxs.size(); ++ds) {
+    ds_off.push_back(order.size());
+    for (int c = 0; c < b->n_chains; ++c) if (b->chain_ds[c] == (int)ds) order.push_back(c);
+  }
+  ds_off.push_back(order.size());
+  if (!b->d_glist) BMG_HIP(hipMalloc(&b->d_glist, sizeof(int) * b->n_chains));
+  BMG_HIP(hipMemcpy(b->d_glist, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice));
+  for (size_t ds = 0; ds < b->ctxs.size(); ++ds)
+    if (ensure_rb_scratch(b, per, b->h_data[ds].n_pad, b->h_data[ds].p)) return BMG_E_HIP;
+  int scans = 0;
+  BMG_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed));
+  int rc = launch_chain_kernel(b, b->n_rao, 0);
+  for (size_t ds = 0; ds < b->ctxs.size() && !rc; ++ds) {
+    bmg_ctx* c = b->ctxs[ds];
+    const DevData& D = b->h_data[ds];
+    c->no_tune = 1;
+    for (size_t s0 = ds_off[ds]; s0 < ds_off[ds + 1] && !rc; s0 += per) {
+      const int nv = (int)std::min<size_t>(per, ds_off[ds + 1] - s0);
+      const int* list = b->d_glist + s0;
+      if (launch_residuals(c->y, D.n, D.n_pad, b->d_arrays, list, nv, b->r, b->rsum, b->stream)) {
+        set_error("residual kernel launch failed"); rc = BMG_E_HIP; break;
+      }
+      const int t = D.type;
+      rc = launch_scan(c, b->r, nv, t == BMG_AH ? 0 : t, digits, b->xa, b->stream);
+      if (!rc && t == BMG_AH) rc = launch_scan(c, b->r, nv, 1, digits, b->xh, b->stream);
+      scans += (t == BMG_AH) ? 2 : 1;
+      if (!rc && launch_rb_finalize(b->d_data, b->d_consts, b->d_scal, b->d_arrays, list, nv, b->kmax, D.p,
+                                    b->xa, b->xh, b->rsum, b->stream)) {
+        set_error("rb finalize launch failed"); rc = BMG_E_HIP;
+      }
+    }
+    for (size_t s0 = ds_off[ds]; s0 < ds_off[ds + 1] && !rc; s0 += 1024) {
+      const int nv = (int)std::min<size_t>(1024, ds_off[ds + 1] - s0);
+      const int* list = b->d_glist + s0;
+      if (launch_rb_update(b->d_data, b->d_consts, b->d_scal, b->d_arrays, list, nv, D.p, b->stream) ||
+          launch_tree_build(b->d_data, b->d_consts, b->d_scal, b->d_arrays, list, nv, D.lev_len, D.top, 1, 1,
+                            b->stream)) {
+        set_error("rb update launch failed"); rc = BMG_E_HIP;
+      }
+    }
+    c->no_tune = 0;
+  }
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(b->stream, &g);
+  if (rc) { if (g) hipGraphDestroy(g); return rc; }
+  if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+  b->graph = g;
+  e = hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+  b->graph_sig = graph_signature(b);
+  b->graph_scans = scans;
+  b->n_builds += 1;
+  return BMG_OK;
+}
+
+// replays `segments` captured segments; the batch is at an RB boundary in
+// its steady state
+static int run_graph(bmg_batch* b, uint64_t segments) {
+  if (!b->graph_exec || graph_signature(b) != b->graph_sig) {
+    int rc = build_graph(b);
+    if (rc) return rc;
+  }
+  hipEvent_t g0, g1;
+  BMG_HIP(hipEventCreate(&g0));
+  BMG_HIP(hipEventCreate(&g1));
+  BMG_HIP(hipEventRecord(g0, b->stream));
+  for (uint64_t k = 0; k < segments; ++k) BMG_HIP(hipGraphLaunch(b->graph_exec, b->stream));
+  BMG_HIP(hipEventRecord(g1, b->stream));
+  BMG_HIP(hipEventSynchronize(g1));
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, g0, g1);
+  (void)hipEventDestroy(g0);
+  (void)hipEventDestroy(g1);
+  b->t_graph += ms;
+  b->n_replays += (double)segments;
+  b->n_scans += (int)(segments * b->graph_scans);
+  b->iter += segments * b->n_rao;
+  return BMG_OK;
+}
+
 extern "C" int bmg_batch_run(bmg_batch* b, uint64_t n_iter) {
   if (!b) { set_error("null batch"); return BMG_E_ARG; }
   if (!b->setup_done) { set_error("call bmg_batch_setup first"); return BMG_E_STATE; }
@@ -915,6 +1034,13 @@ extern "C" int bmg_batch_run(bmg_batch* b, uint64_t n_iter) {
   uint64_t remaining = n_iter;
   int rc = BMG_OK;
   while (remaining > 0) {
+    if (b->cfg.use_graph && b->iter % b->n_rao == 0 && remaining >= b->n_rao && steady_state(b)) {
+      const uint64_t segs = remaining / b->n_rao;
+      rc = run_graph(b, segs);
+      if (rc) break;
+      remaining -= segs * b->n_rao;
+      continue;
+    }
     uint64_t next_rb = (b->iter / b->n_rao + 1) * b->n_rao;
     uint64_t run = std::min<uint64_t>(remaining, next_rb - b->iter);
     hipEventRecord(c0, b->stream);
@@ -944,6 +1070,12 @@ extern "C" int bmg_batch_run(bmg_batch* b, uint64_t n_iter) {
   std::vector<ChainScal> cs(b->n_chains);
   BMG_HIP(hipMemcpy(cs.data(), b->d_scal, sizeof(ChainScal) * b->n_chains, hipMemcpyDeviceToHost));
   return numeric_error(cs);
+}
+
+extern "C" int bmg_batch_graph_stats(bmg_batch* b, double out[3]) {
+  if (!b || !out) { set_error("null argument"); return BMG_E_ARG; }
+  out[0] = b->n_replays; out[1] = b->t_graph; out[2] = b->n_builds;
+  return BMG_OK;
 }
 
 // learned swap on/off for the chains of dataset ds (see move_learned_swap)

@@ -650,6 +650,10 @@ static int scan_tuned_shape(bmg_ctx* c, int nvec, int coding, int digits, hipStr
     *out = make_shape(c, NC, c->scan_nt[NC], c->scan_wv[NC], c->scan_tw[NC], c->scan_gx[NC]);
     return BMG_OK;
   }
+  if (c->no_tune) {  // a stream capture cannot time: a fixed shape, not remembered
+    *out = make_shape(c, NC, 2, 8, tile_for(NC, 163840), 256);
+    return BMG_OK;
+  }
   // 256 CUs: grids of 1 to 8 workgroups per CU, over the individual tiles
   std::vector<ScanShape> cand;
   const uint32_t tws[2] = {tile_for(NC, 65536), tile_for(NC, 163840)};

@@ -35,6 +35,7 @@ struct bmg_ctx {
   double* tmp_out = nullptr; size_t tmp_out_bytes = 0;
   // scan shapes picked by the autotuner, per number of digit columns
   int scan_nt[17] = {0}, scan_wv[17] = {0};
+  int no_tune = 0;  // inside a graph capture: no timed tuning (default shape)
   uint32_t scan_tw[17] = {0};
   unsigned scan_gx[17] = {0};
 };

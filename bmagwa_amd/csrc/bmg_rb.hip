@@ -11,6 +11,9 @@
 using namespace bmg;
 
 // r_v = y - y_hat of chain list[v] (zero past n); sums of r per vector
+// r = y - y_hat for chain list[v], and rsum[v] = sum of r: one workgroup per
+// vector, so the sum is reduced in a fixed order (the RB PIPs do not depend
+// on the order in which workgroups finish)
 __global__ void k_residuals(const double* __restrict__ y, uint32_t n, uint32_t n_pad,
                             const ChainArrays* __restrict__ arrays, const int* __restrict__ list,
                             double* __restrict__ r, double* __restrict__ rsum) {
@@ -18,13 +21,13 @@ __global__ void k_residuals(const double* __restrict__ y, uint32_t n, uint32_t n
   int v = blockIdx.y;
   const double* yh = arrays[list[v]].y_hat;
   double s = 0.0;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_pad; i += gridDim.x * blockDim.x) {
+  for (uint32_t i = threadIdx.x; i < n_pad; i += blockDim.x) {
     double ri = i < n ? y[i] - yh[i] : 0.0;
     r[(size_t)v * n_pad + i] = ri;
     s += ri;
   }
   s = block_sum(s, red);
-  if (threadIdx.x == 0) atomicAdd(&rsum[v], s);
+  if (threadIdx.x == 0) rsum[v] = s;
 }
 
 __device__ inline double rb_prior_add(const ChainConst& K, double Ns_t, double n_old) {
@@ -188,10 +191,7 @@ namespace bmg {
 
 int launch_residuals(const double* y, uint32_t n, uint32_t n_pad, const ChainArrays* arrays,
                      const int* list, int nv, double* r, double* rsum, hipStream_t s) {
-  hipMemsetAsync(rsum, 0, sizeof(double) * nv, s);
-  unsigned gx = (n_pad + 255) / 256;
-  if (gx > 64) gx = 64;
-  hipLaunchKernelGGL(k_residuals, dim3(gx, nv), dim3(256), 0, s, y, n, n_pad, arrays, list, r, rsum);
+  hipLaunchKernelGGL(k_residuals, dim3(1, nv), dim3(256), 0, s, y, n, n_pad, arrays, list, r, rsum);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

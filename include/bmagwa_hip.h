@@ -211,7 +211,7 @@ typedef struct {
   int32_t learn_freeze_iter;   /* learned: freeze at the first RB block at or
                                   after this iteration (0: end of burn-in) */
   int32_t scan_digits;         /* fixed-point digits of the RB scan (4..5) */
-  int32_t use_graph;           /* replay segments as a captured hipGraph */
+  int32_t use_graph;           /* after burn-in, replay segments as a captured hipGraph */
   uint64_t trace_capacity;     /* thinned loci entries kept per chain */
   int32_t learn_window;        /* learned: partners within this many SNPs */
   int32_t learn_steps;         /* learned: Adam steps per training round */
@@ -275,6 +275,12 @@ int bmg_batch_rb(bmg_batch* b, int chain, const double* y_hat, double* p_r);
 /* event timing summed over every bmg_batch_run since set-up: total ms, ms
  * in chain kernels, ms in RB blocks, number of scan launches */
 int bmg_batch_timing(bmg_batch* b, double out[4]);
+/* The steady-state graph (no counterpart in the reference; use_graph = 1):
+ * once every chain is past burn-in and every learner frozen, a segment (the
+ * chain kernel over n_rao iterations and the RB block after it) is captured
+ * into a hipGraph and replayed.  out: segments replayed, ms in replays (HIP
+ * events), graph builds. */
+int bmg_batch_graph_stats(bmg_batch* b, double out[3]);
 /* milliseconds one chain's workgroup spent, since set-up, in: tau2/alpha
  * refreshes, add/remove moves, swaps, state changes, learned swaps,
  * beta/sigma2 draws, PVE + thinning, whole iterations; then within term

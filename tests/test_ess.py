@@ -32,3 +32,37 @@ def test_constant_and_indicator_series():
     Mi = ess.indicator_matrix(sizes, loci, [5, 7, 9])
     assert Mi.tolist() == [[True, False, False, True], [True, True, False, False],
                            [False, False, False, True]]
+
+
+def _ar1(rng, rho, T, x0=0.0):
+    z, out = x0, np.zeros(T)
+    for t in range(T):
+        z = rho * z + rng.normal()
+        out[t] = z
+    return out
+
+
+def test_multichain_ess_closed_forms():
+    """Independent chains of an AR(1) series: ESS close to C T (1-rho)/(1+rho);
+    one chain: close to the single-series estimator; chains stuck in
+    different states: far below the sample count; nothing moves: 0."""
+    import pytest
+    rng = np.random.default_rng(7)
+    C, T = 8, 4000
+    for rho in (0.0, 0.5, 0.9):
+        X = np.stack([_ar1(rng, rho, T)[None, :] for _ in range(C)])
+        e = ess.ess_multichain(X)[0]
+        assert e / (C * T) == pytest.approx((1 - rho) / (1 + rho), rel=0.25)
+    x = _ar1(rng, 0.6, T)
+    assert ess.ess_multichain(x[None, None, :])[0] == pytest.approx(ess.ess_geyer(x[None, :])[0], rel=0.05)
+    # half the chains always include the SNP, half never: the pooled variance
+    # is all between-chain, the chains carry almost no information
+    stuck = np.zeros((C, 1, T))
+    stuck[: C // 2] = 1.0
+    stuck[:, :, :5] = 0.5           # a few early values so the within-chain variance is not 0
+    assert ess.ess_multichain(stuck)[0] < 0.01 * C * T
+    assert ess.ess_multichain(np.zeros((C, 1, T)))[0] == 0.0
+    assert ess.ess_multichain(np.ones((1, 1, T)))[0] == 0.0
+    # mixing indicator chains: ESS of the same order as C T
+    M = (rng.random((C, 1, T)) < 0.3).astype(float)
+    assert ess.ess_multichain(M)[0] == pytest.approx(C * T, rel=0.15)

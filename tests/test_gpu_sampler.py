@@ -93,3 +93,46 @@ def test_rb_scan_matches_oracle(bmg):
     pg = b.rb(0, yh)
     po = s.rb(yh)
     assert np.allclose(pg, po, rtol=1e-9, atol=1e-12)
+
+
+def test_uneven_datasets_and_steady_state_graph(bmg):
+    """Chains on three datasets with different sample sizes (the same SNP
+    panel, different individuals) in one batch: every chain reproduces its
+    own oracle run, and the steady-state hipGraph replay gives the same
+    trajectories, samples and RB PIPs as the eager run, bit for bit."""
+    p, n_iter, per_ds = 400, 1600, 2
+    sets = [dataset(n, p, (0.9, 0.9, 10, 30), 4, 40 + n) for n in (180, 333, 520)]
+    cfgs, chain_ctx = [], []
+    for d, (body, y, _) in enumerate(sets):
+        n = len(y)
+        for k in range(per_ds):
+            c = len(cfgs)
+            cfgs.append(synth.sampler_cfg(n, p, 4, seed=700 + c, chain=c, n_rao=100, n_rao_burnin=3,
+                                          thin=10, max_move_size=8, do_n_iter=n_iter))
+            chain_ctx.append(d)
+    out = {}
+    for g in (0, 1):
+        ctxs = [bmg.Ctx(len(y), p, body, y) for body, y, _ in sets]
+        b = bmg.Batch(ctxs, chain_ctx, cfgs, dict(trace_every_iter=1, use_graph=g))
+        b.setup()
+        b.run(700)                    # burn-in ends inside this call
+        b.run(n_iter - 700)           # whole segments: replayed when g = 1
+        st = b.graph_stats()
+        if g:
+            assert st["builds"] == 1 and st["replays"] >= 8, st
+        else:
+            assert st["replays"] == 0
+        out[g] = [(b.trace(c), b.thinned(c), b.p_rao(c)) for c in range(len(cfgs))]
+        b.close()
+    for c in range(len(cfgs)):
+        (t0, h0, r0), (t1, h1, r1) = out[0][c], out[1][c]
+        for key in t0:
+            assert np.array_equal(t0[key], t1[key]), (c, key)
+        for key in h0:
+            assert np.array_equal(h0[key], h1[key]), (c, key)
+        assert np.array_equal(r0, r1)
+    for c in (0, 3, 5):
+        body, y, _ = sets[chain_ctx[c]]
+        s = oracle_run(cfgs[c], body, y, n_iter)
+        compare(out[1][c][0], s.trace(), n_iter)
+        assert np.allclose(out[1][c][2], s.p_rao(), rtol=1e-8, atol=1e-10)
