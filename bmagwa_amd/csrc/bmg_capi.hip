@@ -922,8 +922,10 @@ static int steady_state(bmg_batch* b) {
   std::vector<ChainScal> cs(b->n_chains);
   BMG_HIP(hipStreamSynchronize(b->stream));
   BMG_HIP(hipMemcpy(cs.data(), b->d_scal, sizeof(ChainScal) * b->n_chains, hipMemcpyDeviceToHost));
+  // (capacity and tree flags only record rejected proposals; a stopped
+  // chain or a full trace buffer ends the graph path, as it ends the run)
   for (const ChainScal& c : cs)
-    if (c.n_rao_burnin > 0 || c.rb_pending || c.err) return 0;
+    if (c.n_rao_burnin > 0 || c.rb_pending || (c.err & (ERR_TRACE | ERR_STOP))) return 0;
   for (LearnDS* L : b->learn)
     if (L && !learn_frozen(L)) return 0;
   return 1;
