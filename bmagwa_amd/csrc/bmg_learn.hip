@@ -20,6 +20,7 @@
 namespace bmg {
 
 #define LEARN_RING (1u << 19)
+#define LEARN_MAX_ROWS (16 * 160)   // label rows of one RB group: chains x model indices
 #define LEARN_CHUNK (1 << 20)
 
 struct LearnView {
@@ -44,6 +45,8 @@ struct LearnDS {
   int32_t* ro = nullptr;
   float* rl = nullptr;
   unsigned long long* counter = nullptr;
+  uint32_t* lcnt = nullptr;               // labels per (chain of the group, model index)
+  unsigned long long* lbase = nullptr;    // their first ring positions
   MlpArgs mlp{};
   uint16_t* Xb = nullptr;
   float* lb = nullptr;
@@ -129,24 +132,23 @@ __device__ __forceinline__ double lbf(double rx, double css, double d, double s2
   return rx * rx / (2.0 * s2 * q) - 0.5 * log(q / d);
 }
 
-// one workgroup per (model index, chain of the group); one lane per offset
-__global__ void k_learn_labels(LearnView L, int type, const DevData* __restrict__ data,
-                               const ChainConst* __restrict__ consts, const ChainScal* __restrict__ scal,
-                               const ChainArrays* __restrict__ arrays, const int* __restrict__ list, int kmax,
-                               const double* __restrict__ xa, const double* __restrict__ rsum,
-                               uint32_t* __restrict__ rj, int32_t* __restrict__ ro, float* __restrict__ rl,
-                               unsigned long long* __restrict__ counter) {
-  const int v = blockIdx.y;
+// label of lane `lane` (an offset) for model index m of chain list[v]; false
+// when the pair gives no sample (partner outside the genome or in the model,
+// degenerate moments, non-finite label)
+__device__ __forceinline__ bool label_at(const LearnView& L, int type, const ChainScal* __restrict__ scal,
+                                         const ChainArrays* __restrict__ arrays, const int* __restrict__ list,
+                                         int kmax, const double* __restrict__ xa,
+                                         const double* __restrict__ rsum, int v, int m, int lane,
+                                         uint64_t* j_out, int* o_out, double* lab_out) {
   const int c = list[v];
   const ChainScal& CS = scal[c];
   const ChainArrays& A = arrays[c];
-  const int m = blockIdx.x;
-  if (m >= CS.nloci || (int)threadIdx.x >= 2 * L.H) return;
+  if (m >= CS.nloci || lane >= 2 * L.H) return false;
   const uint64_t j = (uint64_t)A.idx[m];
-  const int o = learn_off(L.H, threadIdx.x);
+  const int o = learn_off(L.H, lane);
   const int64_t k = (int64_t)j + o;
-  if (k < 0 || k >= (int64_t)L.p) return;
-  if (A.model_inds[k] >= 0) return;
+  if (k < 0 || k >= (int64_t)L.p) return false;
+  if (A.model_inds[k] >= 0) return false;
   const double n = (double)L.n;
   const double mr = rsum[v] / n;
   const double sj = L.pre[j * L.off + L.ot], cj = L.pre[j * L.off + L.ot + 1];
@@ -159,11 +161,53 @@ __global__ void k_learn_labels(LearnView L, int type, const DevData* __restrict_
   const double rxk = xa[(size_t)v * L.p + (uint64_t)k] - sk * mr + bj * cov;
   const double d = CS.inv_tau2_alpha2[type];
   const double s2 = CS.sigma2;
-  if (!(cj > 0.0) || !(ck > 0.0) || !(d > 0.0) || !(s2 > 0.0)) return;
+  if (!(cj > 0.0) || !(ck > 0.0) || !(d > 0.0) || !(s2 > 0.0)) return false;
   double lab = lbf(rxk, ck, d, s2) - lbf(rxj, cj, d, s2);
-  if (!isfinite(lab)) return;
-  lab = fmin(30.0, fmax(-30.0, lab));
-  unsigned long long slot = atomicAdd(counter, 1ull) % LEARN_RING;
+  if (!isfinite(lab)) return false;
+  *j_out = j;
+  *o_out = o;
+  *lab_out = fmin(30.0, fmax(-30.0, lab));
+  return true;
+}
+
+// Labels go to the ring in a fixed order -- chain of the group, model index,
+// offset -- so the ring, and the network trained from it, do not depend on
+// the order in which workgroups run: a count pass, one ordered prefix sum,
+// then a write pass.  One workgroup (one wave) per (model index, chain of
+// the group); one lane per offset.
+__global__ void k_learn_count(LearnView L, int type, const ChainScal* __restrict__ scal,
+                              const ChainArrays* __restrict__ arrays, const int* __restrict__ list, int kmax,
+                              const double* __restrict__ xa, const double* __restrict__ rsum,
+                              uint32_t* __restrict__ cnt) {
+  const int v = blockIdx.y, m = blockIdx.x;
+  uint64_t j; int o; double lab;
+  const bool ok = label_at(L, type, scal, arrays, list, kmax, xa, rsum, v, m, threadIdx.x, &j, &o, &lab);
+  const unsigned long long mask = __ballot(ok);
+  if (threadIdx.x == 0) cnt[v * kmax + m] = (uint32_t)__popcll(mask);
+}
+
+// base[i] = ring position of row i's first label; the counter advances by
+// the total
+__global__ void k_learn_base(const uint32_t* __restrict__ cnt, int rows, unsigned long long* __restrict__ base,
+                             unsigned long long* __restrict__ counter) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  unsigned long long at = *counter;
+  for (int i = 0; i < rows; ++i) { base[i] = at; at += cnt[i]; }
+  *counter = at;
+}
+
+__global__ void k_learn_write(LearnView L, int type, const ChainScal* __restrict__ scal,
+                              const ChainArrays* __restrict__ arrays, const int* __restrict__ list, int kmax,
+                              const double* __restrict__ xa, const double* __restrict__ rsum,
+                              const unsigned long long* __restrict__ base, uint32_t* __restrict__ rj,
+                              int32_t* __restrict__ ro, float* __restrict__ rl) {
+  const int v = blockIdx.y, m = blockIdx.x;
+  uint64_t j; int o; double lab;
+  const bool ok = label_at(L, type, scal, arrays, list, kmax, xa, rsum, v, m, threadIdx.x, &j, &o, &lab);
+  const unsigned long long mask = __ballot(ok);
+  if (!ok) return;
+  const unsigned long long below = threadIdx.x ? (mask & ((1ull << threadIdx.x) - 1ull)) : 0ull;
+  const unsigned long long slot = (base[v * kmax + m] + (unsigned long long)__popcll(below)) % LEARN_RING;
   rj[slot] = (uint32_t)j;
   ro[slot] = o;
   rl[slot] = (float)lab;
@@ -260,6 +304,8 @@ int learn_create(LearnDS** out, bmg_ctx* ctx, const DevData& D, const LearnCfg& 
       hipMalloc(&L->ro, sizeof(int32_t) * LEARN_RING) != hipSuccess ||
       hipMalloc(&L->rl, sizeof(float) * LEARN_RING) != hipSuccess ||
       hipMalloc(&L->counter, sizeof(unsigned long long)) != hipSuccess ||
+      hipMalloc(&L->lcnt, sizeof(uint32_t) * LEARN_MAX_ROWS) != hipSuccess ||
+      hipMalloc(&L->lbase, sizeof(unsigned long long) * LEARN_MAX_ROWS) != hipSuccess ||
       hipMalloc(&L->Xb, sizeof(uint16_t) * 16 * cfg.batch) != hipSuccess ||
       hipMalloc(&L->lb, sizeof(float) * cfg.batch) != hipSuccess ||
       hipMalloc(&L->Xc, sizeof(uint16_t) * 16 * LEARN_CHUNK) != hipSuccess ||
@@ -315,7 +361,7 @@ int learn_create(LearnDS** out, bmg_ctx* ctx, const DevData& D, const LearnCfg& 
 void learn_destroy(LearnDS* L) {
   if (!L) return;
   hipFree(L->band); hipFree(L->z); hipFree(L->pbar); hipFree(L->rj); hipFree(L->ro); hipFree(L->rl);
-  hipFree(L->counter); hipFree(L->Xb); hipFree(L->lb); hipFree(L->Xc); hipFree(L->oc); hipFree(L->table);
+  hipFree(L->counter); hipFree(L->lcnt); hipFree(L->lbase); hipFree(L->Xb); hipFree(L->lb); hipFree(L->Xc); hipFree(L->oc); hipFree(L->table);
   hipFree(L->d_chains);
   mlp_free(&L->mlp);
   delete L;
@@ -330,9 +376,14 @@ uint64_t learn_rounds(const LearnDS* L) { return L->rounds; }
 int learn_labels(LearnDS* L, const DevData* d_data, const ChainConst* consts, const ChainScal* scal,
                  const ChainArrays* arrays, const int* d_list, int nv, int kmax, const double* xa,
                  const double* rsum, hipStream_t s) {
+  (void)d_data; (void)consts;
   if (L->frozen || nv <= 0) return BMG_OK;
-  hipLaunchKernelGGL(k_learn_labels, dim3(kmax, nv), dim3(64), 0, s, L->v, L->type, d_data, consts, scal, arrays,
-                     d_list, kmax, xa, rsum, L->rj, L->ro, L->rl, L->counter);
+  if (nv * kmax > LEARN_MAX_ROWS) { set_error("learned proposal: label rows exceed the buffer"); return BMG_E_ARG; }
+  hipLaunchKernelGGL(k_learn_count, dim3(kmax, nv), dim3(64), 0, s, L->v, L->type, scal, arrays, d_list, kmax, xa,
+                     rsum, L->lcnt);
+  hipLaunchKernelGGL(k_learn_base, dim3(1), dim3(64), 0, s, L->lcnt, nv * kmax, L->lbase, L->counter);
+  hipLaunchKernelGGL(k_learn_write, dim3(kmax, nv), dim3(64), 0, s, L->v, L->type, scal, arrays, d_list, kmax, xa,
+                     rsum, L->lbase, L->rj, L->ro, L->rl);
   BMG_HIP(hipGetLastError());
   return BMG_OK;
 }

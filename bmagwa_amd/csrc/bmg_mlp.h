@@ -21,8 +21,11 @@ namespace bmg {
 // moments; G: fp32 gradient accumulator (cleared by the Adam kernel);
 // bf16 copies laid out so that every MFMA operand is a 16-byte load with
 // the reduction index contiguous.
+#define MLP_GP_WG 64         // workgroups per training launch (partial gradients kept)
+
 struct MlpArgs {
   float* P; float* G; float* M; float* V;
+  float* GP;       // [MLP_GP_WG][MLP_NPAR] per-workgroup gradients of one launch
   uint16_t* W1t;   // [256 h][16 k]
   uint16_t* W2t;   // [256 g][256 h]  (forward: n = g, k = h)
   uint16_t* W2b;   // [256 h][256 g]  (backward: n = h, k = g)
@@ -35,8 +38,10 @@ void mlp_free(MlpArgs* m);
 int mlp_launch_copies(const MlpArgs& m, hipStream_t s);
 // out[r] = network(X[r]) for X: rows x 16 bf16
 int mlp_launch_forward(const MlpArgs& m, const uint16_t* X, int64_t rows, float* out, hipStream_t s);
-// G += gradient of the mean Huber loss over `rows` (labels: fp32);
-// out (optional) receives the forward outputs
+// G += gradient of the mean Huber loss over `rows` (labels: fp32), summed
+// in a fixed order (workgroup partials, then an ordered reduction), so the
+// result does not depend on scheduling; out (optional) receives the forward
+// outputs
 int mlp_launch_train(const MlpArgs& m, const uint16_t* X, const float* labels, int64_t rows,
                      float* out, hipStream_t s);
 // one Adam step (t counts from 1) from G, which it clears; bf16 copies follow

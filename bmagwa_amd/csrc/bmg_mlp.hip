@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "bmg_internal.h"
+#include <algorithm>
 #include "bmg_mlp.h"
 
 namespace bmg {
@@ -181,12 +182,14 @@ __global__ __launch_bounds__(256) void k_mlp(const uint16_t* __restrict__ X, con
   }
   if (!TRAIN) return;
   __syncthreads();
+  // this workgroup's gradient: every parameter is written exactly once below
+  float* gp = m.GP + (size_t)(blockIdx.x % MLP_GP_WG) * MLP_NPAR;
 
   // ---- backward through layer 3: dw3, db2, dz2 = g w3 [z2 > 0]
   if (t == 0) {
     float s = 0.0f;
     for (int i = 0; i < 64; ++i) s += gs[i];
-    atomicAdd(m.G + MLP_OFF_B3, s);
+    gp[MLP_OFF_B3] = s;
   }
 #pragma unroll
   for (int nt = 0; nt < 4; ++nt) {
@@ -210,8 +213,8 @@ __global__ __launch_bounds__(256) void k_mlp(const uint16_t* __restrict__ X, con
     pw3 = sum4g(pw3);
     pb2 = sum4g(pb2);
     if (grp == 0) {
-      atomicAdd(m.G + MLP_OFF_W3 + col, pw3);
-      atomicAdd(m.G + MLP_OFF_B2 + col, pb2);
+      gp[MLP_OFF_W3 + col] = pw3;
+      gp[MLP_OFF_B2 + col] = pb2;
     }
   }
   __syncthreads();
@@ -244,7 +247,7 @@ __global__ __launch_bounds__(256) void k_mlp(const uint16_t* __restrict__ X, con
         for (int r = 0; r < 4; ++r) {
           int h = 64 * w + 16 * mt + 4 * grp + r;
           int g = 128 * half + 16 * nt + lr;
-          atomicAdd(m.G + MLP_OFF_W2 + h * MLP_HID + g, a2[mt][nt][r]);
+          gp[MLP_OFF_W2 + h * MLP_HID + g] = a2[mt][nt][r];
         }
   }
   __syncthreads();   // A1T is reused for dz1 below
@@ -282,7 +285,7 @@ __global__ __launch_bounds__(256) void k_mlp(const uint16_t* __restrict__ X, con
           A1T[h * 64 + row] = f2bf(dz1);
         }
       pb1 = sum4g(pb1);
-      if (grp == 0) atomicAdd(m.G + MLP_OFF_B1 + h, pb1);
+      if (grp == 0) gp[MLP_OFF_B1 + h] = pb1;
     }
   }
   __syncthreads();
@@ -301,7 +304,7 @@ __global__ __launch_bounds__(256) void k_mlp(const uint16_t* __restrict__ X, con
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       int k = 4 * grp + r;
-      atomicAdd(m.G + MLP_OFF_W1 + k * MLP_HID + (64 * w + 16 * nt) + lr, a4[r]);
+      gp[MLP_OFF_W1 + k * MLP_HID + (64 * w + 16 * nt) + lr] = a4[r];
     }
     (void)h;
   }
@@ -320,6 +323,15 @@ __device__ __forceinline__ void write_copy(const MlpArgs& m, int i, float p) {
   } else if (i >= MLP_OFF_W3 && i < MLP_OFF_B3) {
     m.w3b[i - MLP_OFF_W3] = b;
   }
+}
+
+// G[i] += sum over the launch's workgroups of their partials, in order
+__global__ void k_mlp_reduce(MlpArgs m, int nwg) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= MLP_NPAR) return;
+  float acc = 0.0f;
+  for (int b = 0; b < nwg; ++b) acc += m.GP[(size_t)b * MLP_NPAR + i];
+  m.G[i] += acc;
 }
 
 __global__ void k_mlp_copies(MlpArgs m) {
@@ -362,6 +374,7 @@ int mlp_alloc(MlpArgs* m) {
   BMG_HIP(hipMalloc(&m->G, sizeof(float) * MLP_NPAR));
   BMG_HIP(hipMalloc(&m->M, sizeof(float) * MLP_NPAR));
   BMG_HIP(hipMalloc(&m->V, sizeof(float) * MLP_NPAR));
+  BMG_HIP(hipMalloc(&m->GP, sizeof(float) * MLP_NPAR * MLP_GP_WG));
   BMG_HIP(hipMalloc(&m->W1t, sizeof(uint16_t) * MLP_HID * MLP_IN));
   BMG_HIP(hipMalloc(&m->W2t, sizeof(uint16_t) * MLP_HID * MLP_HID));
   BMG_HIP(hipMalloc(&m->W2b, sizeof(uint16_t) * MLP_HID * MLP_HID));
@@ -375,7 +388,7 @@ int mlp_alloc(MlpArgs* m) {
 }
 
 void mlp_free(MlpArgs* m) {
-  hipFree(m->P); hipFree(m->G); hipFree(m->M); hipFree(m->V);
+  hipFree(m->P); hipFree(m->G); hipFree(m->M); hipFree(m->V); hipFree(m->GP);
   hipFree(m->W1t); hipFree(m->W2t); hipFree(m->W2b); hipFree(m->w3b);
   *m = MlpArgs{};
 }
@@ -398,9 +411,17 @@ int mlp_launch_forward(const MlpArgs& m, const uint16_t* X, int64_t rows, float*
 int mlp_launch_train(const MlpArgs& m, const uint16_t* X, const float* labels, int64_t rows, float* out,
                      hipStream_t s) {
   if (rows <= 0) return BMG_OK;
-  unsigned grid = (unsigned)((rows + MLP_ROWS - 1) / MLP_ROWS);
-  hipLaunchKernelGGL(k_mlp<true>, dim3(grid), dim3(256), SM_TRAIN, s, X, labels, rows, 1.0f / (float)rows, m,
-                     out);
+  // launches of at most MLP_GP_WG workgroups, each reduced in order into G;
+  // every launch scales by the whole batch, so G is the mean's gradient
+  const float inv_b = 1.0f / (float)rows;
+  const int64_t chunk = (int64_t)MLP_GP_WG * MLP_ROWS;
+  for (int64_t r0 = 0; r0 < rows; r0 += chunk) {
+    const int64_t nr = std::min<int64_t>(chunk, rows - r0);
+    const unsigned grid = (unsigned)((nr + MLP_ROWS - 1) / MLP_ROWS);
+    hipLaunchKernelGGL(k_mlp<true>, dim3(grid), dim3(256), SM_TRAIN, s, X + r0 * MLP_IN, labels + r0, nr, inv_b, m,
+                       out ? out + r0 : nullptr);
+    hipLaunchKernelGGL(k_mlp_reduce, dim3((MLP_NPAR + 255) / 256), dim3(256), 0, s, m, (int)grid);
+  }
   BMG_HIP(hipGetLastError());
   return BMG_OK;
 }

@@ -108,3 +108,35 @@ def test_burn_in_trains_freezes_and_keeps_posterior(bmg):
             assert (learned & (tr["jd"] > 0)).sum() > 0
     diff = np.abs(runs[0][1] - runs[1][1])
     assert diff.max() < 0.1, diff.max()
+
+
+def test_learned_run_is_reproducible(bmg):
+    """Two learned batches with the same seeds train the same network and run
+    the same chains: the label ring is filled in a fixed order and the
+    network's gradients are summed in a fixed order, so the ring, the
+    parameters, the frozen table and every trajectory are identical."""
+    n, p, n_iter = 300, 900, 3000
+    body, y, _ = dataset(n, p, (0.9, 0.95, 10, 40), 6, 31)
+    outs = []
+    for rep in range(2):
+        cfgs = [synth.sampler_cfg(n, p, 6, seed=40 + c, chain=c, n_rao=100, n_rao_burnin=6, thin=10,
+                                  max_move_size=10, do_n_iter=n_iter) for c in range(3)]
+        ctx = bmg.Ctx(n, p, body, y)
+        b = bmg.Batch([ctx], [0] * 3, cfgs, dict(LCFG, use_graph=1))
+        b.setup()
+        b.run(n_iter)
+        st = b.learn_state(0)
+        assert st["frozen"] and st["rounds"] >= 2
+        outs.append(dict(ring=b.learn_samples(0), params=b.learn_params(0), table=b.learn_table(0),
+                         traces=[b.trace(c) for c in range(3)], p_rao=[b.p_rao(c) for c in range(3)]))
+        b.close()
+    a, c = outs
+    for x, z in zip(a["ring"], c["ring"]):
+        assert np.array_equal(np.asarray(x), np.asarray(z))
+    assert np.array_equal(a["params"], c["params"])
+    assert np.array_equal(a["table"], c["table"])
+    for ta, tc in zip(a["traces"], c["traces"]):
+        for key in ta:
+            assert np.array_equal(ta[key], tc[key]), key
+    for ra, rc in zip(a["p_rao"], c["p_rao"]):
+        assert np.array_equal(ra, rc)
