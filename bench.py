@@ -60,6 +60,13 @@ N, P, N_CAUSAL, H2 = 10000, 500000, 100, 0.5
 MAF, LD = (0.02, 0.5), (0.7, 0.95, 20, 100)
 SEED = 20261202 + 4000
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the tuned scan at this workload, from rocprofv3
+# FETCH_SIZE / WRITE_SIZE passes scaled by a copy kernel of known traffic
+# (profiles/kernels_scan_band_copy_n10000_p500000.md): fetch 2 x 626,386.4
+# KiB, write 31,264 KiB
+SCAN_TRAFFIC = dict(n=10000, p=500000, nvec=3, digits=5,
+                    fetch_bytes=int(2 * 626386.4 * 1024), write_bytes=31264 * 1024,
+                    source="profiles/kernels_scan_band_copy_n10000_p500000.md")
 INT8_PEAK_TOPS = 5000.0
 
 
@@ -352,8 +359,12 @@ def main():
     extra = {}
     if not args.no_extra:
         sc = scan_roofline(bmg, ctx)
+        t = SCAN_TRAFFIC
+        traffic = (t["fetch_bytes"] + t["write_bytes"]
+                   if (t["n"], t["p"], t["nvec"], t["digits"]) == (N, P, sc["nvec"], sc["digits"]) else None)
         roof = dict(bound="hbm", achieved=round(sc["gbs"], 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(sc["gbs"] / HBM_PEAK_GBS, 4), traffic=None,
+                    frac=round(sc["gbs"] / HBM_PEAK_GBS, 4), traffic=traffic,
+                    traffic_source=t["source"] if traffic else None,
                     kernel="k_scan_mfma<%d> (3 vectors x 5 digit planes)" % sc["shape"]["tiles_per_wave"],
                     autotuned_shape=sc["shape"],
                     ms_per_launch=round(sc["ms"], 4), algorithmic_bytes=sc["bytes"])

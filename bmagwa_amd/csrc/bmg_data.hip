@@ -660,6 +660,7 @@ static int scan_tuned_shape(bmg_ctx* c, int nvec, int coding, int digits, hipStr
   for (int waves : {4, 8, 16})
     for (int nt : {2, 4})
       for (int ti = 0; ti < 2; ++ti) {
+        if (nt == 4 && waves == 16) continue;  // spills at the 128-VGPR budget of 1,024 lanes
         ScanShape probe = make_shape(c, NC, nt, waves, tws[ti], 1);
         if (ti == 1 && probe.tile_words == make_shape(c, NC, nt, waves, tws[0], 1).tile_words) continue;
         for (unsigned per_cu : {1u, 2u, 4u, 8u})
