@@ -16,7 +16,7 @@ before the timed steps):
   plus the learned swap (network trained online during burn-in);
 * batched:         `--chains` chains with the plain moves only;
 * plain:           1 chain, plain moves (the reference's sampler on the GPU),
-                   `--plain-steps` timed steps (80: long enough for its own ESS);
+                   `--plain-steps` timed steps (60: long enough for its own ESS);
 * batched_full:    `--full-chains` chains (256: one per CU) with the learned
   swap, to show the whole GPU (reported beside the headline).
 
@@ -231,7 +231,7 @@ def cpu_baseline(ctx, y, monitored, seconds, thin):
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--iters-per-step", type=int, default=1000)
     ap.add_argument("--chains", type=int, default=16)
@@ -248,10 +248,10 @@ def main():
     ap.add_argument("--thin", type=int, default=10,
                     help="thinning (part of the sampler's schedule: beta and sigma2 are drawn at "
                          "every thinned iteration, as in the reference)")
-    ap.add_argument("--plain-steps", type=int, default=80,
+    ap.add_argument("--plain-steps", type=int, default=60,
                     help="timed steps of the 1-chain plain run (long enough for its own ESS)")
-    ap.add_argument("--full-steps", type=int, default=0,
-                    help="timed steps of the --full-chains run (default: --steps)")
+    ap.add_argument("--full-steps", type=int, default=10,
+                    help="timed steps of the --full-chains run")
     ap.add_argument("--pip-ref-chains", type=int, default=256,
                     help="chains of the long plain-moves reference run for the PIP check (0: skip)")
     ap.add_argument("--pip-ref-steps", type=int, default=30,
@@ -295,7 +295,7 @@ def main():
         runs["plain"] = run_batch(bmg, synth, ctx, 1, args.plain_steps, args.warmup, ips, 100,
                                   dict(proposal=0), th)
     if args.full_chains > 0:
-        runs["batched_full"] = run_batch(bmg, synth, ctx, args.full_chains, args.full_steps or args.steps,
+        runs["batched_full"] = run_batch(bmg, synth, ctx, args.full_chains, args.full_steps,
                                          args.warmup, ips,
                                          100, dict(proposal=1 if args.learned else 0, trace_every_iter=1),
                                          th)

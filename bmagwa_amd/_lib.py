@@ -61,7 +61,7 @@ class BatchCfg(C.Structure):
 
 BATCH_CFG_DEFAULT = dict(trace_every_iter=0, proposal=0, learn_every=500, learn_freeze_iter=0,
                          scan_digits=5, use_graph=0, trace_capacity=0, learn_window=20,
-                         learn_steps=100, learn_batch=4096, learn_pad=0, p_learned=0.3,
+                         learn_steps=100, learn_batch=4096, learn_pad=0, p_learned=0.9,
                          learn_rate=1e-3)
 
 OPT_DEFAULTS = dict(max_move_size=20, max_SNP_neighborhood_size=10, p_move_size=0.2,

@@ -838,7 +838,7 @@ static int batch_create(bmg_ctx** ctxs, int n_ctx, int n_chains, const int32_t* 
     if (bc.learn_steps <= 0) bc.learn_steps = 100;
     if (bc.learn_batch <= 0) bc.learn_batch = 4096;
     if (!(bc.learn_rate > 0)) bc.learn_rate = 1e-3;
-    if (!(bc.p_learned > 0)) bc.p_learned = 0.3;
+    if (!(bc.p_learned > 0)) bc.p_learned = 0.9;
     if (bc.learn_window > BMG_LH_MAX || bc.p_learned > 1.0 || bc.learn_batch > (1 << 20)) {
       bmg_batch_destroy(b); set_error("learned proposal: window <= 32, p_learned <= 1, batch <= 2^20");
       return BMG_E_ARG;

@@ -1612,16 +1612,17 @@ __device__ __forceinline__ void run_iterations(Ctx<KMAX>& C, ChainScal& CS, uint
       uint64_t t1 = wall_clock64(); pr[0] += t1 - t0; t0 = t1;
     }
     // PMV move mix with one effect type (src/sampler.cpp:468-476); with a
-    // learned table, a learned swap first with probability p_learned
+    // learned table, a drawn swap is a learned swap with probability
+    // p_learned (both swaps keep the posterior; the choice does not depend
+    // on the state, and add/remove and state-change moves keep their share)
     int move;
-    if (R.learn_on && rng_u01(R.rng) < K.p_learned) {
-      move = 3;
-    } else {
+    {
       const double pm[3] = {0.7, 0.7 + 0.15, 0.7 + 0.15 + 0.15};
       double u = rng_u01(R.rng);
       double r = u * pm[2];
       move = r < pm[0] ? 0 : (r < pm[1] ? 1 : 2);
     }
+    if (move == 1 && R.learn_on && rng_u01(R.rng) < K.p_learned) move = 3;
     int jd = 0;
     if (move == 3) {
       if (R.c_nloci > 0) jd = move_learned_swap(C);

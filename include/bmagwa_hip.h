@@ -217,7 +217,7 @@ typedef struct {
   int32_t learn_steps;         /* learned: Adam steps per training round */
   int32_t learn_batch;         /* learned: rows per Adam step */
   int32_t learn_pad;
-  double  p_learned;           /* learned: probability of a learned swap */
+  double  p_learned;           /* learned: share of the drawn swaps that are learned */
   double  learn_rate;          /* learned: Adam learning rate */
 } bmg_batch_cfg;
 
@@ -233,10 +233,11 @@ int bmg_batch_sync(bmg_batch* b);
 /* Learned swap (no counterpart in the reference): frozen partner weights
  * w[p][2H] for the chains of dataset ds (column c <-> SNP offset
  * c < H ? c - H : c - H + 1; negative or non-finite entries count as 0).
- * Each iteration first takes a learned swap with probability p_learned:
- * remove a uniformly chosen model SNP j, add k ~ w[j][.] over the SNPs
- * within H of j outside the model, accept with the exact Metropolis-Hastings
- * ratio.  w = NULL switches it off.  1 <= H <= 32. */
+ * A swap drawn by the plain move draw (probability 0.15) becomes a learned
+ * swap with probability p_learned: remove a uniformly chosen model SNP j,
+ * add k ~ w[j][.] over the SNPs within H of j outside the model, accept
+ * with the exact Metropolis-Hastings ratio; otherwise the plain swap runs.
+ * w = NULL switches it off.  1 <= H <= 32. */
 int bmg_batch_set_learned(bmg_batch* b, int ds, const float* w, int H, double p_learned);
 /* The online learned proposal (proposal = 1): state of dataset ds's learner
  * (out: training rounds, frozen flag, labels collected, window H), its

@@ -141,8 +141,8 @@ int    orc_sampler_sample(orc_sampler* s, uint64_t n_iter);
 /* run only the start-of-sample() setup (src/sampler.cpp:591-620) */
 int    orc_sampler_setup(orc_sampler* s);
 /* learned swap: frozen partner weights w[p][2H] (column c <-> offset
- * c < H ? c - H : c - H + 1), taken with probability p_learned per
- * iteration before the plain move draw; w = NULL switches it off */
+ * c < H ? c - H : c - H + 1); a swap drawn by the plain move draw becomes a
+ * learned swap with probability p_learned; w = NULL switches it off */
 int    orc_sampler_set_learned(orc_sampler* s, const float* w, int H, double p_learned);
 /* test knob: probabilities of the add/remove, swap and state-change moves
  * (normally 0.7 / 0.15 / 0.15, src/sampler.cpp:468-476); call after setup */

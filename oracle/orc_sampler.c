@@ -1409,9 +1409,11 @@ int orc_sampler_iterate(orc_sampler* s, uint64_t n_iter)
         memcpy(s->pr.inv_tau2_alpha2, ita, sizeof(ita));
       }
     }
-    int move;
-    if (s->learn_on && orc_rng_u01(&s->rng) < s->p_learned) move = 3;
-    else move = sample_discrete_naive(s->p_moves_cumsum, NMOVES, &s->rng);
+    /* with a learned table, a drawn swap is a learned swap with probability
+     * p_learned (a state-independent choice between two kernels that each
+     * keep the posterior) */
+    int move = sample_discrete_naive(s->p_moves_cumsum, NMOVES, &s->rng);
+    if (move == 1 && s->learn_on && orc_rng_u01(&s->rng) < s->p_learned) move = 3;
     int jd = 0;
     if (!isfinite(s->cur->loglik)) return -2;
     switch (move) {

@@ -35,13 +35,16 @@ def freq_after_burn(s, burn):
     return counts / kept
 
 
-@pytest.mark.parametrize("H,p_learned,seed", [(3, 0.5, 11), (6, 0.9, 12)])
-def test_learned_swap_pips_match_exact_enumeration(H, p_learned, seed):
+@pytest.mark.parametrize("H,seed", [(3, 11), (6, 12)])
+def test_learned_swap_pips_match_exact_enumeration(H, seed):
+    # every drawn swap is a learned swap (the plain swap is left out: its
+    # own exact-enumeration error would hide the learned move's)
+    p_learned = 1.0
     s, d = small_sampler(dict(n_sample_tau2_and_missing=10 ** 9, n_rao=50, n_rao_burnin=100,
                               thin=1, delay_rejection=0, max_move_size=4, seed=seed))
     s.init_flat()
     s.setup()
-    assert s.set_move_mix([1.0, 0.0, 0.0]) == 0
+    assert s.set_move_mix([0.5, 0.5, 0.0]) == 0
     assert s.set_learned(skewed_weights(7, H, seed), H, p_learned) == 0
     s.iterate(400000)
     exact = exact_pips(s, d, s.prior()["inv_tau2_alpha2_A"])
@@ -49,7 +52,7 @@ def test_learned_swap_pips_match_exact_enumeration(H, p_learned, seed):
     assert np.max(np.abs(freq - exact)) < 0.02, (freq, exact)
     tr = s.trace()
     n_learned = int((tr["move"] == 3).sum())
-    assert n_learned > 0.8 * p_learned * len(tr["move"])
+    assert n_learned > 0.8 * 0.5 * p_learned * len(tr["move"])
     # learned swaps were accepted (the move is live, not a no-op)
     assert int(((tr["move"] == 3) & (tr["jd"] > 0)).sum()) > 1000
 
