@@ -407,6 +407,8 @@ struct bmg_batch {
   // online learned proposal per dataset (proposal = 1)
   std::vector<LearnDS*> learn;
   std::vector<int> learn_on;
+  // x_j . y tables per dataset and coding (set-up)
+  std::vector<double*> xy_dev;
 };
 
 struct bmg_model {
@@ -652,6 +654,7 @@ extern "C" int bmg_batch_destroy(bmg_batch* b) {
   if (b->graph_exec) hipGraphExecDestroy(b->graph_exec);
   if (b->graph) hipGraphDestroy(b->graph);
   hipFree(b->d_glist);
+  for (double* x : b->xy_dev) hipFree(x);
   if (b->stream) hipStreamDestroy(b->stream);
   delete b;
   return BMG_OK;
@@ -708,6 +711,7 @@ static int batch_create(bmg_ctx** ctxs, int n_ctx, int n_chains, const int32_t* 
   }
   for (int i = 0; i < n_ctx; ++i) {
     b->h_data.push_back(make_devdata(ctxs[i]));
+    b->h_data.back().mixed = model_api ? 1 : 0;
     if (b->h_data.back().lds_lev_size > 512) { delete b; set_error("p too large for the sum-tree cache"); return BMG_E_UNSUPPORTED; }
   }
   b->chain_ds.assign(chain_ctx, chain_ctx + n_chains);
@@ -903,6 +907,22 @@ extern "C" int bmg_batch_setup(bmg_batch* b) {
   if (!b) { set_error("null batch"); return BMG_E_ARG; }
   if (b->setup_done) { set_error("setup already done"); return BMG_E_STATE; }
   hipSetDevice(b->device);
+  // x_j . y of every SNP for each coding the dataset's chains add with (y is
+  // fixed, so each add reads one double instead of the whole of y)
+  for (size_t ds = 0; ds < b->h_data.size(); ++ds) {
+    DevData& D = b->h_data[ds];
+    const int codings[2] = {D.type == BMG_AH ? 0 : D.type, D.type == BMG_AH ? 1 : -1};
+    for (int cd : codings) {
+      if (cd < 0 || D.xy[cd]) continue;
+      double* t = nullptr;
+      BMG_HIP(hipMalloc(&t, sizeof(double) * D.p));
+      b->xy_dev.push_back(t);
+      if (launch_xy_all(D, cd, t, b->stream)) { set_error("x.y table launch failed"); return BMG_E_HIP; }
+      D.xy[cd] = t;
+    }
+  }
+  BMG_HIP(hipMemcpyAsync(b->d_data, b->h_data.data(), sizeof(DevData) * b->h_data.size(), hipMemcpyHostToDevice,
+                         b->stream));
   int rc = launch_chain_kernel(b, 0, 1);
   if (rc) return rc;
   BMG_HIP(hipStreamSynchronize(b->stream));

@@ -28,6 +28,8 @@ struct DevData {
   uint64_t lds_lev_size; // doubles of levels >= 2
   const float* lw;       // learned-swap weights p x 2*lH (null: none)
   int lH;                // learned-swap window (SNPs each side, <= 32)
+  const double* xy[4];   // x_j . y of every SNP per coding (null: computed per add)
+  int mixed;             // columns may carry codings other than `type` (Model API)
 };
 
 // Per-chain constants (options + solved prior, src/sampler.hpp:107-203,
@@ -131,5 +133,10 @@ struct ModelOpArgs {
 };
 
 namespace bmg {
+
+// out[j] = x_j . y for every SNP j of D under `coding`, summed in exactly
+// the order the chain kernel's per-add dot uses, so the chain reads the
+// same double it would have computed
+int launch_xy_all(const DevData& D, int coding, double* out, hipStream_t s);
 
 }  // namespace bmg

@@ -14,8 +14,11 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 
+#include <algorithm>
+
 #include "bmg_chain.h"
 #include "bmg_device.h"
+#include "../../include/bmagwa_hip.h"
 
 using namespace bmg;
 
@@ -74,6 +77,7 @@ struct Smem {
   double tlev[2][TLEV_LDS];              // sum-tree levels >= 2 (add, rem)
   int32_t win[512];                      // window of the current model_inds
   uint64_t slots_cur[(KMAX + 63) / 64], slots_prop[(KMAX + 63) / 64];
+  int32_t iacc[KMAX + 1];                // integer column dots of new_column_dots
   double red[8];
   long long redi[8];
   double bcd[4];
@@ -136,11 +140,11 @@ __device__ __forceinline__ int coding_of_type(int type, int second) {
 }
 
 // x_j^T y (FP64) for one SNP/coding
-template <int KMAX>
-__device__ __forceinline__ double snp_dot_y(Ctx<KMAX>& C, uint64_t snp, int coding) {
-  const uint32_t W = C.D.W32;
-  const uint32_t* __restrict__ w = C.D.g + snp * W;
-  const double* __restrict__ y = C.D.y;
+// x . y of one SNP's packed row: lane-strided words, 16 fields each in
+// order, then the workgroup sum (the order both the per-add dot and the
+// set-up table use)
+__device__ __forceinline__ double dot_y_block(const uint32_t* __restrict__ w, uint32_t W,
+                                              const double* __restrict__ y, int coding, double* red) {
   double s = 0.0;
   for (uint32_t k = threadIdx.x; k < W; k += BMG_BLOCK) {
     uint32_t x = w[k];
@@ -151,68 +155,129 @@ __device__ __forceinline__ double snp_dot_y(Ctx<KMAX>& C, uint64_t snp, int codi
       if (iv) s += (double)iv * yy[f];
     }
   }
-  return block_sum(s, C.S.red);
+  return block_sum(s, red);
+}
+
+__global__ __launch_bounds__(BMG_BLOCK) void k_xy_all(DevData D, int coding, double* __restrict__ out) {
+  __shared__ double red[8];
+  for (uint64_t j = blockIdx.x; j < D.p; j += gridDim.x) {
+    const double v = dot_y_block(D.g + j * D.W32, D.W32, D.y, coding, red);
+    if (threadIdx.x == 0) out[j] = v;
+  }
+}
+
+template <int KMAX>
+__device__ __forceinline__ double snp_dot_y(Ctx<KMAX>& C, uint64_t snp, int coding) {
+  const double* pre = C.D.xy[coding];
+  if (pre) return pre[snp];   // precomputed at set-up in this same order
+  return dot_y_block(C.D.g + snp * C.D.W32, C.D.W32, C.D.y, coding, C.S.red);
+}
+
+// value planes of a word under a coding known at compile time (CODING >= 0)
+// or given at run time (CODING < 0)
+template <int CODING>
+__device__ __forceinline__ void planes_c(uint32_t w, int coding, uint32_t& p1, uint32_t& p2) {
+  if (CODING < 0) {
+    planes(w, coding, p1, p2);
+  } else {
+    const uint32_t lo = w & BMG_EVEN, hi = (w >> 1) & BMG_EVEN;
+    if (CODING == 0) { p1 = hi; p2 = hi & lo; }
+    else if (CODING == 1) { p1 = hi & ~lo; p2 = 0u; }
+    else if (CODING == 2) { p1 = hi; p2 = 0u; }
+    else { p1 = hi & lo; p2 = 0u; }
+  }
 }
 
 // S.work[c] = x_c^T x_new for c in [0, cols], column `cols` being the new
-// one itself; column 0 is the constant.  Exact integer popcount dots.  Each
-// wave takes groups of 4 columns; a lane loads 4 words of the new column and
-// of each of the 4 columns before using any, so 20 loads are in flight.
-template <int KMAX>
-__device__ __forceinline__ void new_column_dots(Ctx<KMAX>& C, uint64_t snp_new, int coding_new, int cols) {
-  // pointers read once: loads of struct fields inside the loops would be
-  // repeated (and waited for) on every column group
-  const uint32_t W = C.D.W32;
+// one itself; column 0 is the constant.  Exact integer popcount dots.
+//
+// A lane owns one 16-word chunk of the individuals and a group of columns:
+// it decodes the new column's chunk once, then streams its columns' chunks
+// (the next column's words are loaded while the current one is summed);
+// the chunk sums meet in LDS integer accumulators (exact, so the order is
+// free).  W / 16 lanes cover a column, so 256 / (W / 16) column groups run
+// side by side.  CODING >= 0: every column has that coding (one effect type).
+template <int KMAX, int CODING>
+__device__ __forceinline__ void column_dots(Ctx<KMAX>& C, uint64_t snp_new, int coding_new, int cols) {
+  const uint32_t W = C.D.W32;                 // a multiple of 32 words
   const uint32_t* __restrict__ wn = C.D.g + snp_new * W;
   const uint32_t* __restrict__ xcol = C.A.xcol;
-  const int lane = lane_id();
-  for (int c0 = 4 * wave_id(); c0 <= cols; c0 += 4 * (BMG_BLOCK / BMG_WAVE)) {
-    const uint32_t* wc[4];
-    int cc[4];
-    bool on[4];
+  Smem<KMAX>& S = C.S;
+  const uint32_t cpc = W / 16;                // chunks per column
+  for (uint32_t ch0 = 0; ch0 < cpc; ch0 += BMG_BLOCK) {
+    const uint32_t span = min(cpc - ch0, (uint32_t)BMG_BLOCK);
+    const int groups = (int)(BMG_BLOCK / span);
+    const int grp = (int)(threadIdx.x / span);
+    const uint32_t w0 = (ch0 + threadIdx.x % span) * 16;
+    if (grp < groups) {
+      uint32_t n1[16], n2[16];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int c = c0 + q;
-      on[q] = c <= cols;
-      cc[q] = (c >= cols) ? coding_new : (c == 0 ? coding_new : (int)C.S.xtype[c]);
-      // the model's columns from the chain's contiguous slot cache
-      wc[q] = (c >= cols || c == 0) ? wn : xcol + (size_t)C.S.slot[c] * W;
-    }
-    int s[4] = {0, 0, 0, 0};   // per-lane partials: at most 4 * 16 per word
-    // straight-line body: every load is issued unconditionally at a valid
-    // address (index clamped, value masked), so nothing waits on a branch
-    for (uint32_t kb = lane; kb < W; kb += 4 * BMG_WAVE) {
-      uint32_t xn[4], xc[4][4], keep[4];
+      for (int q = 0; q < 4; ++q) {
+        const uint4 v = *(const uint4*)(wn + w0 + 4 * q);
+        const uint32_t xs[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const uint32_t k = kb + u * BMG_WAVE;
-        keep[u] = k < W ? 0xFFFFFFFFu : 0u;
-        const uint32_t kk = k < W ? k : W - 1;
-        xn[u] = wn[kk];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) xc[q][u] = wc[q][kk];
+        for (int u = 0; u < 4; ++u) planes_c<CODING>(xs[u], coding_new, n1[4 * q + u], n2[4 * q + u]);
       }
+      auto col_ptr = [&](int c) -> const uint32_t* {
+        return (c >= cols || c == 0) ? wn : xcol + (size_t)S.slot[c] * W;
+      };
+      int c = grp;
+      uint4 nx[4];
+      if (c <= cols) {
+        const uint32_t* pc = col_ptr(c);
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const uint32_t x = xn[u] & keep[u];
-        uint32_t n1, n2;
-        planes(x, coding_new, n1, n2);
+        for (int q = 0; q < 4; ++q) nx[q] = *(const uint4*)(pc + w0 + 4 * q);
+      }
+      for (; c <= cols; c += groups) {
+        uint4 cur[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) cur[q] = nx[q];
+        const int cn = c + groups;
+        if (cn <= cols) {
+          const uint32_t* pc = col_ptr(cn);
+#pragma unroll
+          for (int q = 0; q < 4; ++q) nx[q] = *(const uint4*)(pc + w0 + 4 * q);
+        }
+        const int cc = (c >= cols || c == 0) ? coding_new : (int)S.xtype[c];
+        int sum = 0;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          uint32_t a1, a2;
-          planes(xc[q][u], cc[q], a1, a2);
-          // column 0 is the constant: its planes are all ones
-          if (c0 + q == 0) { a1 = BMG_EVEN; a2 = 0u; }
-          s[q] += (int)(__popc(a1 & n1) + __popc(a1 & n2) + __popc(a2 & n1) + __popc(a2 & n2));
+          const uint32_t xs[4] = {cur[q].x, cur[q].y, cur[q].z, cur[q].w};
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            uint32_t a1, a2;
+            planes_c<CODING>(xs[u], cc, a1, a2);
+            // column 0 is the constant: its planes are all ones
+            if (c == 0) { a1 = BMG_EVEN; a2 = 0u; }
+            const int k = 4 * q + u;
+            sum += (int)(__popc(a1 & n1[k]) + __popc(a1 & n2[k]) + __popc(a2 & n1[k]) + __popc(a2 & n2[k]));
+          }
         }
+        atomicAdd(&S.iacc[c], sum);
       }
     }
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      int v = wave_sum_i32(s[q]);
-      if (lane == 0 && on[q]) C.S.work[c0 + q] = (double)v;
-    }
   }
+}
+
+template <int KMAX>
+__device__ __forceinline__ void new_column_dots(Ctx<KMAX>& C, uint64_t snp_new, int coding_new, int cols) {
+  Smem<KMAX>& S = C.S;
+  for (int c = threadIdx.x; c <= cols; c += BMG_BLOCK) S.iacc[c] = 0;
+  __syncthreads();
+  // the sampler's columns share one coding unless its type is AH (the
+  // Model API may mix them)
+  if (!C.D.mixed && C.D.type != BMG_AH && coding_new == C.D.type) {
+    switch (coding_new) {
+      case 0: column_dots<KMAX, 0>(C, snp_new, coding_new, cols); break;
+      case 1: column_dots<KMAX, 1>(C, snp_new, coding_new, cols); break;
+      case 2: column_dots<KMAX, 2>(C, snp_new, coding_new, cols); break;
+      default: column_dots<KMAX, 3>(C, snp_new, coding_new, cols); break;
+    }
+  } else {
+    column_dots<KMAX, -1>(C, snp_new, coding_new, cols);
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c <= cols; c += BMG_BLOCK) S.work[c] = (double)S.iacc[c];
   __syncthreads();
 }
 
@@ -1830,3 +1895,13 @@ __global__ __launch_bounds__(256) void k_model_op(ModelOpArgs a) {
 template __global__ void k_model_op<64>(ModelOpArgs);
 template __global__ void k_model_op<128>(ModelOpArgs);
 template __global__ void k_model_op<160>(ModelOpArgs);
+
+namespace bmg {
+
+int launch_xy_all(const DevData& D, int coding, double* out, hipStream_t s) {
+  const unsigned grid = (unsigned)std::min<uint64_t>(D.p, 65536);
+  hipLaunchKernelGGL(k_xy_all, dim3(grid), dim3(BMG_BLOCK), 0, s, D, coding, out);
+  return hipGetLastError() == hipSuccess ? BMG_OK : BMG_E_HIP;
+}
+
+}  // namespace bmg
