@@ -615,7 +615,7 @@ class Batch:
         o = np.zeros(16)
         check(lib().bmg_batch_profile(self.h, c, _p(o)))
         keys = ["tau2", "addrem", "swap", "statechange", "learned", "beta_sigma2", "pve_thin", "total",
-                "add_xy", "add_xtx", "add_trsv", "add_rest", "n_add", "n_rem", "rem", "unused"]
+                "add_xy", "add_xtx", "add_trsv", "add_rest", "n_add", "n_rem", "rem", "refresh_draws"]
         return dict(zip(keys, o))
 
     def timing(self):

@@ -291,36 +291,58 @@ __device__ __forceinline__ double readlane_d(double v, int src) {
 }
 
 // x := U^-T x for x[0..m) in LDS (dtrsv Upper/Trans order).  Wave 0 only.
-// Lane l holds x[l + 64 t] in b[t]; the outer loop over t is unrolled so the
-// register array is only ever indexed by constants.  Every lane forms
-// x_i = b_i / U_ii from the owner's b_i (a cross-lane read), so the serial
-// chain per row is one read and one division.
+// Lane l holds x[l + 64 t] in b[t] and the diagonal entry of that row in
+// dg[t]; the outer loop over t is unrolled so the register arrays are only
+// ever indexed by constants.  Row i: every lane divides its own pair and the
+// owner's quotient is read across lanes (so the serial chain per row is one
+// division and one cross-lane read, with no LDS wait), and the next row's
+// coefficients are loaded while this row's updates run.  Per element the
+// arithmetic is the sequential loop's.
 template <int KMAX>
 __device__ __forceinline__ void trsv_ut_w0(Smem<KMAX>& S, double* x, int m) {
   constexpr int NT = (KMAX + 63) / 64;
   const int lane = lane_id();
-  double b[NT];
+  double b[NT], dg[NT];
 #pragma unroll
-  for (int t = 0; t < NT; ++t) { int j = lane + 64 * t; b[t] = j < m ? x[j] : 0.0; }
+  for (int t = 0; t < NT; ++t) {
+    int j = lane + 64 * t;
+    b[t] = j < m ? x[j] : 0.0;
+    dg[t] = j < m ? S.L[P(j) + j] : 1.0;
+  }
 #pragma unroll
   for (int ti = 0; ti < NT; ++ti) {
+    const int i0 = 64 * ti;
     const int iend = m < 64 * (ti + 1) ? m : 64 * (ti + 1);
-    for (int i = 64 * ti; i < iend; ++i) {
+    if (i0 >= iend) break;
+    double cf[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      int j = lane + 64 * t;
+      cf[t] = (t >= ti && j > i0 && j < m) ? S.L[P(j) + i0] : 0.0;
+    }
+    for (int i = i0; i < iend; ++i) {
       const int owner = i & 63;
-      const double xi = readlane_d(b[ti], owner) / S.L[P(i) + i];
+      const double xi = readlane_d(b[ti] / dg[ti], owner);
+      double cn[NT];
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        int j = lane + 64 * t;
+        cn[t] = (t >= ti && j > i + 1 && j < m) ? S.L[P(j) + i + 1] : 0.0;
+      }
       if (lane == owner) b[ti] = xi;
 #pragma unroll
       for (int t = ti; t < NT; ++t) {
         int j = lane + 64 * t;
-        if (j > i && j < m) b[t] = b[t] - S.L[P(j) + i] * xi;
+        if (j > i && j < m) b[t] = b[t] - cf[t] * xi;
       }
+#pragma unroll
+      for (int t = 0; t < NT; ++t) cf[t] = cn[t];
     }
   }
 #pragma unroll
   for (int t = 0; t < NT; ++t) { int j = lane + 64 * t; if (j < m) x[j] = b[t]; }
 }
 
-// x := U^-1 x (dtrsv Upper/NoTrans order, columns from the last).  Wave 0.
 template <int KMAX>
 __device__ __forceinline__ void trsv_un_w0(Smem<KMAX>& S, double* x, int m) {
   constexpr int NT = (KMAX + 63) / 64;
@@ -370,29 +392,48 @@ __device__ __forceinline__ void drotg(double& a, double& b, double& c, double& s
 
 // Cholesky downdate removing column cr of the m x m factor (LINPACK dchex
 // job 2 + drop + sign fix, src/symmmatrix.cpp:166-195).  Logical column jl >=
-// cr lives in physical column jl + 1 until the final compaction; rotation t
-// is generated from logical column cr + t and applied to all later columns
-// at once (per element the same sequence as the sequential loop).
+// cr lives in physical column jl + 1 until the final compaction; rotation j
+// is generated from logical column j's rows (j, j + 1) and applied to every
+// later column at once (per element the same sequence as the sequential
+// loop).  The two rows a rotation touches stay in registers: lane jl % 64
+// keeps rows (j, j + 1) of its columns, reads the next row ahead from LDS
+// (rows below j + 1 are not rotated yet) and stores only the finished row,
+// and the owner of column j hands the rotation's inputs to all lanes with a
+// cross-lane read, so no step waits on an LDS round trip.
 template <int KMAX>
 __device__ __forceinline__ void chol_downdate(Smem<KMAX>& S, int m, int cr) {
   constexpr int NT = (KMAX + 63) / 64;
   if (wave_id() == 0) {
     const int lane = lane_id();
-    for (int j = cr; j <= m - 2; ++j) {
-      // every lane generates the rotation from the same two LDS values
-      double a = S.L[P(j + 1) + j], b = S.L[P(j + 1) + j + 1];
-      double cc, ss;
-      drotg(a, b, cc, ss);
-      if (lane == (j & 63)) S.L[P(j + 1) + j] = a;
+    double top[NT], nxt[NT];
 #pragma unroll
-      for (int t = 0; t < NT; ++t) {
-        int jl = lane + 64 * t;
-        if (jl > j && jl <= m - 2) {
-          double* col = S.L + P(jl + 1);
-          double x1 = col[j], x2 = col[j + 1];
-          double tt = cc * x1 + ss * x2;
-          col[j + 1] = cc * x2 - ss * x1;
-          col[j] = tt;
+    for (int t = 0; t < NT; ++t) {
+      const int jl = lane + 64 * t;
+      const bool on = jl >= cr && jl <= m - 2;
+      top[t] = on ? S.L[P(jl + 1) + cr] : 0.0;
+      nxt[t] = on ? S.L[P(jl + 1) + cr + 1] : 0.0;
+    }
+#pragma unroll
+    for (int ti = 0; ti < NT; ++ti) {
+      const int jbeg = cr > 64 * ti ? cr : 64 * ti;
+      const int jend = (m - 1) < 64 * (ti + 1) ? (m - 1) : 64 * (ti + 1);
+      for (int j = jbeg; j < jend; ++j) {
+        const int owner = j & 63;
+        // every lane generates the rotation from the same two values
+        double a = readlane_d(top[ti], owner), b = readlane_d(nxt[ti], owner);
+        double cc, ss;
+        drotg(a, b, cc, ss);
+        if (lane == owner) S.L[P(j + 1) + j] = a;
+#pragma unroll
+        for (int t = ti; t < NT; ++t) {
+          const int jl = lane + 64 * t;
+          if (jl > j && jl <= m - 2) {
+            const double x1 = top[t], x2 = nxt[t];
+            const double tt = cc * x1 + ss * x2;
+            top[t] = cc * x2 - ss * x1;
+            S.L[P(jl + 1) + j] = tt;
+            nxt[t] = S.L[P(jl + 1) + j + 2];   // j + 2 <= jl + 1: inside the column
+          }
         }
       }
     }
@@ -962,15 +1003,20 @@ __device__ __forceinline__ void sample_beta_sigma2(Ctx<KMAX>& C) {
   __syncthreads();
 }
 
-// y_hat = X_g beta_g (+ constant) into the chain's y_hat; returns sum x_b^2,
-// sum x_b (x_b = X_g beta_g) and sum of y_hat, y_hat^2 for the PVE
-// X_g beta_g (without the constant) for every individual, one packed word
-// (16 individuals) per lane: the columns' words are fetched eight at a time
-// (coalesced across lanes, several loads in flight) and each individual's
-// sum is formed in model-column order, as the per-individual loop does.
-// out5: sum xb, sum xb^2, sum y_hat, sum y_hat^2, sum (beta_0 - y) xb.
-template <int KMAX>
-__device__ __forceinline__ void xbeta(Ctx<KMAX>& C, double* out5, int write_yhat) {
+// value of a 2-bit genotype code times t under a coding known at compile
+// time: 0, t or 2t, selected (t * (double)value is exactly one of these, and
+// adding +0 leaves a sum that is never -0 unchanged, so the sums are the
+// multiply-add loop's bit for bit)
+template <int CODING>
+__device__ __forceinline__ double coded_times(uint32_t code, double t, double t2) {
+  if (CODING == 0) return code == 3u ? t2 : (code == 2u ? t : 0.0);
+  if (CODING == 1) return code == 2u ? t : 0.0;
+  if (CODING == 2) return code >= 2u ? t : 0.0;
+  return code == 3u ? t : 0.0;
+}
+
+template <int KMAX, int CODING>
+__device__ __forceinline__ void xbeta_impl(Ctx<KMAX>& C, double* out5, int write_yhat) {
   Smem<KMAX>& S = C.S;
   Reg& R = C.R;
   const double b0 = S.beta[0];
@@ -997,11 +1043,18 @@ __device__ __forceinline__ void xbeta(Ctx<KMAX>& C, double* out5, int write_yhat
         if (c < L) {
           const double t = S.beta[c];
           if (t != 0.0) {
-            const int xt = S.xtype[c];
+            if (CODING >= 0) {
+              const double t2 = t + t;
 #pragma unroll
-            for (int q = 0; q < 16; ++q) {
-              int iv = code_ival((wd[u] >> (2 * q)) & 3u, xt);
-              acc[q] = acc[q] + t * (double)iv;
+              for (int q = 0; q < 16; ++q)
+                acc[q] = acc[q] + coded_times<CODING>((wd[u] >> (2 * q)) & 3u, t, t2);
+            } else {
+              const int xt = S.xtype[c];
+#pragma unroll
+              for (int q = 0; q < 16; ++q) {
+                int iv = code_ival((wd[u] >> (2 * q)) & 3u, xt);
+                acc[q] = acc[q] + t * (double)iv;
+              }
             }
           }
         }
@@ -1025,6 +1078,27 @@ __device__ __forceinline__ void xbeta(Ctx<KMAX>& C, double* out5, int write_yhat
   out5[2] = block_sum(sy, S.red);
   out5[3] = block_sum(sy2, S.red);
   out5[4] = block_sum(syx, S.red);
+}
+
+// y_hat = X_g beta_g (+ constant) into the chain's y_hat (write_yhat) and
+// out5: sum xb, sum xb^2, sum y_hat, sum y_hat^2, sum (beta_0 - y) xb, with
+// xb = X_g beta_g without the constant.  One packed word (16 individuals)
+// per lane: the columns' words are fetched eight at a time (coalesced across
+// lanes, several loads in flight) and each individual's sum is formed in
+// model-column order, as the per-individual loop does.
+template <int KMAX>
+__device__ __forceinline__ void xbeta(Ctx<KMAX>& C, double* out5, int write_yhat) {
+  // the sampler's columns share one coding unless its type is AH (the
+  // Model API may mix them)
+  if (!C.D.mixed && C.D.type != BMG_AH) {
+    switch (C.D.type) {
+      case 0: xbeta_impl<KMAX, 0>(C, out5, write_yhat); return;
+      case 1: xbeta_impl<KMAX, 1>(C, out5, write_yhat); return;
+      case 2: xbeta_impl<KMAX, 2>(C, out5, write_yhat); return;
+      default: xbeta_impl<KMAX, 3>(C, out5, write_yhat); return;
+    }
+  }
+  xbeta_impl<KMAX, -1>(C, out5, write_yhat);
 }
 
 // src/model.hpp:345-392 (no covariates: pves[2] = 0)
@@ -1644,7 +1718,9 @@ __device__ __forceinline__ void refresh_tau2(Ctx<KMAX>& C) {
   double ita[4];
 #pragma unroll
   for (int q = 0; q < 4; ++q) ita[q] = R.ita2[q];
+  const uint64_t t0 = wall_clock64();
   sample_alpha_and_tau2(C);
+  if (threadIdx.x == 0 && C.prof) C.prof[15] += wall_clock64() - t0;
   compute_log_likelihood(C);
   R.lmin = 0;
   if (isfinite(R.ll)) {
