@@ -194,6 +194,39 @@ def read_calibration(bmg, nbytes=1 << 31):
     return dict(bytes=nbytes, ms=ms, gbs=nbytes / (ms * 1e-3) / 1e9)
 
 
+def phase_profile(bmg, synth, ctx, args):
+    """Per-chain device clocks (Batch.profile) over the timed steps: us per
+    iteration by phase, and per add / per removal."""
+    ips, n = args.iters_per_step, args.chains
+    total = (args.warmup + args.steps) * ips + 1000
+    cfgs = [synth.sampler_cfg(N, P, N_CAUSAL, types="A", seed=100 + c, chain=c, do_n_iter=total,
+                              n_rao_burnin=4, thin=args.thin) for c in range(n)]
+    b = bmg.Batch([ctx], [0] * n, cfgs, dict(proposal=0, use_graph=0,
+                                              trace_capacity=(total // args.thin + 16) * 200))
+    b.setup()
+    b.run(args.warmup * ips)
+    p0 = [b.profile(c) for c in range(n)]
+    t0 = b.timing()
+    b.run(args.steps * ips)
+    p1 = [b.profile(c) for c in range(n)]
+    t1 = b.timing()
+    it = args.steps * ips
+    for c in range(n):
+        d = {k: p1[c][k] - p0[c][k] for k in p0[c]}
+        na, nr = max(d["n_add"], 1), max(d["n_rem"], 1)
+        us = lambda k, per: round(d[k] / per * 1e3, 2)
+        print(json.dumps(dict(chain=c, size=b.state(c)["size"], us_per_iter=us("total", it),
+                              adds_per_iter=round(d["n_add"] / it, 2), removes_per_iter=round(d["n_rem"] / it, 2),
+                              add_xy=us("add_xy", na), add_xtx=us("add_xtx", na), add_solve=us("add_trsv", na),
+                              add_rest=us("add_rest", na), remove=us("rem", nr),
+                              refresh=us("tau2", it), refresh_draws=us("refresh_draws", it),
+                              beta_sigma2=us("beta_sigma2", it), pve_thin=us("pve_thin", it),
+                              addrem_moves=us("addrem", it), statechange=us("statechange", it),
+                              swap=us("swap", it))), flush=True)
+    print(json.dumps(dict(kernel_us_per_iter=round((t1["chain_ms"] - t0["chain_ms"]) / it * 1e3, 1))), flush=True)
+    b.close()
+
+
 def cpu_baseline(ctx, y, monitored, seconds, thin):
     import oracle
     oracle.load()
@@ -245,6 +278,9 @@ def main():
                     help="also run the learned batch with this many chains (e.g. 256: one per CU)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="time only the scan, band and copy kernels (for profiler runs)")
+    ap.add_argument("--phase-profile", action="store_true",
+                    help="print each chain's per-phase device clocks over the timed steps of a plain batch "
+                         "(eager, so no graph hides the segments) and exit")
     ap.add_argument("--thin", type=int, default=10,
                     help="thinning (part of the sampler's schedule: beta and sigma2 are drawn at "
                          "every thinned iteration, as in the reference)")
@@ -274,6 +310,10 @@ def main():
     ctx.set_y(y)
     t_data = time.perf_counter() - t0
     log("dataset on device in %.1f s" % t_data)
+
+    if args.phase_profile:
+        phase_profile(bmg, synth, ctx, args)
+        return
 
     if args.kernels_only:
         sc = scan_roofline(bmg, ctx)
