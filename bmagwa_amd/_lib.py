@@ -174,6 +174,7 @@ SIGNATURES = {
     "bmg_batch_p_rao": (C.c_int, _P, C.c_int, _D),
     "bmg_batch_p_proposal": (C.c_int, _P, C.c_int, _D),
     "bmg_batch_trace": (C.c_int, _P, C.c_int, _U8, _U8, _U8, _U32, _D, C.c_int64, _I64),
+    "bmg_batch_write_outputs": (C.c_int, _P, C.c_int, C.c_char_p),
     "bmg_batch_thinned": (C.c_int, _P, C.c_int, _U32, C.c_int64, _U32, C.c_int64, _D, _D, _D, _I64, _I64),
     "bmg_batch_rb": (C.c_int, _P, C.c_int, _D, _D),
     "bmg_batch_timing": (C.c_int, _P, _D),
@@ -622,6 +623,10 @@ class Batch:
         o = np.zeros(4)
         check(lib().bmg_batch_timing(self.h, _p(o)))
         return dict(total_ms=o[0], chain_ms=o[1], scan_ms=o[2], n_scans=int(o[3]))
+
+    def write_outputs(self, c, basename):
+        """The reference's binary output files of chain c (basename + "_loci.dat", ...)."""
+        check(lib().bmg_batch_write_outputs(self.h, c, basename.encode()))
 
     def graph_stats(self):
         """Steady-state graph: segments replayed, ms in replays, builds."""

@@ -737,7 +737,7 @@ static int batch_create(bmg_ctx** ctxs, int n_ctx, int n_chains, const int32_t* 
   size_t o_trm = add(tr_cap), o_trj = add(tr_cap), o_trs = add(tr_cap), o_trz = add(4 * tr_cap),
          o_trl = add(8 * tr_cap);
   size_t o_ths = add(4 * th_cap), o_thl = add(4 * th_loci_cap), o_thll = add(8 * th_cap),
-         o_ths2 = add(8 * th_cap), o_tha = add(8 * th_cap);
+         o_ths2 = add(8 * th_cap), o_tha = add(8 * th_cap), o_thp = add(8 * 3 * th_cap);
   b->per_chain = sz;
   hipError_t e = hipMalloc(&b->pool, sz * n_chains);
   if (e != hipSuccess) { bmg_batch_destroy(b); return hip_fail(e, "hipMalloc(chain pool)"); }
@@ -761,6 +761,7 @@ static int batch_create(bmg_ctx** ctxs, int n_ctx, int n_chains, const int32_t* 
     A.tr_size = (uint32_t*)(base + o_trz); A.tr_ll = (double*)(base + o_trl);
     A.th_size = (uint32_t*)(base + o_ths); A.th_loci = (uint32_t*)(base + o_thl);
     A.th_ll = (double*)(base + o_thll); A.th_s2 = (double*)(base + o_ths2); A.th_alpha = (double*)(base + o_tha);
+    A.th_pve = (double*)(base + o_thp);
     b->h_arrays.push_back(A);
     // constants (src/sampler.hpp:107-203, src/prior.hpp:45-122)
     ChainConst Kc{};
@@ -1305,6 +1306,71 @@ extern "C" int bmg_batch_thinned(bmg_batch* b, int chain, uint32_t* sizes, int64
     if (alpha) BMG_HIP(hipMemcpy(alpha, A.th_alpha, 8 * cs, hipMemcpyDeviceToHost));
   }
   if (cl > 0 && loci) BMG_HIP(hipMemcpy(loci, A.th_loci, 4 * cl, hipMemcpyDeviceToHost));
+  return BMG_OK;
+}
+
+// --- output files of Sampler::sample (src/sampler.cpp:560-584,687-728,846-858)
+static int write_file(const std::string& path, const void* data, size_t bytes) {
+  FILE* f = fopen(path.c_str(), "wb");
+  if (!f) { set_error("cannot open " + path); return BMG_E_ARG; }
+  size_t w = bytes ? fwrite(data, 1, bytes, f) : 0;
+  int bad = fclose(f) != 0 || w != bytes;
+  if (bad) { set_error("cannot write " + path); return BMG_E_ARG; }
+  return BMG_OK;
+}
+
+extern "C" int bmg_batch_write_outputs(bmg_batch* b, int chain, const char* basename) {
+  if (!basename) { set_error("null basename"); return BMG_E_ARG; }
+  ChainScal s;
+  int rc = get_scal(b, chain, &s);
+  if (rc) return rc;
+  if (!b->cfg.trace_every_iter) {
+    set_error("the per-iteration files need trace_every_iter = 1"); return BMG_E_STATE;
+  }
+  if (s.err & ERR_TRACE) { set_error("trace buffers overflowed: the files would be incomplete"); return BMG_E_CAPACITY; }
+  const ChainArrays& A = b->h_arrays[chain];
+  const ChainConst& K = b->h_consts[chain];
+  const std::string base(basename);
+  const size_t ns = (size_t)s.th_count, nl = (size_t)s.th_loci_len, nt = (size_t)s.tr_count;
+  std::vector<uint32_t> sizes(ns), loci(nl);
+  std::vector<double> ll(ns), s2(ns), al(ns), pve(3 * ns), lp(ns);
+  std::vector<uint8_t> mv(nt), jd(nt), ms(nt);
+  std::vector<double> rao(b->h_data[b->chain_ds[chain]].p);
+  if (ns) {
+    BMG_HIP(hipMemcpy(sizes.data(), A.th_size, 4 * ns, hipMemcpyDeviceToHost));
+    BMG_HIP(hipMemcpy(ll.data(), A.th_ll, 8 * ns, hipMemcpyDeviceToHost));
+    BMG_HIP(hipMemcpy(s2.data(), A.th_s2, 8 * ns, hipMemcpyDeviceToHost));
+    BMG_HIP(hipMemcpy(al.data(), A.th_alpha, 8 * ns, hipMemcpyDeviceToHost));
+    BMG_HIP(hipMemcpy(pve.data(), A.th_pve, 8 * 3 * ns, hipMemcpyDeviceToHost));
+  }
+  if (nl) BMG_HIP(hipMemcpy(loci.data(), A.th_loci, 4 * nl, hipMemcpyDeviceToHost));
+  if (nt) {
+    BMG_HIP(hipMemcpy(mv.data(), A.tr_move, nt, hipMemcpyDeviceToHost));
+    BMG_HIP(hipMemcpy(jd.data(), A.tr_jd, nt, hipMemcpyDeviceToHost));
+    BMG_HIP(hipMemcpy(ms.data(), A.tr_ms, nt, hipMemcpyDeviceToHost));
+  }
+  BMG_HIP(hipMemcpy(rao.data(), A.p_rao, 8 * rao.size(), hipMemcpyDeviceToHost));
+  // the learned swap has no move code in the reference (3 is its type switch)
+  for (uint8_t& m : mv) if (m == 3) m = BMG_MOVE_LEARNED;
+  // log model prior (src/prior.hpp:169-183) of one effect type with n_g terms:
+  // lgamma(tp + n) + lgamma(a + n) + lgamma(b + m - n) - lgamma(tp_sum + n)
+  for (size_t t = 0; t < ns; ++t) {
+    const double n = (double)sizes[t];
+    lp[t] = std::lgamma(K.tp_type + n) + (std::lgamma(K.g_a + n) + std::lgamma(K.g_b + K.m_g - n) -
+                                          std::lgamma(K.tp_sum + n));
+  }
+  struct F { const char* name; const void* p; size_t bytes; };
+  const F files[] = {
+      {"_loci.dat", loci.data(), 4 * nl}, {"_modelsize.dat", sizes.data(), 4 * ns},
+      {"_jumpdistance.dat", jd.data(), nt}, {"_log_likelihood.dat", ll.data(), 8 * ns},
+      {"_log_prior.dat", lp.data(), 8 * ns}, {"_move_type.dat", mv.data(), nt},
+      {"_move_size.dat", ms.data(), nt}, {"_pve.dat", pve.data(), 8 * 3 * ns},
+      {"_alpha.dat", al.data(), 8 * ns}, {"_sigma2.dat", s2.data(), 8 * ns},
+      {"_rao.dat", rao.data(), 8 * rao.size()}};
+  for (const F& f : files) {
+    rc = write_file(base + f.name, f.p, f.bytes);
+    if (rc) return rc;
+  }
   return BMG_OK;
 }
 

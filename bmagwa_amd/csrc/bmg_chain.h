@@ -95,6 +95,7 @@ struct ChainArrays {
   double* adapt;             // 6 x 256: q_pms, q_pms_nbs, q_pms_nbc, r_ms_sum, r_ms_n, q_p0_ms
   uint8_t* tr_move; uint8_t* tr_jd; uint8_t* tr_ms; uint32_t* tr_size; double* tr_ll;
   uint32_t* th_size; uint32_t* th_loci; double* th_ll; double* th_s2; double* th_alpha;
+  double* th_pve;        // 3 per thinned sample (src/model.hpp:345-392)
 };
 
 // error flags of a chain; every flag from ERR_NONFINITE on stops the chain

@@ -1811,6 +1811,7 @@ __device__ __forceinline__ void run_iterations(Ctx<KMAX>& C, ChainScal& CS, uint
           C.A.th_size[t] = (uint32_t)R.c_nloci;
           for (int q = 0; q < R.c_nloci; ++q) C.A.th_loci[ll0 + q] = S.c_loci[q];
           C.A.th_ll[t] = R.c_ll; C.A.th_s2[t] = R.sigma2; C.A.th_alpha[t] = R.alpha;
+          C.A.th_pve[3 * t] = pves[0]; C.A.th_pve[3 * t + 1] = pves[1]; C.A.th_pve[3 * t + 2] = pves[2];
           CS.th_count = t + 1; CS.th_loci_len = ll0 + R.c_nloci;
         } else {
           R.err |= ERR_TRACE;

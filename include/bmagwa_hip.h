@@ -270,6 +270,19 @@ int bmg_batch_trace(bmg_batch* b, int chain, uint8_t* move, uint8_t* jd, uint8_t
 int bmg_batch_thinned(bmg_batch* b, int chain, uint32_t* sizes, int64_t cap_sizes,
                       uint32_t* loci, int64_t cap_loci, double* loglik, double* sigma2,
                       double* alpha, int64_t* n_samples, int64_t* n_loci);
+/* The binary output files of Sampler::sample for one chain
+ * (src/sampler.cpp:560-584,687-728,846-858; README "Binary files"), named
+ * basename + "_loci.dat", "_modelsize.dat", "_jumpdistance.dat",
+ * "_log_likelihood.dat", "_log_prior.dat", "_move_type.dat",
+ * "_move_size.dat", "_pve.dat", "_alpha.dat", "_sigma2.dat" and "_rao.dat",
+ * in the reference's value types (uint32, uint8, double; native byte order);
+ * one effect type, so no "_types.dat" / "_rao_types.dat", as the reference
+ * with a single type.  The per-iteration files need trace_every_iter = 1.  A
+ * learned swap (no counterpart in the reference) is move code
+ * BMG_MOVE_LEARNED.  The basename is the reference's basename plus the
+ * chain number. */
+#define BMG_MOVE_LEARNED 7
+int bmg_batch_write_outputs(bmg_batch* b, int chain, const char* basename);
 /* RB scan of one chain's current state with a given y_hat (the drop-in for
  * RaoBlackwellizer::p_raoblackwell, src/sampler.cpp:32-261) */
 int bmg_batch_rb(bmg_batch* b, int chain, const double* y_hat, double* p_r);

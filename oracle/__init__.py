@@ -141,6 +141,7 @@ def load():
     _sig(lib, "orc_sampler_p_proposal", _D, _P)
     _sig(lib, "orc_sampler_trace", C.c_int64, _P, _U8, _U8, _U8, _U32, _D)
     _sig(lib, "orc_sampler_thinned", C.c_int64, _P, _U32, _U32, C.c_int64, _D, _D, _D)
+    _sig(lib, "orc_sampler_thinned_pve", C.c_int64, _P, _D)
     _sig(lib, "orc_sampler_state", None, _P, _D)
     _sig(lib, "orc_sampler_loci", C.c_int, _P, _U32)
     _sig(lib, "orc_sampler_rb", None, _P, _D, _D)
@@ -411,8 +412,10 @@ class Sampler:
         ll = np.zeros(n + 1); s2 = np.zeros(n + 1); al = np.zeros(n + 1)
         cnt = lib().orc_sampler_thinned(self.h, ptr(sizes, C.c_uint32), ptr(loci, C.c_uint32),
                                         nl, ptr(ll), ptr(s2), ptr(al))
+        pve = np.zeros(3 * (n + 1))
+        lib().orc_sampler_thinned_pve(self.h, ptr(pve))
         return dict(sizes=sizes[:cnt], loci=loci[:nl], loglik=ll[:cnt], sigma2=s2[:cnt],
-                    alpha=al[:cnt])
+                    alpha=al[:cnt], pve=pve[:3 * cnt].reshape(cnt, 3))
 
     def state(self):
         o = np.zeros(8)

@@ -159,6 +159,8 @@ int64_t orc_sampler_trace(const orc_sampler* s, uint8_t* move, uint8_t* jd,
 int64_t orc_sampler_thinned(const orc_sampler* s, uint32_t* sizes,
                             uint32_t* loci, int64_t loci_cap, double* ll,
                             double* sigma2, double* alpha);
+/* PVE of every thinned sample (3 values each, src/model.hpp:345-392) */
+int64_t orc_sampler_thinned_pve(const orc_sampler* s, double* pve);
 void   orc_sampler_state(const orc_sampler* s, double out[8]);
 int    orc_sampler_loci(const orc_sampler* s, uint32_t* loci);
 /* RB scan on the current model with a given y_hat, src/sampler.cpp:32-261 */

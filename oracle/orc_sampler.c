@@ -248,7 +248,7 @@ struct orc_sampler {
   int64_t tr_len, tr_cap;
   uint8_t *tr_move, *tr_jd, *tr_ms; uint32_t* tr_size; double* tr_ll;
   int64_t th_len, th_cap, th_loci_len, th_loci_cap;
-  uint32_t* th_size; uint32_t* th_loci; double *th_ll, *th_s2, *th_alpha;
+  uint32_t* th_size; uint32_t* th_loci; double *th_ll, *th_s2, *th_alpha, *th_pve;
 };
 
 /* src/sampler.cpp:455-515 (PMV only) */
@@ -368,7 +368,7 @@ void orc_sampler_destroy(orc_sampler* s)
   free(s->q_pms_nbs); free(s->q_pms_nbc);
   free(s->dr_probs); free(s->dr_q_add); free(s->dr_q_rem); free(s->dr_lqt); free(s->dr_b2n);
   free(s->tr_move); free(s->tr_jd); free(s->tr_ms); free(s->tr_size); free(s->tr_ll);
-  free(s->th_size); free(s->th_loci); free(s->th_ll); free(s->th_s2); free(s->th_alpha);
+  free(s->th_size); free(s->th_loci); free(s->th_ll); free(s->th_s2); free(s->th_alpha); free(s->th_pve);
   free(s);
 }
 
@@ -1346,6 +1346,7 @@ static void thinned_push(orc_sampler* s)
     s->th_ll = (double*)realloc(s->th_ll, s->th_cap * sizeof(double));
     s->th_s2 = (double*)realloc(s->th_s2, s->th_cap * sizeof(double));
     s->th_alpha = (double*)realloc(s->th_alpha, s->th_cap * sizeof(double));
+    s->th_pve = (double*)realloc(s->th_pve, 3 * s->th_cap * sizeof(double));
   }
   int k = s->cur->nloci;
   while (s->th_loci_len + k > s->th_loci_cap) {
@@ -1358,6 +1359,7 @@ static void thinned_push(orc_sampler* s)
   s->th_ll[s->th_len] = s->cur->loglik;
   s->th_s2[s->th_len] = s->cur->sigma2;
   s->th_alpha[s->th_len] = s->pr.alpha;
+  memcpy(s->th_pve + 3 * s->th_len, s->pves, 3 * sizeof(double));
   s->th_len++;
 }
 
@@ -1516,6 +1518,12 @@ int64_t orc_sampler_thinned(const orc_sampler* s, uint32_t* sizes,
   if (ll) memcpy(ll, s->th_ll, s->th_len * sizeof(double));
   if (sigma2) memcpy(sigma2, s->th_s2, s->th_len * sizeof(double));
   if (alpha) memcpy(alpha, s->th_alpha, s->th_len * sizeof(double));
+  return s->th_len;
+}
+
+int64_t orc_sampler_thinned_pve(const orc_sampler* s, double* pve)
+{
+  if (pve) memcpy(pve, s->th_pve, 3 * s->th_len * sizeof(double));
   return s->th_len;
 }
 

@@ -136,3 +136,44 @@ def test_uneven_datasets_and_steady_state_graph(bmg):
         s = oracle_run(cfgs[c], body, y, n_iter)
         compare(out[1][c][0], s.trace(), n_iter)
         assert np.allclose(out[1][c][2], s.p_rao(), rtol=1e-8, atol=1e-10)
+
+
+def test_output_files_in_the_reference_format(bmg, tmp_path):
+    """bmg_batch_write_outputs writes Sampler::sample's binary files
+    (src/sampler.cpp:560-584,687-728,846-858; the README's value types): the
+    thinned files hold the oracle run's samples (same stream: identical loci
+    and sizes, log-likelihoods, sigma2, alpha and PVE within 1e-10), the
+    per-iteration files its moves, jump distances and move sizes, the prior
+    file the model-size prior of src/prior.hpp:169-183 at each sample, and
+    _rao.dat the RB PIPs."""
+    import math
+    n, p, n_iter = 256, 300, 1200
+    body, y, _ = dataset(n, p, (0.9, 0.9, 10, 30), 4, 99)
+    cfg = synth.sampler_cfg(n, p, 4, seed=12, n_rao=100, n_rao_burnin=3, thin=10, max_move_size=8,
+                            do_n_iter=n_iter)
+    ctx = bmg.Ctx(n, p, body, y)
+    b = bmg.Batch([ctx], [0], [cfg], dict(trace_every_iter=1))
+    b.setup()
+    b.run(n_iter)
+    base = str(tmp_path / "run1")
+    b.write_outputs(0, base)
+    s = oracle_run(cfg, body, y, n_iter)
+    th, tr = s.thinned(), s.trace()
+    rd = lambda name, dt: np.fromfile(base + name, dtype=dt)
+    assert np.array_equal(rd("_modelsize.dat", np.uint32), th["sizes"])
+    assert np.array_equal(rd("_loci.dat", np.uint32), th["loci"])
+    for name, key in (("_log_likelihood.dat", "loglik"), ("_sigma2.dat", "sigma2"), ("_alpha.dat", "alpha")):
+        assert np.allclose(rd(name, np.float64), th[key], rtol=1e-10, atol=0), name
+    assert np.allclose(rd("_pve.dat", np.float64).reshape(-1, 3), th["pve"], rtol=1e-9, atol=1e-14)
+    assert np.array_equal(rd("_move_type.dat", np.uint8), tr["move"])
+    assert np.array_equal(rd("_jumpdistance.dat", np.uint8), tr["jd"])
+    assert np.array_equal(rd("_move_size.dat", np.uint8), tr["ms"])
+    assert len(rd("_move_type.dat", np.uint8)) == n_iter and len(th["sizes"]) == n_iter // 10
+    # model-size prior of one effect type: beta-binomial with the solved (a, b)
+    pr = s.prior()
+    lp = rd("_log_prior.dat", np.float64)
+    ref = [math.lgamma(1.0 + k) + (math.lgamma(pr["g_a"] + k) + math.lgamma(pr["g_b"] + p - k)
+                                    - math.lgamma(1.0 + k)) for k in th["sizes"].astype(float)]
+    assert np.allclose(lp, ref, rtol=1e-12, atol=1e-9)
+    assert np.allclose(rd("_rao.dat", np.float64), s.p_rao(), rtol=1e-8, atol=1e-10)
+    assert not (tmp_path / "run1_types.dat").exists()
