@@ -219,6 +219,7 @@ def phase_profile(bmg, synth, ctx, args):
                               adds_per_iter=round(d["n_add"] / it, 2), removes_per_iter=round(d["n_rem"] / it, 2),
                               add_xy=us("add_xy", na), add_xtx=us("add_xtx", na), add_solve=us("add_trsv", na),
                               add_rest=us("add_rest", na), remove=us("rem", nr),
+                              remove_downdate=us("rem_downdate", nr), remove_solve=us("rem_solve", nr),
                               refresh=us("tau2", it), refresh_draws=us("refresh_draws", it),
                               beta_sigma2=us("beta_sigma2", it), pve_thin=us("pve_thin", it),
                               addrem_moves=us("addrem", it), statechange=us("statechange", it),

@@ -63,7 +63,7 @@ struct ChainScal {
   // change, learned swap, beta/sigma2, pve + thinning, whole iterations;
   // then inside additions: x.y, X^T x, triangular solve, the rest of the
   // update, and the count of additions and removals, time in removals
-  uint64_t prof[16];
+  uint64_t prof[20];
   uint32_t rb_calls;
   int32_t movesize;
   double pves[3];

@@ -772,11 +772,14 @@ __device__ __forceinline__ void update_on_remove_impl(Ctx<KMAX>& C, int x_ind) {
     R.ldl -= log(lxx);
     R.syx += vx * vx;
   } else {
+    const uint64_t ta = wall_clock64();
     chol_downdate(S, k, x_ind);
+    const uint64_t tb = wall_clock64();
     for (int i = threadIdx.x; i < R.cols; i += BMG_BLOCK) S.v[i] = S.xy[i];
     __syncthreads();
     if (wave_id() == 0) trsv_ut_w0(S, S.v, R.cols);
     __syncthreads();
+    if (threadIdx.x == 0 && C.prof) { C.prof[16] += tb - ta; C.prof[17] += wall_clock64() - tb; }
     double ld = 0.0, vv = 0.0;
     for (int i = threadIdx.x; i < R.cols; i += BMG_BLOCK) {
       ld += log(S.L[P(i) + i]);

@@ -299,9 +299,10 @@ int bmg_batch_graph_stats(bmg_batch* b, double out[3]);
  * refreshes, add/remove moves, swaps, state changes, learned swaps,
  * beta/sigma2 draws, PVE + thinning, whole iterations; then within term
  * additions: x.y, X^T x, the triangular solve, the rest; then the number of
- * additions, the number of removals, ms in removals, and ms in the alpha /
- * tau2 draws of the refreshes (device clock) */
-int bmg_batch_profile(bmg_batch* b, int chain, double out[16]);
+ * additions, the number of removals, ms in removals, ms in the alpha /
+ * tau2 draws of the refreshes; then within removals of an inner column: the
+ * Cholesky downdate, the triangular solve; 0, 0 (device clock) */
+int bmg_batch_profile(bmg_batch* b, int chain, double out[20]);
 
 /* --------------------------------------- learned-proposal network ----- */
 /* No counterpart in the reference.  The network of the learned proposal:
