@@ -233,6 +233,7 @@ const double* orc__data_miss_prior(const orc_data* d, size_t s)
 /* ------------------------------------------------------------ DataModel */
 struct orc_datamodel {
   const orc_data* d;
+  uint64_t m_g;         /* the data's SNP count, so destroy does not read d */
   int n_types; int types[5];
   int allow_types[5], allow_terms[5];
   char** miss_val;      /* same indexing as miss_loc (slot 0 unused) */
@@ -251,6 +252,7 @@ orc_datamodel* orc_datamodel_create(const orc_data* d, const int32_t* types,
     if (types[i] == ORC_AH) { dm->allow_terms[ORC_A] = 1; dm->allow_terms[ORC_H] = 1; }
     else dm->allow_terms[types[i]] = 1;
   }
+  dm->m_g = d->m_g;
   dm->miss_val = (char**)calloc(d->m_g, sizeof(char*));
   for (uint64_t s = 0; s < d->m_g; ++s)
     if (d->miss_loc[s])
@@ -271,7 +273,7 @@ orc_datamodel* orc_datamodel_create(const orc_data* d, const int32_t* types,
 void orc_datamodel_destroy(orc_datamodel* dm)
 {
   if (!dm) return;
-  for (uint64_t s = 0; s < dm->d->m_g; ++s) free(dm->miss_val[s]);
+  for (uint64_t s = 0; s < dm->m_g; ++s) free(dm->miss_val[s]);
   free(dm->miss_val); free(dm);
 }
 

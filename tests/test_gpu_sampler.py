@@ -177,3 +177,24 @@ def test_output_files_in_the_reference_format(bmg, tmp_path):
     assert np.allclose(lp, ref, rtol=1e-12, atol=1e-9)
     assert np.allclose(rd("_rao.dat", np.float64), s.p_rao(), rtol=1e-8, atol=1e-10)
     assert not (tmp_path / "run1_types.dat").exists()
+
+
+def test_reference_cpu_configuration_trajectory(bmg):
+    """The reference's CPU-runnable workload shape: n = 1,000, p = 10,000
+    independent SNPs (MAF ~ U(0.05, 0.5), Hardy-Weinberg), 10 causal SNPs,
+    h2 = 0.3, additive coding, one chain: the device reproduces the oracle's
+    trajectory (moves, jump distances, move sizes, model sizes identical;
+    log-likelihoods within 1e-10) over 3,000 iterations, and the RB PIPs."""
+    n, p, n_iter = 1000, 10000, 3000
+    thresh, rho, _ = synth.snp_layout(p, (0.05, 0.5), None, 2024)
+    body = oracle.synth_genotypes(2024, n, p, thresh, rho)
+    dec = lambda snps: np.stack([oracle.decode(body, n, int(j), 0) for j in snps])
+    y, causal, _ = synth.phenotype(dec, n, p, 10, 0.3, 2024)
+    cfg = synth.sampler_cfg(n, p, 10, seed=5, n_rao=500, n_rao_burnin=2, thin=10, do_n_iter=n_iter)
+    ctx = bmg.Ctx(n, p, body, y)
+    b = bmg.Batch([ctx], [0], [cfg], dict(trace_every_iter=1))
+    b.setup()
+    b.run(n_iter)
+    s = oracle_run(cfg, body, y, n_iter)
+    compare(b.trace(0), s.trace(), n_iter)
+    assert np.allclose(b.p_rao(0), s.p_rao(), rtol=1e-8, atol=1e-10)
