@@ -685,12 +685,12 @@ static int batch_create(bmg_ctx** ctxs, int n_ctx, int n_chains, const int32_t* 
   kmax = need <= 64 ? 64 : (need <= 128 ? 128 : 160);
   if (model_api) kmax = 160;
   b->kmax = kmax;
+  uint64_t dr_max = 0;
   for (int c = 0; c < n_chains; ++c) {
     const bmg_opts& o = opts[c];
     if (chain_ctx[c] < 0 || chain_ctx[c] >= n_ctx) { delete b; set_error("bad chain_ctx"); return BMG_E_ARG; }
     bmg_ctx* x = ctxs[chain_ctx[c]];
     if (o.sampler_type != 0) { delete b; set_error("only the PMV sampler runs on the device"); return BMG_E_UNSUPPORTED; }
-    if (o.delay_rejection != 0) { delete b; set_error("delay_rejection > 0 is not built for the device"); return BMG_E_UNSUPPORTED; }
     if (!model_api && (o.n_types != 1 || x->n_types != 1 || o.types[0] != x->types[0])) {
       delete b; set_error("the device sampler needs exactly one effect type, equal to the dataset's");
       return BMG_E_UNSUPPORTED;
@@ -708,6 +708,13 @@ static int batch_create(bmg_ctx** ctxs, int n_ctx, int n_chains, const int32_t* 
     }
     uint64_t mm = std::min<uint64_t>(std::min<uint64_t>(x->p, 255), o.max_move_size);
     if (mm > BMG_MAXMS || mm < 1) { delete b; set_error("max_move_size must be in 1..64 on the device"); return BMG_E_UNSUPPORTED; }
+    // the second stage scores 2^dr sub-models (src/sampler.hpp:163,272-281)
+    const uint64_t dr = std::min<uint64_t>(mm, o.delay_rejection);
+    if (dr > BMG_DR_MAX) {
+      delete b; set_error("min(delay_rejection, max_move_size) must be at most 20 on the device");
+      return BMG_E_UNSUPPORTED;
+    }
+    dr_max = std::max(dr_max, dr);
   }
   for (int i = 0; i < n_ctx; ++i) {
     b->h_data.push_back(make_devdata(ctxs[i]));
@@ -738,6 +745,7 @@ static int batch_create(bmg_ctx** ctxs, int n_ctx, int n_chains, const int32_t* 
          o_trl = add(8 * tr_cap);
   size_t o_ths = add(4 * th_cap), o_thl = add(4 * th_loci_cap), o_thll = add(8 * th_cap),
          o_ths2 = add(8 * th_cap), o_tha = add(8 * th_cap), o_thp = add(8 * 3 * th_cap);
+  size_t o_dr = add(dr_max ? (size_t)8 << dr_max : 0);
   b->per_chain = sz;
   hipError_t e = hipMalloc(&b->pool, sz * n_chains);
   if (e != hipSuccess) { bmg_batch_destroy(b); return hip_fail(e, "hipMalloc(chain pool)"); }
@@ -762,6 +770,7 @@ static int batch_create(bmg_ctx** ctxs, int n_ctx, int n_chains, const int32_t* 
     A.th_size = (uint32_t*)(base + o_ths); A.th_loci = (uint32_t*)(base + o_thl);
     A.th_ll = (double*)(base + o_thll); A.th_s2 = (double*)(base + o_ths2); A.th_alpha = (double*)(base + o_tha);
     A.th_pve = (double*)(base + o_thp);
+    A.dr_lp = dr_max ? (double*)(base + o_dr) : nullptr;
     b->h_arrays.push_back(A);
     // constants (src/sampler.hpp:107-203, src/prior.hpp:45-122)
     ChainConst Kc{};
@@ -769,6 +778,7 @@ static int batch_create(bmg_ctx** ctxs, int n_ctx, int n_chains, const int32_t* 
     Kc.thin = o.thin; Kc.n_rao = o.n_rao; Kc.n_st = o.n_sample_tau2_and_missing;
     Kc.max_ms = (uint32_t)std::min<uint64_t>(std::min<uint64_t>(x->p, 255), o.max_move_size);
     Kc.max_nbh = o.max_SNP_neighborhood_size;
+    Kc.dr = (int32_t)std::min<uint64_t>(Kc.max_ms, o.delay_rejection);
     Kc.adapt_p_ms = o.adapt_p_move_size && Kc.max_ms > 1;
     Kc.adaptation = o.adaptation; Kc.flat = o.flat_proposal_dist;
     Kc.trace_iter = b->cfg.trace_every_iter;

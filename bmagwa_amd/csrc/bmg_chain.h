@@ -7,6 +7,8 @@
 #define BMG_KMAXG 256    // largest compiled model capacity (columns)
 #define BMG_TREE_LEV 8
 #define BMG_LH_MAX 32    // largest learned-swap window
+#define BMG_DR_MAX 20    // largest delayed-rejection move size (2^20 sub-models)
+#define BMG_DR_COLS (2 * BMG_DR_MAX)   // columns of the toggled loci (AH: two each)
 
 namespace bmg {
 
@@ -47,6 +49,7 @@ struct ChainConst {
   double inv_tau2_e0;
   uint64_t tr_cap, th_cap, th_loci_cap;
   double p_learned;      // probability of a learned swap per iteration
+  int32_t dr;            // delayed-rejection move-size limit (0: none)
 };
 
 // Mutable per-chain scalars (device global memory).
@@ -96,6 +99,7 @@ struct ChainArrays {
   uint8_t* tr_move; uint8_t* tr_jd; uint8_t* tr_ms; uint32_t* tr_size; double* tr_ll;
   uint32_t* th_size; uint32_t* th_loci; double* th_ll; double* th_s2; double* th_alpha;
   double* th_pve;        // 3 per thinned sample (src/model.hpp:345-392)
+  double* dr_lp;         // 2^dr second-stage log-probabilities (null: dr == 0)
 };
 
 // error flags of a chain; every flag from ERR_NONFINITE on stops the chain

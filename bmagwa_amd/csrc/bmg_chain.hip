@@ -78,6 +78,15 @@ struct Smem {
   int32_t win[512];                      // window of the current model_inds
   uint64_t slots_cur[(KMAX + 63) / 64], slots_prop[(KMAX + 63) / 64];
   int32_t iacc[KMAX + 1];                // integer column dots of new_column_dots
+  // delayed rejection: factor block, v and d of the toggled loci's columns,
+  // their column offsets and types, the walk's order and the moved SNPs'
+  // proposal weights
+  double ex_L[BMG_DR_COLS * (BMG_DR_COLS + 1) / 2];
+  double ex_v[BMG_DR_COLS], ex_d[BMG_DR_COLS];
+  double dq_add[BMG_DR_MAX], dq_rem[BMG_DR_MAX];
+  int32_t ex_x1[BMG_DR_MAX], ex_x2[BMG_DR_MAX];
+  int8_t ex_t[BMG_DR_MAX];
+  uint8_t ex_ci[BMG_DR_MAX + 2], ex_b2n[BMG_DR_MAX];
   double red[8];
   long long redi[8];
   double bcd[4];
@@ -1210,7 +1219,446 @@ __device__ __forceinline__ void undo_dd(Ctx<KMAX>& C, int ms) {
   }
 }
 
-// src/sampler.hpp:899-1123 without delayed rejection
+// ------------------------------------------------ delayed rejection -----
+// ExhModel (src/model.hpp:585-844) on the proposed model once every moved SNP
+// is in it: loci from position `cl` on are the ones the second stage toggles,
+// and their columns c0..cols-1 form the trailing block.  The walk's adjacent
+// swaps only permute the rows above c0 of those columns, and its
+// log-likelihoods read only the block's diagonal, v and d, so the block (in
+// indices relative to c0) is all that is copied.  One thread walks the
+// 2^ms sub-models; the working model itself is left as it is.
+struct Exh {
+  int ms;            // loci of the block in the walk's model (model_size - cl)
+  int vs;            // columns of the block in it (v_size - c0)
+  int nT;            // columns of the block
+  int Ns[5];
+  double syx, ldq, ldl, lmp;
+};
+
+template <int KMAX>
+__device__ __forceinline__ double& exl(Smem<KMAX>& S, int r, int c) { return S.ex_L[P(c) + r]; }
+
+// src/model.hpp:673-701
+template <int KMAX>
+__device__ __forceinline__ double exh_add(Smem<KMAX>& S, const ChainConst& K, Exh& X, int cl) {
+  const int type = S.ex_t[X.ms];
+  const int ncol = type == 4 ? 2 : 1;
+  for (int q = 0; q < ncol; ++q) {
+    const int s = X.vs;
+    X.syx -= S.ex_v[s] * S.ex_v[s];
+    X.ldq += 0.5 * log(S.ex_d[s]);
+    X.ldl += log(exl(S, s, s));
+    X.vs = s + 1;
+  }
+  const double ll = X.ldq - X.ldl + K.minus_n_plus_nu_2 * log(X.syx);
+  X.lmp += prior_add(K, X.Ns, cl + X.ms, type);
+  put(X.Ns, type, sel(X.Ns, type) + 1);
+  ++X.ms;
+  return ll + X.lmp;
+}
+
+// src/symmmatrix.cpp:220-265 on the block (columns s, s+1 relative to c0)
+template <int KMAX>
+__device__ __forceinline__ void exh_swapadj(Smem<KMAX>& S, int nT, int s) {
+  for (int r = 0; r < s; ++r) {
+    const double t = exl(S, r, s);
+    exl(S, r, s) = exl(S, r, s + 1);
+    exl(S, r, s + 1) = t;
+  }
+  double a = exl(S, s, s + 1), b = exl(S, s + 1, s + 1);
+  exl(S, s, s + 1) = exl(S, s, s);
+  exl(S, s + 1, s + 1) = 0.0;
+  double c, sn;
+  drotg(a, b, c, sn);
+  if (a < 0) { a = -a; sn = -sn; c = -c; }
+  exl(S, s, s) = a;
+  for (int i = s + 1; i < nT; ++i) {
+    const double x1 = exl(S, s, i), x2 = exl(S, s + 1, i);
+    const double tmp = x1 * sn - x2 * c;
+    exl(S, s, i) = x2 * sn + x1 * c;
+    exl(S, s + 1, i) = tmp;
+  }
+  const double x1 = S.ex_v[s], x2 = S.ex_v[s + 1];
+  const double tmp = x1 * sn - x2 * c;
+  S.ex_v[s] = x2 * sn + x1 * c;
+  S.ex_v[s + 1] = tmp;
+}
+
+__device__ __forceinline__ void swap_d(double& a, double& b) { const double t = a; a = b; b = t; }
+
+// src/model.hpp:703-832: the last locus of the walk's model moves one
+// position left and the locus it passes leaves the model
+template <int KMAX>
+__device__ __forceinline__ double exh_moveleft(Smem<KMAX>& S, const ChainConst& K, Exh& X, int cl) {
+  --X.ms;
+  const int ik = X.ms, ir = X.ms - 1;
+  const int tk = S.ex_t[ik], tr = S.ex_t[ir];
+  double* v = S.ex_v;
+  double* dv = S.ex_d;
+  int32_t* x1 = S.ex_x1;
+  int32_t* x2 = S.ex_x2;
+  const int nT = X.nT;
+  --X.vs;
+  X.syx += v[X.vs] * v[X.vs];
+  X.ldl -= log(exl(S, X.vs, X.vs));
+  if (tk == 4) {
+    --X.vs;
+    X.syx += v[X.vs] * v[X.vs];
+    X.ldl -= log(exl(S, X.vs, X.vs));
+  }
+  --X.vs;
+  X.syx += v[X.vs] * v[X.vs];
+  X.ldq -= 0.5 * log(dv[X.vs]);
+  X.ldl -= log(exl(S, X.vs, X.vs));
+  if (tr == 4) {
+    --X.vs;
+    X.syx += v[X.vs] * v[X.vs];
+    X.ldq -= 0.5 * log(dv[X.vs]);
+    X.ldl -= log(exl(S, X.vs, X.vs));
+  }
+  S.ex_t[ik] = (int8_t)tr;
+  S.ex_t[ir] = (int8_t)tk;
+  if (tr == 4) {
+    if (tk == 4) {
+      int si = x2[ir];
+      exh_swapadj(S, nT, si); ++si;
+      exh_swapadj(S, nT, si); si -= 2;
+      exh_swapadj(S, nT, si); ++si;
+      exh_swapadj(S, nT, si);
+      swap_d(dv[x1[ik]], dv[x2[ir]]); swap_d(dv[x1[ik]], dv[x2[ik]]);
+      swap_d(dv[x1[ir]], dv[x2[ir]]); swap_d(dv[x1[ik]], dv[x2[ir]]);
+    } else {
+      int si = x2[ir];
+      exh_swapadj(S, nT, si); --si;
+      exh_swapadj(S, nT, si);
+      swap_d(dv[x2[ir]], dv[x1[ik]]); swap_d(dv[x1[ir]], dv[x2[ir]]);
+      x2[ik] = x1[ik];
+      --x1[ik];
+    }
+  } else {
+    if (tk == 4) {
+      int si = x1[ir];
+      exh_swapadj(S, nT, si); ++si;
+      exh_swapadj(S, nT, si);
+      swap_d(dv[x1[ir]], dv[x1[ik]]); swap_d(dv[x1[ik]], dv[x2[ik]]);
+      x2[ir] = x1[ik];
+      x1[ik] = x2[ik];
+    } else {
+      exh_swapadj(S, nT, x1[ir]);
+      swap_d(dv[x1[ik]], dv[x1[ir]]);
+    }
+  }
+  X.syx -= v[X.vs] * v[X.vs];
+  X.ldl += log(exl(S, X.vs, X.vs));
+  ++X.vs;
+  if (tk == 4) {
+    X.syx -= v[X.vs] * v[X.vs];
+    X.ldl += log(exl(S, X.vs, X.vs));
+    ++X.vs;
+  }
+  const double ll = X.ldq - X.ldl + K.minus_n_plus_nu_2 * log(X.syx);
+  X.lmp += prior_rem(K, X.Ns, cl + X.ms + 1, tr);
+  put(X.Ns, tr, sel(X.Ns, tr) - 1);
+  return ll + X.lmp;
+}
+
+// The walk over all 2^n sub-models (src/sampler.cpp:882-980): additions,
+// left moves and "two new + swap" steps in the reference's order, so that
+// every log-probability is reached by the same sequence of updates.  Thread
+// 0 only; returns the largest log-probability.
+template <int KMAX>
+__device__ __noinline__ double exh_walk(Smem<KMAX>& S, const ChainConst& K, Exh X, int cl, int n,
+                                        double ll0, double* lp) {
+  uint8_t* ci = S.ex_ci;
+  for (int i = 0; i < n; ++i) ci[i] = (uint8_t)i;
+  uint64_t cmb = 0, tmp;
+  int msz = 0;
+  double mx = ll0;
+  lp[0] = ll0;
+  for (int i = 0; i < n; ++i) {
+    ++msz;
+    if (i > 1) { ++msz; exh_add(S, K, X, cl); }
+    cmb = (1ull << msz) - 1;
+    double v = exh_add(S, K, X, cl);
+    lp[cmb] = v;
+    if (v > mx) mx = v;
+    for (int j = 0; j < i; ++j) {
+      --msz;
+      tmp = ci[msz - 1]; ci[msz - 1] = ci[msz]; ci[msz] = (uint8_t)tmp;
+      cmb &= ~(1ull << tmp);
+      v = exh_moveleft(S, K, X, cl);
+      lp[cmb] = v;
+      if (v > mx) mx = v;
+    }
+    const uint64_t nmodels = (1ull << i) - i - 1;
+    uint64_t j = 0, nK = 0;
+    int do_lefts = 0;
+    while (j < nmodels) {
+      if (do_lefts < 2) {
+        ++msz;
+        tmp = ci[msz]; ci[msz] = ci[msz - 1]; ci[msz - 1] = (uint8_t)tmp;
+        cmb |= 1ull << tmp;
+        exh_add(S, K, X, cl);
+        exh_add(S, K, X, cl);
+        v = exh_moveleft(S, K, X, cl);
+        lp[cmb] = v;
+        if (v > mx) mx = v;
+        ++j; ++do_lefts;
+      } else {
+        ++nK;
+        const int kb = __builtin_ctzll(nK);
+        for (int kk = 0; kk <= kb; ++kk) {
+          --msz;
+          tmp = ci[msz - 1]; ci[msz - 1] = ci[msz]; ci[msz] = (uint8_t)tmp;
+          cmb &= ~(1ull << tmp);
+          v = exh_moveleft(S, K, X, cl);
+          lp[cmb] = v;
+          if (v > mx) mx = v;
+          ++j;
+        }
+        do_lefts = 0;
+      }
+    }
+  }
+  return mx;
+}
+
+// src/model.hpp:602-671 (ExhModel::update_to_model) and the walk: fills
+// lp[0 .. 2^n) with the log posterior (up to a constant) of every sub-model,
+// bit i of the index standing for the proposed model's locus cl + i.  All
+// threads; returns the largest entry.
+template <int KMAX>
+__device__ __forceinline__ double exhaustive_modelset(Ctx<KMAX>& C, int cl, int n, double* lp) {
+  Smem<KMAX>& S = C.S;
+  Reg& R = C.R;
+  int c0 = 1;   // the intercept column
+  if (cl > 0) c0 = (C.D.types[S.tind[cl - 1]] == 4 ? S.xi2[cl - 1] : S.xi1[cl - 1]) + 1;
+  const int nT = R.cols - c0;
+  for (int c = 0; c < nT; ++c)
+    for (int r = threadIdx.x; r <= c; r += BMG_BLOCK) S.ex_L[P(c) + r] = S.L[P(c0 + c) + c0 + r];
+  for (int i = threadIdx.x; i < nT; i += BMG_BLOCK) { S.ex_v[i] = S.v[c0 + i]; S.ex_d[i] = S.d[c0 + i]; }
+  for (int q = threadIdx.x; q < R.nloci - cl; q += BMG_BLOCK) {
+    S.ex_x1[q] = S.xi1[cl + q] - c0;
+    S.ex_x2[q] = S.xi2[cl + q] - c0;
+    S.ex_t[q] = (int8_t)C.D.types[S.tind[cl + q]];
+  }
+  double vv = 0.0, ldl = 0.0, ldq = 0.0;
+  for (int i = threadIdx.x; i < c0; i += BMG_BLOCK) {
+    vv += S.v[i] * S.v[i];
+    ldl += log(S.L[P(i) + i]);
+    if (i >= 1) ldq += log(S.d[i]);
+  }
+  vv = block_sum(vv, S.red);
+  ldl = block_sum(ldl, S.red);
+  ldq = block_sum(ldq, S.red);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    Exh X;
+    X.ms = 0; X.vs = 0; X.nT = nT;
+    for (int q = 0; q < 5; ++q) X.Ns[q] = 0;
+    for (int q = 0; q < cl; ++q) { const int t = C.D.types[S.tind[q]]; put(X.Ns, t, sel(X.Ns, t) + 1); }
+    X.syx = C.K.nus2_plus_yy - vv;
+    X.ldl = ldl;
+    X.ldq = 0.5 * ldq;
+    X.lmp = 0.0;
+    const double ll0 = X.ldq - X.ldl + C.K.minus_n_plus_nu_2 * log(X.syx);
+    S.bcd[0] = exh_walk(S, C.K, X, cl, n, ll0, lp);
+  }
+  __syncthreads();
+  const double mx = S.bcd[0];
+  __syncthreads();
+  return mx;
+}
+
+// src/sampler.cpp:982-1049 with one effect type: adds the log proposal
+// probability of reaching each sub-model by the multi-step add/remove move
+// from the current model to lp[i].  Thread per sub-model; the terms of one
+// sub-model are summed in the reference's order.
+template <int KMAX>
+__device__ __forceinline__ void dr_proposal_probs(Smem<KMAX>& S, int n, double z_add, double z_rem,
+                                                  uint64_t cl, uint64_t m_g, double* lp) {
+  const uint64_t nmodels = 1ull << n;
+  for (uint64_t i = threadIdx.x; i < nmodels; i += BMG_BLOCK) {
+    uint64_t max_adds = m_g - cl, max_rems = cl;
+    double za = z_add, zr = z_rem;
+    uint64_t isadd = 0;
+    for (int j = 0; j < n; ++j) {
+      const int nind = S.ex_b2n[j];
+      if ((i >> j) & 1) {
+        za -= S.dq_add[nind]; zr += S.dq_rem[nind];
+        ++max_rems; --max_adds;
+      } else {
+        isadd |= 1ull << nind;
+      }
+    }
+    double acc = lp[i];
+    int lrp = n - 1;
+    for (int j = 0; j < n; ++j) {
+      if (max_adds > 0 && max_rems > 0) acc += LOG_HALF;
+      if ((isadd >> j) & 1) {
+        --max_adds;
+        acc += log(S.dq_add[j]) - log(za);
+        za -= S.dq_add[j];
+      } else {
+        --max_rems;
+        while ((isadd >> lrp) & 1) --lrp;
+        acc += log(S.dq_rem[lrp]) - log(zr);
+        zr -= S.dq_rem[lrp];
+        --lrp;
+      }
+    }
+    lp[i] = acc;
+  }
+}
+
+// Second-stage draw (src/sampler.hpp:1008-1028 and 1592-1612): pairs each
+// sub-model with its complement, keeps the better of the two with the
+// probability of the difference, and draws one pair from the cumulative
+// weights (src/utils.cpp:73-100 with `ms/2 - 3` bisection steps).  The
+// pairwise transform runs a thread per pair; thread 0 sums in order.
+template <int KMAX>
+__device__ __forceinline__ uint64_t dr_second_stage(Ctx<KMAX>& C, int n, double max_log, double* p) {
+  const uint64_t nmodels = 1ull << n, mask = nmodels - 1, half = nmodels / 2;
+  for (uint64_t i = threadIdx.x; i < half; i += BMG_BLOCK) {
+    const uint64_t j = (~i) & mask;
+    const double pi = p[i], pj = p[j];
+    const double a = pj - pi;
+    if (a > 0) { p[i] = exp(pj - max_log) * (1 - exp(-a)); p[j] = -1.0; }
+    else { p[i] = exp(pi - max_log) * (1 - exp(a)); p[j] = 1.0; }
+  }
+  __syncthreads();
+  const double u = rng_u01(C.R.rng);
+  if (threadIdx.x == 0) {
+    double sum = 0.0;
+    for (uint64_t i = 0; i < half; ++i) { const double v = p[i] + sum; p[i] = v; sum = v; }
+    int level = n / 2 - 3;
+    uint64_t a = 0, b = half - 1;
+    const double r = u * p[b];
+    while (level > 0) {
+      const uint64_t c = (a + b) / 2;
+      if (r < p[c]) b = c; else a = c + 1;
+      --level;
+    }
+    uint64_t sm = half - 1;
+    for (uint64_t i = a; i <= b; ++i) if (r < p[i]) { sm = i; break; }
+    if (p[(~sm) & mask] < 0) sm = (~sm) & mask;
+    C.S.bci[0] = (long long)sm;
+  }
+  __syncthreads();
+  const uint64_t sm = (uint64_t)C.S.bci[0];
+  __syncthreads();
+  return sm;
+}
+
+// the second stage re-adds the SNPs the first stage removed, each with the
+// prior precision it has in the current model (src/sampler.hpp:931-960);
+// false when the model capacity does not allow it
+template <int KMAX>
+__device__ __forceinline__ bool readd_removed(Ctx<KMAX>& C, uint64_t ind) {
+  Smem<KMAX>& S = C.S;
+  const int mi = C.A.model_inds[ind];
+  const int t_ind = S.c_tind[mi];
+  double d2[2];
+  d2[0] = S.c_d[S.c_xi1[mi]];
+  d2[1] = C.D.types[t_ind] == 4 ? S.c_d[S.c_xi2[mi]] : 0.0;
+  add_term(C, (uint32_t)ind, t_ind, d2);
+  return !(C.R.err & ERR_CAPACITY);
+}
+
+// removes the proposed model's loci cl + i whose bit of `sm` is clear
+// (src/sampler.hpp:1081-1091)
+template <int KMAX>
+__device__ __forceinline__ void dr_keep(Ctx<KMAX>& C, int cl, int n, uint64_t sm) {
+  uint64_t removals = sm & ((1ull << n) - 1);
+  int c = cl;
+  for (int i = 0; i < n; ++i) {
+    if ((removals & 1) == 0) { remove_term(C, c + i); --c; }
+    removals >>= 1;
+  }
+}
+
+// Second stage of the multi-step add/remove move after a first-stage
+// rejection (src/sampler.hpp:931-1099): the removed SNPs are re-added, every
+// sub-model between the current and the proposed one is scored with its
+// proposal probability, and one is drawn.  Returns the number of moved SNPs
+// kept (0: the current model stays).
+template <int KMAX>
+__device__ __forceinline__ int dr_addrem(Ctx<KMAX>& C, int ms, double* adapt) {
+  Smem<KMAX>& S = C.S;
+  Reg& R = C.R;
+  int ms_rem = 0;
+  for (int i = 0; i < ms; ++i) ms_rem += S.mi_isadd[i] ? 0 : 1;
+  const int cl = R.c_nloci - ms_rem;
+  const int ms_add = ms - ms_rem;
+  double z_add = R.tot_add, z_rem = R.tot_rem;
+  const uint64_t nmb = (1ull << ms_add) - 1, mask = (1ull << ms) - 1;
+  bool ok = true;
+  for (int i = 0; i < ms && ok; ++i) {
+    const uint64_t ind = S.mi[i];
+    if (S.mi_isadd[i]) { z_add += C.A.q_add[ind]; z_rem -= C.A.q_rem[ind]; }
+    else ok = readd_removed(C, ind);
+  }
+  if (!ok) { restore(C); undo_dd(C, ms); return 0; }
+  for (int i = 0; i < ms; ++i) {
+    const uint64_t ind = (uint64_t)S.mi_map[i];
+    const int pos = find_locus(C, ind);
+    if (threadIdx.x == 0 && pos >= cl && pos - cl < ms) {
+      S.ex_b2n[pos - cl] = (uint8_t)i;
+      S.dq_add[i] = C.A.q_add[ind];
+      S.dq_rem[i] = C.A.q_rem[ind];
+    }
+  }
+  __syncthreads();
+  double* lp = C.A.dr_lp;
+  const double max_log = exhaustive_modelset(C, cl, ms, lp);
+  dr_proposal_probs(S, ms, z_add, z_rem, (uint64_t)cl, C.D.p, lp);
+  __syncthreads();
+  const uint64_t sm = dr_second_stage(C, ms, max_log, lp);
+  if (sm == ((~nmb) & mask)) { restore(C); undo_dd(C, ms); return 0; }
+  int moved = 0;
+  for (int i = 0; i < ms; ++i) {
+    const int bit = (int)((sm >> i) & 1), nb = (int)((nmb >> i) & 1);
+    if (bit == nb) { ++moved; continue; }
+    const uint64_t ind = (uint64_t)S.mi_map[S.ex_b2n[i]];
+    if (bit) { tree_set(C, 0, ind, 1); tree_set(C, 1, ind, 0); }
+    else { tree_set(C, 0, ind, 0); tree_set(C, 1, ind, 1); }
+  }
+  dr_keep(C, cl, ms, sm);
+  if (threadIdx.x == 0) adapt[3 * 256 + ms - 1] += (double)moved / (double)ms;
+  commit(C);
+  return moved;
+}
+
+// Second stage of the state-change move (src/sampler.hpp:1547-1667): as
+// above, scored by model size instead of proposal probabilities.
+template <int KMAX>
+__device__ __forceinline__ int dr_statechange(Ctx<KMAX>& C, int ms, int ms_add, int ms_rem) {
+  Smem<KMAX>& S = C.S;
+  Reg& R = C.R;
+  const int cl = R.c_nloci - ms_rem;
+  const uint64_t nmb = (1ull << ms_add) - 1, mask = (1ull << ms) - 1;
+  for (int i = 0; i < ms_rem; ++i)
+    if (!readd_removed(C, S.mi_rem[i])) { restore(C); return 0; }
+  double* lp = C.A.dr_lp;
+  const double max_log = exhaustive_modelset(C, cl, ms, lp);
+  for (uint64_t i = threadIdx.x; i <= mask; i += BMG_BLOCK) lp[i] -= log((double)(cl + __popcll(i)));
+  __syncthreads();
+  const uint64_t sm = dr_second_stage(C, ms, max_log, lp);
+  if (sm == ((~nmb) & mask)) { restore(C); return 0; }
+  int moved = 0;
+  for (int i = 0; i < ms; ++i) {
+    const int bit = (int)((sm >> i) & 1), nb = (int)((nmb >> i) & 1);
+    if (bit != nb) continue;
+    ++moved;
+    if (bit) { const uint64_t ind = S.mi_add[i]; tree_set(C, 0, ind, 1); tree_set(C, 1, ind, 0); }
+    else { const uint64_t ind = S.mi_rem[i - ms_add]; tree_set(C, 0, ind, 0); tree_set(C, 1, ind, 1); }
+  }
+  dr_keep(C, cl, ms, sm);
+  commit(C);
+  return moved;
+}
+
+// src/sampler.hpp:899-1123
 template <int KMAX>
 __device__ __forceinline__ int move_addrem(Ctx<KMAX>& C, double* adapt) {
   Smem<KMAX>& S = C.S;
@@ -1297,13 +1745,16 @@ __device__ __forceinline__ int move_addrem(Ctx<KMAX>& C, double* adapt) {
   double* r_ms_sum = adapt + 3 * 256;
   double* r_ms_n = adapt + 4 * 256;
   if (threadIdx.x == 0) {
-    r_ms_sum[ms - 1] += (log_r >= 0 ? 1.0 : exp(log_r));
+    if (K.dr == 0) r_ms_sum[ms - 1] += (log_r >= 0 ? 1.0 : exp(log_r));
     r_ms_n[ms - 1] += 1.0;
   }
   if (log_r >= 0 || log(rng_u01(R.rng)) <= log_r) {
     commit(C);
+    if (K.dr != 0 && threadIdx.x == 0) r_ms_sum[ms - 1] += 1.0;
     return ms;
   }
+  // a proposal past the compiled capacity has no second stage
+  if (ms <= K.dr && ms > 1 && !(R.err & ERR_CAPACITY)) return dr_addrem(C, ms, adapt);
   restore(C);
   undo_dd(C, ms);
   return 0;
@@ -1504,7 +1955,7 @@ __device__ __forceinline__ int move_learned_swap(Ctx<KMAX>& C) {
   return 0;
 }
 
-// src/sampler.hpp:1421-1669 without delayed rejection
+// src/sampler.hpp:1421-1669
 template <int KMAX>
 __device__ __forceinline__ int move_statechange(Ctx<KMAX>& C, double* adapt) {
   Smem<KMAX>& S = C.S;
@@ -1584,6 +2035,7 @@ __device__ __forceinline__ int move_statechange(Ctx<KMAX>& C, double* adapt) {
     for (int i = 0; i < ms_rem; ++i) { tree_set(C, 0, S.mi_rem[i], 0); tree_set(C, 1, S.mi_rem[i], 1); }
     return ms;
   }
+  if (ms <= K.dr && ms > 1 && !(R.err & ERR_CAPACITY)) return dr_statechange(C, ms, ms_add, ms_rem);
   restore(C);
   return 0;
 }

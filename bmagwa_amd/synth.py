@@ -66,7 +66,8 @@ def phenotype(decode_causal, n, p, n_causal, h2, seed, dominance=False):
 def sampler_cfg(n, p, n_causal, types="A", seed=1, chain=0, **over):
     """Plain PMV settings of the benchmark: e_qg = #causal,
     var_qg = #causal^2, the reference example's move settings, no delayed
-    rejection on the device."""
+    rejection unless `delay_rejection` is given (the reference example uses
+    10)."""
     e_qg = max(float(n_causal), 1.0)
     var_qg = max(float(n_causal) ** 2, 2.0 * e_qg)
     cfg = dict(n=n, m_g=p, m_e=0, types=types, seed=seed, chain=chain,

@@ -201,8 +201,9 @@ int bmg_model_set_d(bmg_model* m, int x_ind, double val);
  * FP64 model in LDS, all-SNP RB scans between segments.  chain_ctx[c] picks
  * the dataset of chain c (phenotypes with different individual sets are
  * separate datasets).  opts[c] are the options of chain c (seed = its
- * reference seed).  Unsupported on the device: delay_rejection > 0,
- * multiple effect types, covariates, missing genotypes. */
+ * reference seed).  Delayed rejection (src/sampler.cpp:882-1049) runs on
+ * the device for min(delay_rejection, max_move_size) <= 20.  Unsupported on
+ * the device: multiple effect types, covariates, missing genotypes. */
 typedef struct {
   int32_t trace_every_iter;    /* keep per-iteration move/jd/size/loglik */
   int32_t proposal;            /* 0 plain, 1 plain + learned swap (online) */

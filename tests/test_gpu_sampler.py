@@ -65,6 +65,32 @@ def test_trajectory_parity_single_chain(bmg, types, ld, use_ind):
     assert np.array_equal(np.sort(b.loci(0)), np.sort(s.loci()))
 
 
+@pytest.mark.parametrize("types,ld,mms,dr", [("A", (0.9, 0.9, 10, 30), 8, 8), ("A", (0.8, 0.95, 5, 20), 10, 4),
+                                            ("AH", (0.8, 0.9, 10, 20), 6, 6)])
+def test_trajectory_parity_delayed_rejection(bmg, types, ld, mms, dr):
+    """Delayed rejection on the device (second stage of add/remove and
+    state-change moves over all 2^ms sub-models): the trajectory is the
+    oracle's, including the moves the second stage accepts, with the
+    move-size limit equal to and above the delayed-rejection limit."""
+    n, p, n_iter = 300, 400, 3000
+    body, y, causal = dataset(n, p, ld, 5, 4321, types)
+    cfg = synth.sampler_cfg(n, p, 5, types=types, seed=91, n_rao=100, n_rao_burnin=5,
+                            thin=10, n_sample_tau2_and_missing=10, max_move_size=mms,
+                            delay_rejection=dr, do_n_iter=n_iter)
+    s = oracle_run(cfg, body, y, n_iter)
+    ctx = bmg.Ctx(n, p, body, y, types=(types,))
+    b = bmg.Batch([ctx], [0], [cfg], dict(trace_every_iter=1))
+    b.setup()
+    b.run(n_iter)
+    tr = b.trace(0)
+    compare(tr, s.trace(), n_iter)
+    # the second stage did run: accepted moves that kept fewer SNPs than drawn
+    part = (tr["jd"] > 0) & (tr["jd"] < tr["ms"]) & (tr["move"] != 1)
+    assert part.sum() > 0
+    assert np.allclose(b.p_rao(0), s.p_rao(), rtol=1e-8, atol=1e-10)
+    assert np.array_equal(np.sort(b.loci(0)), np.sort(s.loci()))
+
+
 def test_batch_of_chains_each_match_their_oracle_run(bmg):
     n, p, n_iter = 256, 300, 1500
     body, y, _ = dataset(n, p, (0.9, 0.9, 10, 30), 4, 99)
@@ -179,18 +205,22 @@ def test_output_files_in_the_reference_format(bmg, tmp_path):
     assert not (tmp_path / "run1_types.dat").exists()
 
 
-def test_reference_cpu_configuration_trajectory(bmg):
+@pytest.mark.parametrize("dr", [0, 10])
+def test_reference_cpu_configuration_trajectory(bmg, dr):
     """The reference's CPU-runnable workload shape: n = 1,000, p = 10,000
     independent SNPs (MAF ~ U(0.05, 0.5), Hardy-Weinberg), 10 causal SNPs,
-    h2 = 0.3, additive coding, one chain: the device reproduces the oracle's
-    trajectory (moves, jump distances, move sizes, model sizes identical;
-    log-likelihoods within 1e-10) over 3,000 iterations, and the RB PIPs."""
+    h2 = 0.3, additive coding, one chain, without delayed rejection and with
+    the reference example's delay_rejection = 10 (max_move_size = 20): the
+    device reproduces the oracle's trajectory (moves, jump distances, move
+    sizes, model sizes identical; log-likelihoods within 1e-10) over 3,000
+    iterations, and the RB PIPs."""
     n, p, n_iter = 1000, 10000, 3000
     thresh, rho, _ = synth.snp_layout(p, (0.05, 0.5), None, 2024)
     body = oracle.synth_genotypes(2024, n, p, thresh, rho)
     dec = lambda snps: np.stack([oracle.decode(body, n, int(j), 0) for j in snps])
     y, causal, _ = synth.phenotype(dec, n, p, 10, 0.3, 2024)
-    cfg = synth.sampler_cfg(n, p, 10, seed=5, n_rao=500, n_rao_burnin=2, thin=10, do_n_iter=n_iter)
+    cfg = synth.sampler_cfg(n, p, 10, seed=5, n_rao=500, n_rao_burnin=2, thin=10, do_n_iter=n_iter,
+                            delay_rejection=dr)
     ctx = bmg.Ctx(n, p, body, y)
     b = bmg.Batch([ctx], [0], [cfg], dict(trace_every_iter=1))
     b.setup()
