@@ -57,6 +57,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 N, P, N_CAUSAL, H2 = 10000, 500000, 100, 0.5
+DELAY_REJECTION = 0      # --delay-rejection: the sampler option of every run
 MAF, LD = (0.02, 0.5), (0.7, 0.95, 20, 100)
 SEED = 20261202 + 4000
 HBM_PEAK_GBS = 8000.0
@@ -89,7 +90,8 @@ def pooled_counts(th, first, p):
 def run_batch(bmg, synth, ctx, n_chains, steps, warmup, ips, seed0, batch_cfg=None, thin=1):
     total = (warmup + steps) * ips + 1000     # sizes the trace buffers
     cfgs = [synth.sampler_cfg(N, P, N_CAUSAL, types="A", seed=seed0 + c, chain=c,
-                              do_n_iter=total, n_rao_burnin=4, thin=thin)
+                              do_n_iter=total, n_rao_burnin=4, thin=thin,
+                              delay_rejection=DELAY_REJECTION)
             for c in range(n_chains)]
     bc = dict(batch_cfg or {})
     bc["trace_capacity"] = (total // thin + 16) * 200    # thinned loci: up to 200 per sample
@@ -200,7 +202,8 @@ def phase_profile(bmg, synth, ctx, args):
     ips, n = args.iters_per_step, args.chains
     total = (args.warmup + args.steps) * ips + 1000
     cfgs = [synth.sampler_cfg(N, P, N_CAUSAL, types="A", seed=100 + c, chain=c, do_n_iter=total,
-                              n_rao_burnin=4, thin=args.thin) for c in range(n)]
+                              n_rao_burnin=4, thin=args.thin, delay_rejection=DELAY_REJECTION)
+            for c in range(n)]
     b = bmg.Batch([ctx], [0] * n, cfgs, dict(proposal=0, use_graph=0,
                                               trace_capacity=(total // args.thin + 16) * 200))
     b.setup()
@@ -223,7 +226,8 @@ def phase_profile(bmg, synth, ctx, args):
                               refresh=us("tau2", it), refresh_draws=us("refresh_draws", it),
                               beta_sigma2=us("beta_sigma2", it), pve_thin=us("pve_thin", it),
                               addrem_moves=us("addrem", it), statechange=us("statechange", it),
-                              swap=us("swap", it))), flush=True)
+                              swap=us("swap", it), second_stage=us("dr", it),
+                              second_stage_scoring=us("dr_score", it))), flush=True)
     print(json.dumps(dict(kernel_us_per_iter=round((t1["chain_ms"] - t0["chain_ms"]) / it * 1e3, 1))), flush=True)
     b.close()
 
@@ -237,7 +241,7 @@ def cpu_baseline(ctx, y, monitored, seconds, thin):
     del body
     from bmagwa_amd import synth
     cfg = synth.sampler_cfg(N, P, N_CAUSAL, types="A", seed=9001, chain=0, do_n_iter=10 ** 8,
-                            n_rao_burnin=4, thin=thin)
+                            n_rao_burnin=4, thin=thin, delay_rejection=DELAY_REJECTION)
     s = oracle.Sampler(cfg, d)
     s.init_flat()
     s.setup()
@@ -285,6 +289,9 @@ def main():
     ap.add_argument("--thin", type=int, default=10,
                     help="thinning (part of the sampler's schedule: beta and sigma2 are drawn at "
                          "every thinned iteration, as in the reference)")
+    ap.add_argument("--delay-rejection", type=int, default=0,
+                    help="the sampler's delay_rejection option in every run (0: none; the "
+                         "reference example uses 10)")
     ap.add_argument("--plain-steps", type=int, default=60,
                     help="timed steps of the 1-chain plain run (long enough for its own ESS)")
     ap.add_argument("--full-steps", type=int, default=10,
@@ -294,6 +301,8 @@ def main():
     ap.add_argument("--pip-ref-steps", type=int, default=30,
                     help="timed steps of the PIP reference run")
     args = ap.parse_args()
+    global DELAY_REJECTION
+    DELAY_REJECTION = args.delay_rejection
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU only: chains and phenotypes are independent, so more "
@@ -445,6 +454,7 @@ def main():
         "config": {"workload": "PMV MCMC over SNP inclusion, n=10000 p=500000 AR(1) LD, "
                                "100 causal, h2=0.5, additive",
                    "chains": args.chains, "iters_per_step": ips,
+                   "delay_rejection": DELAY_REJECTION,
                    "proposal": ("plain moves + learned swap (MLP 16-256-256-1, bf16)" if args.learned
                                 else "plain PMV moves (learned swap not run at this size)")},
         "iters_per_s": lb["iters_per_s"],

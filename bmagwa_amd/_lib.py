@@ -55,13 +55,13 @@ class BatchCfg(C.Structure):
         ("learn_freeze_iter", C.c_int32), ("scan_digits", C.c_int32), ("use_graph", C.c_int32),
         ("trace_capacity", C.c_uint64),
         ("learn_window", C.c_int32), ("learn_steps", C.c_int32), ("learn_batch", C.c_int32),
-        ("learn_pad", C.c_int32), ("p_learned", C.c_double), ("learn_rate", C.c_double),
+        ("capacity", C.c_int32), ("p_learned", C.c_double), ("learn_rate", C.c_double),
     ]
 
 
 BATCH_CFG_DEFAULT = dict(trace_every_iter=0, proposal=0, learn_every=500, learn_freeze_iter=0,
                          scan_digits=5, use_graph=0, trace_capacity=0, learn_window=20,
-                         learn_steps=100, learn_batch=4096, learn_pad=0, p_learned=0.9,
+                         learn_steps=100, learn_batch=4096, capacity=0, p_learned=0.9,
                          learn_rate=1e-3)
 
 OPT_DEFAULTS = dict(max_move_size=20, max_SNP_neighborhood_size=10, p_move_size=0.2,
@@ -617,7 +617,7 @@ class Batch:
         check(lib().bmg_batch_profile(self.h, c, _p(o)))
         keys = ["tau2", "addrem", "swap", "statechange", "learned", "beta_sigma2", "pve_thin", "total",
                 "add_xy", "add_xtx", "add_trsv", "add_rest", "n_add", "n_rem", "rem", "refresh_draws",
-                "rem_downdate", "rem_solve", "unused18", "unused19"]
+                "rem_downdate", "rem_solve", "dr", "dr_score"]
         return dict(zip(keys, o))
 
     def timing(self):

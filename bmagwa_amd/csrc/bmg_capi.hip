@@ -492,7 +492,8 @@ static int launch_chain_kernel(bmg_batch* b, uint64_t n_iter, int mode) {
   switch (b->kmax) {
     case 64: hipLaunchKernelGGL(k_chains<64>, dim3(b->n_chains), dim3(256), 0, b->stream, a); break;
     case 128: hipLaunchKernelGGL(k_chains<128>, dim3(b->n_chains), dim3(256), 0, b->stream, a); break;
-    default: hipLaunchKernelGGL(k_chains<160>, dim3(b->n_chains), dim3(256), 0, b->stream, a); break;
+    case 160: hipLaunchKernelGGL(k_chains<160>, dim3(b->n_chains), dim3(256), 0, b->stream, a); break;
+    default: hipLaunchKernelGGL(k_chains<256>, dim3(b->n_chains), dim3(256), 0, b->stream, a); break;
   }
   BMG_HIP(hipGetLastError());
   return BMG_OK;
@@ -682,8 +683,15 @@ static int batch_create(bmg_ctx** ctxs, int n_ctx, int n_chains, const int32_t* 
     for (int t = 0; t < opts[c].n_types; ++t) if (opts[c].types[t] == BMG_AH) cols *= 2;
     need = std::max(need, cols + 1);
   }
-  kmax = need <= 64 ? 64 : (need <= 128 ? 128 : 160);
+  kmax = need <= 64 ? 64 : (need <= 128 ? 128 : (need <= 160 ? 160 : 256));
   if (model_api) kmax = 160;
+  if (b->cfg.capacity) {
+    if (b->cfg.capacity != 64 && b->cfg.capacity != 128 && b->cfg.capacity != 160 && b->cfg.capacity != 256) {
+      delete b; set_error("capacity must be 0, 64, 128, 160 or 256");
+      return BMG_E_ARG;
+    }
+    if (!model_api) kmax = b->cfg.capacity;
+  }
   b->kmax = kmax;
   uint64_t dr_max = 0;
   for (int c = 0; c < n_chains; ++c) {
@@ -739,6 +747,7 @@ static int batch_create(bmg_ctx** ctxs, int n_ctx, int n_chains, const int32_t* 
   size_t o_ta = add(8 * tmax), o_tr = add(8 * tmax);
   size_t o_pp = add(8 * pmax), o_pr = add(8 * pmax), o_prr = add(8 * pmax);
   size_t o_L = add(8 * (size_t)K * (K + 1) / 2), o_xx = add(8 * (size_t)K * K);
+  size_t o_Lw = add(K > 160 ? 8 * (size_t)K * (K + 1) / 2 : 0);
   size_t o_xc = add(4 * (size_t)K * wmax);
   size_t o_yh = add(8 * npmax), o_vec = add(8 * 5 * K), o_idx = add(4 * 8 * K), o_ad = add(8 * 6 * 256);
   size_t o_trm = add(tr_cap), o_trj = add(tr_cap), o_trs = add(tr_cap), o_trz = add(4 * tr_cap),
@@ -763,6 +772,7 @@ static int batch_create(bmg_ctx** ctxs, int n_ctx, int n_chains, const int32_t* 
     A.tree_add = (double*)(base + o_ta); A.tree_rem = (double*)(base + o_tr);
     A.p_proposal = (double*)(base + o_pp); A.p_rao = (double*)(base + o_pr); A.p_r = (double*)(base + o_prr);
     A.Lcur = (double*)(base + o_L); A.xxs = (double*)(base + o_xx); A.y_hat = (double*)(base + o_yh);
+    A.Lwork = K > 160 ? (double*)(base + o_Lw) : nullptr;
     A.xcol = (uint32_t*)(base + o_xc);
     A.vec = (double*)(base + o_vec); A.idx = (int32_t*)(base + o_idx); A.adapt = (double*)(base + o_ad);
     A.tr_move = base + o_trm; A.tr_jd = base + o_trj; A.tr_ms = base + o_trs;
@@ -1411,7 +1421,8 @@ static int launch_model_op(bmg_batch* b, ModelOpArgs a) {
   switch (b->kmax) {
     case 64: hipLaunchKernelGGL(k_model_op<64>, dim3(1), dim3(256), 0, b->stream, a); break;
     case 128: hipLaunchKernelGGL(k_model_op<128>, dim3(1), dim3(256), 0, b->stream, a); break;
-    default: hipLaunchKernelGGL(k_model_op<160>, dim3(1), dim3(256), 0, b->stream, a); break;
+    case 160: hipLaunchKernelGGL(k_model_op<160>, dim3(1), dim3(256), 0, b->stream, a); break;
+    default: hipLaunchKernelGGL(k_model_op<256>, dim3(1), dim3(256), 0, b->stream, a); break;
   }
   BMG_HIP(hipGetLastError());
   BMG_HIP(hipStreamSynchronize(b->stream));

@@ -65,7 +65,10 @@ struct ChainScal {
   // wall-clock ticks per phase: tau2/alpha refresh, add/remove, swap, state
   // change, learned swap, beta/sigma2, pve + thinning, whole iterations;
   // then inside additions: x.y, X^T x, triangular solve, the rest of the
-  // update, and the count of additions and removals, time in removals
+  // update, and the count of additions and removals, time in removals,
+  // tau2 draws of the refresh, the downdate and the solve of removals, whole
+  // delayed-rejection second stages and their scoring (walk, proposal
+  // probabilities, draw)
   uint64_t prof[20];
   uint32_t rb_calls;
   int32_t movesize;
@@ -90,6 +93,7 @@ struct ChainArrays {
   double* tree_add; double* tree_rem;    // tree_size each
   double* p_proposal; double* p_rao; double* p_r;  // p
   double* Lcur;              // KMAX (KMAX+1) / 2
+  double* Lwork;             // KMAX (KMAX+1) / 2 working factor when it is not in LDS (else null)
   double* xxs;               // KMAX x KMAX slot matrix
   uint32_t* xcol;            // KMAX x W32: packed genotypes of each slot's column
   double* y_hat;             // n_pad

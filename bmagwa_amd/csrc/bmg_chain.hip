@@ -55,9 +55,25 @@ __device__ __forceinline__ void put(T (&a)[N], int i, T v) {
 }
 
 // ------------------------------------------------------------- state -----
+// The proposed model's factor (packed upper): in LDS up to 160 columns; a
+// larger capacity keeps it in a per-chain global array (L2/HBM), whose
+// address thread 0 stores here at kernel entry.
+constexpr int KMAX_LDS = 160;
+template <int KMAX, bool IN_LDS = (KMAX <= KMAX_LDS)>
+struct FactorStore { double L[KMAX * (KMAX + 1) / 2]; };
 template <int KMAX>
-struct Smem {
-  double L[KMAX * (KMAX + 1) / 2];       // proposed model's factor (packed upper)
+struct FactorStore<KMAX, false> { double* L; };
+
+template <int KMAX>
+__device__ __forceinline__ void bind_factor(FactorStore<KMAX, true>&, double*) {}
+template <int KMAX>
+__device__ __forceinline__ void bind_factor(FactorStore<KMAX, false>& F, double* g) {
+  if (threadIdx.x == 0) F.L = g;
+  __syncthreads();
+}
+
+template <int KMAX>
+struct Smem : FactorStore<KMAX> {
   double xy[KMAX], v[KMAX], d[KMAX];     // proposed model vectors
   double beta[KMAX], mu[KMAX];           // the chain's beta / mu_beta
   double work[KMAX + 64];
@@ -78,15 +94,16 @@ struct Smem {
   int32_t win[512];                      // window of the current model_inds
   uint64_t slots_cur[(KMAX + 63) / 64], slots_prop[(KMAX + 63) / 64];
   int32_t iacc[KMAX + 1];                // integer column dots of new_column_dots
-  // delayed rejection: factor block, v and d of the toggled loci's columns,
-  // their column offsets and types, the walk's order and the moved SNPs'
-  // proposal weights
+  // delayed rejection: factor block, v, d and log d of the toggled loci's
+  // columns, their column offsets and types, and the moved SNPs' proposal
+  // weights
   double ex_L[BMG_DR_COLS * (BMG_DR_COLS + 1) / 2];
-  double ex_v[BMG_DR_COLS], ex_d[BMG_DR_COLS];
+  double ex_v[BMG_DR_COLS], ex_d[BMG_DR_COLS], ex_ld[BMG_DR_COLS];
+  double ex_pa[BMG_DR_MAX + 1], ex_pr[BMG_DR_MAX + 1];   // prior change by model size
   double dq_add[BMG_DR_MAX], dq_rem[BMG_DR_MAX];
   int32_t ex_x1[BMG_DR_MAX], ex_x2[BMG_DR_MAX];
   int8_t ex_t[BMG_DR_MAX];
-  uint8_t ex_ci[BMG_DR_MAX + 2], ex_b2n[BMG_DR_MAX];
+  uint8_t ex_b2n[BMG_DR_MAX];
   double red[8];
   long long redi[8];
   double bcd[4];
@@ -1225,78 +1242,99 @@ __device__ __forceinline__ void undo_dd(Ctx<KMAX>& C, int ms) {
 // and their columns c0..cols-1 form the trailing block.  The walk's adjacent
 // swaps only permute the rows above c0 of those columns, and its
 // log-likelihoods read only the block's diagonal, v and d, so the block (in
-// indices relative to c0) is all that is copied.  One thread walks the
-// 2^ms sub-models; the working model itself is left as it is.
+// indices relative to c0) is all that is copied; the working model is left
+// as it is.  Wave 0 walks the 2^ms sub-models: the scalar state is
+// replicated in its lanes, the swaps give each lane one row or column.
+// With one effect type the prior terms depend on the model size only and
+// are tabulated; log(d) is kept beside d and permuted with it.  Both give
+// the values the reference's expressions give.
 struct Exh {
   int ms;            // loci of the block in the walk's model (model_size - cl)
   int vs;            // columns of the block in it (v_size - c0)
   int nT;            // columns of the block
-  int Ns[5];
   double syx, ldq, ldl, lmp;
 };
 
 template <int KMAX>
 __device__ __forceinline__ double& exl(Smem<KMAX>& S, int r, int c) { return S.ex_L[P(c) + r]; }
 
+// orders wave 0's LDS accesses across its lanes
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // src/model.hpp:673-701
 template <int KMAX>
-__device__ __forceinline__ double exh_add(Smem<KMAX>& S, const ChainConst& K, Exh& X, int cl) {
-  const int type = S.ex_t[X.ms];
-  const int ncol = type == 4 ? 2 : 1;
+__device__ __forceinline__ double exh_add(Smem<KMAX>& S, const ChainConst& K, Exh& X) {
+  const int ncol = S.ex_t[X.ms] == 4 ? 2 : 1;
   for (int q = 0; q < ncol; ++q) {
     const int s = X.vs;
     X.syx -= S.ex_v[s] * S.ex_v[s];
-    X.ldq += 0.5 * log(S.ex_d[s]);
+    X.ldq += 0.5 * S.ex_ld[s];
     X.ldl += log(exl(S, s, s));
     X.vs = s + 1;
   }
   const double ll = X.ldq - X.ldl + K.minus_n_plus_nu_2 * log(X.syx);
-  X.lmp += prior_add(K, X.Ns, cl + X.ms, type);
-  put(X.Ns, type, sel(X.Ns, type) + 1);
+  X.lmp += S.ex_pa[X.ms];
   ++X.ms;
   return ll + X.lmp;
 }
 
-// src/symmmatrix.cpp:220-265 on the block (columns s, s+1 relative to c0)
+// src/symmmatrix.cpp:220-265 on the block (columns s, s+1 relative to c0):
+// lane r swaps row r < s, lane i - s - 1 rotates column i > s
 template <int KMAX>
 __device__ __forceinline__ void exh_swapadj(Smem<KMAX>& S, int nT, int s) {
-  for (int r = 0; r < s; ++r) {
-    const double t = exl(S, r, s);
-    exl(S, r, s) = exl(S, r, s + 1);
-    exl(S, r, s + 1) = t;
+  const int lane = lane_id();
+  if (lane < s) {
+    const double t = exl(S, lane, s);
+    exl(S, lane, s) = exl(S, lane, s + 1);
+    exl(S, lane, s + 1) = t;
   }
   double a = exl(S, s, s + 1), b = exl(S, s + 1, s + 1);
-  exl(S, s, s + 1) = exl(S, s, s);
-  exl(S, s + 1, s + 1) = 0.0;
+  const double dcc = exl(S, s, s);
+  const double v1 = S.ex_v[s], v2 = S.ex_v[s + 1];
+  wave_sync();
   double c, sn;
   drotg(a, b, c, sn);
   if (a < 0) { a = -a; sn = -sn; c = -c; }
-  exl(S, s, s) = a;
-  for (int i = s + 1; i < nT; ++i) {
-    const double x1 = exl(S, s, i), x2 = exl(S, s + 1, i);
+  // column s+1 holds (dcc, 0) in rows s, s+1 after the swap
+  const int i = s + 1 + lane;
+  if (i < nT) {
+    const double x1 = i == s + 1 ? dcc : exl(S, s, i);
+    const double x2 = i == s + 1 ? 0.0 : exl(S, s + 1, i);
     const double tmp = x1 * sn - x2 * c;
     exl(S, s, i) = x2 * sn + x1 * c;
     exl(S, s + 1, i) = tmp;
   }
-  const double x1 = S.ex_v[s], x2 = S.ex_v[s + 1];
-  const double tmp = x1 * sn - x2 * c;
-  S.ex_v[s] = x2 * sn + x1 * c;
-  S.ex_v[s + 1] = tmp;
+  if (lane == 0) {
+    exl(S, s, s) = a;
+    const double tmp = v1 * sn - v2 * c;
+    S.ex_v[s] = v2 * sn + v1 * c;
+    S.ex_v[s + 1] = tmp;
+  }
+  wave_sync();
 }
 
-__device__ __forceinline__ void swap_d(double& a, double& b) { const double t = a; a = b; b = t; }
+// d and log(d) of two columns trade places (every lane does the same)
+template <int KMAX>
+__device__ __forceinline__ void exh_swap_d(Smem<KMAX>& S, int a, int b) {
+  const double da = S.ex_d[a], db = S.ex_d[b], la = S.ex_ld[a], lb = S.ex_ld[b];
+  wave_sync();
+  S.ex_d[a] = db; S.ex_d[b] = da; S.ex_ld[a] = lb; S.ex_ld[b] = la;
+  wave_sync();
+}
 
 // src/model.hpp:703-832: the last locus of the walk's model moves one
 // position left and the locus it passes leaves the model
 template <int KMAX>
-__device__ __forceinline__ double exh_moveleft(Smem<KMAX>& S, const ChainConst& K, Exh& X, int cl) {
+__device__ __forceinline__ double exh_moveleft(Smem<KMAX>& S, const ChainConst& K, Exh& X) {
   --X.ms;
   const int ik = X.ms, ir = X.ms - 1;
   const int tk = S.ex_t[ik], tr = S.ex_t[ir];
-  double* v = S.ex_v;
-  double* dv = S.ex_d;
-  int32_t* x1 = S.ex_x1;
-  int32_t* x2 = S.ex_x2;
+  const double* v = S.ex_v;
+  const double* lgd = S.ex_ld;
   const int nT = X.nT;
   --X.vs;
   X.syx += v[X.vs] * v[X.vs];
@@ -1308,46 +1346,49 @@ __device__ __forceinline__ double exh_moveleft(Smem<KMAX>& S, const ChainConst& 
   }
   --X.vs;
   X.syx += v[X.vs] * v[X.vs];
-  X.ldq -= 0.5 * log(dv[X.vs]);
+  X.ldq -= 0.5 * lgd[X.vs];
   X.ldl -= log(exl(S, X.vs, X.vs));
   if (tr == 4) {
     --X.vs;
     X.syx += v[X.vs] * v[X.vs];
-    X.ldq -= 0.5 * log(dv[X.vs]);
+    X.ldq -= 0.5 * lgd[X.vs];
     X.ldl -= log(exl(S, X.vs, X.vs));
   }
+  const int x1k = S.ex_x1[ik], x2k = S.ex_x2[ik], x1r = S.ex_x1[ir], x2r = S.ex_x2[ir];
+  wave_sync();
   S.ex_t[ik] = (int8_t)tr;
   S.ex_t[ir] = (int8_t)tk;
   if (tr == 4) {
     if (tk == 4) {
-      int si = x2[ir];
+      int si = x2r;
       exh_swapadj(S, nT, si); ++si;
       exh_swapadj(S, nT, si); si -= 2;
       exh_swapadj(S, nT, si); ++si;
       exh_swapadj(S, nT, si);
-      swap_d(dv[x1[ik]], dv[x2[ir]]); swap_d(dv[x1[ik]], dv[x2[ik]]);
-      swap_d(dv[x1[ir]], dv[x2[ir]]); swap_d(dv[x1[ik]], dv[x2[ir]]);
+      exh_swap_d(S, x1k, x2r); exh_swap_d(S, x1k, x2k);
+      exh_swap_d(S, x1r, x2r); exh_swap_d(S, x1k, x2r);
     } else {
-      int si = x2[ir];
+      int si = x2r;
       exh_swapadj(S, nT, si); --si;
       exh_swapadj(S, nT, si);
-      swap_d(dv[x2[ir]], dv[x1[ik]]); swap_d(dv[x1[ir]], dv[x2[ir]]);
-      x2[ik] = x1[ik];
-      --x1[ik];
+      exh_swap_d(S, x2r, x1k); exh_swap_d(S, x1r, x2r);
+      S.ex_x2[ik] = x1k;
+      S.ex_x1[ik] = x1k - 1;
     }
   } else {
     if (tk == 4) {
-      int si = x1[ir];
+      int si = x1r;
       exh_swapadj(S, nT, si); ++si;
       exh_swapadj(S, nT, si);
-      swap_d(dv[x1[ir]], dv[x1[ik]]); swap_d(dv[x1[ik]], dv[x2[ik]]);
-      x2[ir] = x1[ik];
-      x1[ik] = x2[ik];
+      exh_swap_d(S, x1r, x1k); exh_swap_d(S, x1k, x2k);
+      S.ex_x2[ir] = x1k;
+      S.ex_x1[ik] = x2k;
     } else {
-      exh_swapadj(S, nT, x1[ir]);
-      swap_d(dv[x1[ik]], dv[x1[ir]]);
+      exh_swapadj(S, nT, x1r);
+      exh_swap_d(S, x1k, x1r);
     }
   }
+  wave_sync();
   X.syx -= v[X.vs] * v[X.vs];
   X.ldl += log(exl(S, X.vs, X.vs));
   ++X.vs;
@@ -1357,37 +1398,38 @@ __device__ __forceinline__ double exh_moveleft(Smem<KMAX>& S, const ChainConst& 
     ++X.vs;
   }
   const double ll = X.ldq - X.ldl + K.minus_n_plus_nu_2 * log(X.syx);
-  X.lmp += prior_rem(K, X.Ns, cl + X.ms + 1, tr);
-  put(X.Ns, tr, sel(X.Ns, tr) - 1);
+  X.lmp += S.ex_pr[X.ms + 1];
   return ll + X.lmp;
 }
 
 // The walk over all 2^n sub-models (src/sampler.cpp:882-980): additions,
 // left moves and "two new + swap" steps in the reference's order, so that
-// every log-probability is reached by the same sequence of updates.  Thread
-// 0 only; returns the largest log-probability.
+// every log-probability is reached by the same sequence of updates.  Wave 0
+// only (lane 0 stores); returns the largest log-probability.
 template <int KMAX>
-__device__ __noinline__ double exh_walk(Smem<KMAX>& S, const ChainConst& K, Exh X, int cl, int n,
-                                        double ll0, double* lp) {
-  uint8_t* ci = S.ex_ci;
-  for (int i = 0; i < n; ++i) ci[i] = (uint8_t)i;
-  uint64_t cmb = 0, tmp;
+__device__ __forceinline__ double exh_walk(Smem<KMAX>& S, const ChainConst& K, Exh& X, int n,
+                                           double ll0, double* lp) {
+  const bool st = lane_id() == 0;
+  uint32_t ci[BMG_DR_MAX + 2];   // walk order; indexed only through `sel`/`put`
+  for (int i = 0; i < BMG_DR_MAX + 2; ++i) ci[i] = (uint32_t)i;
+  uint64_t cmb = 0;
+  uint32_t tmp;
   int msz = 0;
   double mx = ll0;
-  lp[0] = ll0;
+  if (st) lp[0] = ll0;
   for (int i = 0; i < n; ++i) {
     ++msz;
-    if (i > 1) { ++msz; exh_add(S, K, X, cl); }
+    if (i > 1) { ++msz; exh_add(S, K, X); }
     cmb = (1ull << msz) - 1;
-    double v = exh_add(S, K, X, cl);
-    lp[cmb] = v;
+    double v = exh_add(S, K, X);
+    if (st) lp[cmb] = v;
     if (v > mx) mx = v;
     for (int j = 0; j < i; ++j) {
       --msz;
-      tmp = ci[msz - 1]; ci[msz - 1] = ci[msz]; ci[msz] = (uint8_t)tmp;
+      tmp = sel(ci, msz - 1); put(ci, msz - 1, sel(ci, msz)); put(ci, msz, tmp);
       cmb &= ~(1ull << tmp);
-      v = exh_moveleft(S, K, X, cl);
-      lp[cmb] = v;
+      v = exh_moveleft(S, K, X);
+      if (st) lp[cmb] = v;
       if (v > mx) mx = v;
     }
     const uint64_t nmodels = (1ull << i) - i - 1;
@@ -1396,12 +1438,12 @@ __device__ __noinline__ double exh_walk(Smem<KMAX>& S, const ChainConst& K, Exh 
     while (j < nmodels) {
       if (do_lefts < 2) {
         ++msz;
-        tmp = ci[msz]; ci[msz] = ci[msz - 1]; ci[msz - 1] = (uint8_t)tmp;
+        tmp = sel(ci, msz); put(ci, msz, sel(ci, msz - 1)); put(ci, msz - 1, tmp);
         cmb |= 1ull << tmp;
-        exh_add(S, K, X, cl);
-        exh_add(S, K, X, cl);
-        v = exh_moveleft(S, K, X, cl);
-        lp[cmb] = v;
+        exh_add(S, K, X);
+        exh_add(S, K, X);
+        v = exh_moveleft(S, K, X);
+        if (st) lp[cmb] = v;
         if (v > mx) mx = v;
         ++j; ++do_lefts;
       } else {
@@ -1409,10 +1451,10 @@ __device__ __noinline__ double exh_walk(Smem<KMAX>& S, const ChainConst& K, Exh 
         const int kb = __builtin_ctzll(nK);
         for (int kk = 0; kk <= kb; ++kk) {
           --msz;
-          tmp = ci[msz - 1]; ci[msz - 1] = ci[msz]; ci[msz] = (uint8_t)tmp;
+          tmp = sel(ci, msz - 1); put(ci, msz - 1, sel(ci, msz)); put(ci, msz, tmp);
           cmb &= ~(1ull << tmp);
-          v = exh_moveleft(S, K, X, cl);
-          lp[cmb] = v;
+          v = exh_moveleft(S, K, X);
+          if (st) lp[cmb] = v;
           if (v > mx) mx = v;
           ++j;
         }
@@ -1436,11 +1478,24 @@ __device__ __forceinline__ double exhaustive_modelset(Ctx<KMAX>& C, int cl, int 
   const int nT = R.cols - c0;
   for (int c = 0; c < nT; ++c)
     for (int r = threadIdx.x; r <= c; r += BMG_BLOCK) S.ex_L[P(c) + r] = S.L[P(c0 + c) + c0 + r];
-  for (int i = threadIdx.x; i < nT; i += BMG_BLOCK) { S.ex_v[i] = S.v[c0 + i]; S.ex_d[i] = S.d[c0 + i]; }
+  for (int i = threadIdx.x; i < nT; i += BMG_BLOCK) {
+    S.ex_v[i] = S.v[c0 + i];
+    S.ex_d[i] = S.d[c0 + i];
+    S.ex_ld[i] = log(S.d[c0 + i]);
+  }
   for (int q = threadIdx.x; q < R.nloci - cl; q += BMG_BLOCK) {
     S.ex_x1[q] = S.xi1[cl + q] - c0;
     S.ex_x2[q] = S.xi2[cl + q] - c0;
     S.ex_t[q] = (int8_t)C.D.types[S.tind[cl + q]];
+  }
+  // prior changes by model size: adding to n_old = cl + q loci, and removing
+  // from n_old = cl + q (src/prior.hpp:144-158, one effect type)
+  if (threadIdx.x <= n) {
+    const int q = threadIdx.x, type = C.D.type;
+    int Ns[5] = {0, 0, 0, 0, 0};
+    put(Ns, type, cl + q);
+    S.ex_pa[q] = prior_add(C.K, Ns, cl + q, type);
+    S.ex_pr[q] = q > 0 || cl > 0 ? prior_rem(C.K, Ns, cl + q, type) : 0.0;
   }
   double vv = 0.0, ldl = 0.0, ldq = 0.0;
   for (int i = threadIdx.x; i < c0; i += BMG_BLOCK) {
@@ -1452,17 +1507,16 @@ __device__ __forceinline__ double exhaustive_modelset(Ctx<KMAX>& C, int cl, int 
   ldl = block_sum(ldl, S.red);
   ldq = block_sum(ldq, S.red);
   __syncthreads();
-  if (threadIdx.x == 0) {
+  if (wave_id() == 0) {
     Exh X;
     X.ms = 0; X.vs = 0; X.nT = nT;
-    for (int q = 0; q < 5; ++q) X.Ns[q] = 0;
-    for (int q = 0; q < cl; ++q) { const int t = C.D.types[S.tind[q]]; put(X.Ns, t, sel(X.Ns, t) + 1); }
     X.syx = C.K.nus2_plus_yy - vv;
     X.ldl = ldl;
     X.ldq = 0.5 * ldq;
     X.lmp = 0.0;
     const double ll0 = X.ldq - X.ldl + C.K.minus_n_plus_nu_2 * log(X.syx);
-    S.bcd[0] = exh_walk(S, C.K, X, cl, n, ll0, lp);
+    const double mx = exh_walk(S, C.K, X, n, ll0, lp);
+    if (lane_id() == 0) S.bcd[0] = mx;
   }
   __syncthreads();
   const double mx = S.bcd[0];
@@ -1515,7 +1569,10 @@ __device__ __forceinline__ void dr_proposal_probs(Smem<KMAX>& S, int n, double z
 // sub-model with its complement, keeps the better of the two with the
 // probability of the difference, and draws one pair from the cumulative
 // weights (src/utils.cpp:73-100 with `ms/2 - 3` bisection steps).  The
-// pairwise transform runs a thread per pair; thread 0 sums in order.
+// pairwise transform runs a thread per pair; wave 0 forms the cumulative
+// sums in order, one chunk of 64 at a time passed along the lanes; the
+// linear part of the search is a workgroup minimum over the first index
+// whose sum exceeds the draw.
 template <int KMAX>
 __device__ __forceinline__ uint64_t dr_second_stage(Ctx<KMAX>& C, int n, double max_log, double* p) {
   const uint64_t nmodels = 1ull << n, mask = nmodels - 1, half = nmodels / 2;
@@ -1527,25 +1584,45 @@ __device__ __forceinline__ uint64_t dr_second_stage(Ctx<KMAX>& C, int n, double 
     else { p[i] = exp(pi - max_log) * (1 - exp(a)); p[j] = 1.0; }
   }
   __syncthreads();
-  const double u = rng_u01(C.R.rng);
-  if (threadIdx.x == 0) {
+  if (wave_id() == 0) {
+    const int lane = lane_id();
     double sum = 0.0;
-    for (uint64_t i = 0; i < half; ++i) { const double v = p[i] + sum; p[i] = v; sum = v; }
-    int level = n / 2 - 3;
-    uint64_t a = 0, b = half - 1;
-    const double r = u * p[b];
-    while (level > 0) {
-      const uint64_t c = (a + b) / 2;
-      if (r < p[c]) b = c; else a = c + 1;
-      --level;
+    for (uint64_t base = 0; base < half; base += 64) {
+      const uint64_t idx = base + lane;
+      const double x = idx < half ? p[idx] : 0.0;
+      const int cnt = half - base < 64 ? (int)(half - base) : 64;
+      double mine = 0.0;
+      for (int k = 0; k < cnt; ++k) {
+        sum = readlane_d(x, k) + sum;
+        if (lane == k) mine = sum;
+      }
+      if (idx < half) p[idx] = mine;
     }
-    uint64_t sm = half - 1;
-    for (uint64_t i = a; i <= b; ++i) if (r < p[i]) { sm = i; break; }
-    if (p[(~sm) & mask] < 0) sm = (~sm) & mask;
-    C.S.bci[0] = (long long)sm;
   }
   __syncthreads();
-  const uint64_t sm = (uint64_t)C.S.bci[0];
+  const double u = rng_u01(C.R.rng);
+  const double r = u * p[half - 1];
+  int level = n / 2 - 3;
+  uint64_t a = 0, b = half - 1;
+  while (level > 0) {
+    const uint64_t c = (a + b) / 2;
+    if (r < p[c]) b = c; else a = c + 1;
+    --level;
+  }
+  long long first = (long long)half;
+  for (uint64_t i = a + threadIdx.x; i <= b; i += BMG_BLOCK)
+    if (r < p[i]) { first = (long long)i; break; }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const long long w = __shfl_xor(first, o, 64);
+    first = w < first ? w : first;
+  }
+  __syncthreads();
+  if (lane_id() == 0) C.S.redi[wave_id()] = first;
+  __syncthreads();
+  for (int w = 0; w < BMG_BLOCK / 64; ++w) first = C.S.redi[w] < first ? C.S.redi[w] : first;
+  uint64_t sm = first < (long long)half ? (uint64_t)first : half - 1;
+  if (p[(~sm) & mask] < 0) sm = (~sm) & mask;
   __syncthreads();
   return sm;
 }
@@ -1610,10 +1687,12 @@ __device__ __forceinline__ int dr_addrem(Ctx<KMAX>& C, int ms, double* adapt) {
   }
   __syncthreads();
   double* lp = C.A.dr_lp;
+  const uint64_t t0 = wall_clock64();
   const double max_log = exhaustive_modelset(C, cl, ms, lp);
   dr_proposal_probs(S, ms, z_add, z_rem, (uint64_t)cl, C.D.p, lp);
   __syncthreads();
   const uint64_t sm = dr_second_stage(C, ms, max_log, lp);
+  if (threadIdx.x == 0 && C.prof) C.prof[19] += wall_clock64() - t0;
   if (sm == ((~nmb) & mask)) { restore(C); undo_dd(C, ms); return 0; }
   int moved = 0;
   for (int i = 0; i < ms; ++i) {
@@ -1640,10 +1719,12 @@ __device__ __forceinline__ int dr_statechange(Ctx<KMAX>& C, int ms, int ms_add, 
   for (int i = 0; i < ms_rem; ++i)
     if (!readd_removed(C, S.mi_rem[i])) { restore(C); return 0; }
   double* lp = C.A.dr_lp;
+  const uint64_t t0 = wall_clock64();
   const double max_log = exhaustive_modelset(C, cl, ms, lp);
   for (uint64_t i = threadIdx.x; i <= mask; i += BMG_BLOCK) lp[i] -= log((double)(cl + __popcll(i)));
   __syncthreads();
   const uint64_t sm = dr_second_stage(C, ms, max_log, lp);
+  if (threadIdx.x == 0 && C.prof) C.prof[19] += wall_clock64() - t0;
   if (sm == ((~nmb) & mask)) { restore(C); return 0; }
   int moved = 0;
   for (int i = 0; i < ms; ++i) {
@@ -1754,7 +1835,12 @@ __device__ __forceinline__ int move_addrem(Ctx<KMAX>& C, double* adapt) {
     return ms;
   }
   // a proposal past the compiled capacity has no second stage
-  if (ms <= K.dr && ms > 1 && !(R.err & ERR_CAPACITY)) return dr_addrem(C, ms, adapt);
+  if (ms <= K.dr && ms > 1 && !(R.err & ERR_CAPACITY)) {
+    const uint64_t t0 = wall_clock64();
+    const int moved = dr_addrem(C, ms, adapt);
+    if (threadIdx.x == 0 && C.prof) C.prof[18] += wall_clock64() - t0;
+    return moved;
+  }
   restore(C);
   undo_dd(C, ms);
   return 0;
@@ -2035,7 +2121,12 @@ __device__ __forceinline__ int move_statechange(Ctx<KMAX>& C, double* adapt) {
     for (int i = 0; i < ms_rem; ++i) { tree_set(C, 0, S.mi_rem[i], 0); tree_set(C, 1, S.mi_rem[i], 1); }
     return ms;
   }
-  if (ms <= K.dr && ms > 1 && !(R.err & ERR_CAPACITY)) return dr_statechange(C, ms, ms_add, ms_rem);
+  if (ms <= K.dr && ms > 1 && !(R.err & ERR_CAPACITY)) {
+    const uint64_t t0 = wall_clock64();
+    const int moved = dr_statechange(C, ms, ms_add, ms_rem);
+    if (threadIdx.x == 0 && C.prof) C.prof[18] += wall_clock64() - t0;
+    return moved;
+  }
   restore(C);
   return 0;
 }
@@ -2342,6 +2433,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   const ChainConst& K = args.consts[c];
   const DevData& D = args.data[K.ds];
   const ChainArrays& A = args.arrays[c];
+  bind_factor<KMAX>(S, A.Lwork);
   Reg R;
   Ctx<KMAX> C{S, R, D, K, A, CS.prof};
   if (args.mode == 1) {
@@ -2365,6 +2457,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 template __global__ void k_chains<64>(KernelArgs);
 template __global__ void k_chains<128>(KernelArgs);
 template __global__ void k_chains<160>(KernelArgs);
+template __global__ void k_chains<256>(KernelArgs);
 
 // ------------------------------------------------------- model op kernel --
 
@@ -2405,6 +2498,7 @@ __global__ __launch_bounds__(256) void k_model_op(ModelOpArgs a) {
   const ChainConst& K = a.consts[0];
   const DevData& D = a.data[K.ds];
   const ChainArrays& A = a.arrays[0];
+  bind_factor<KMAX>(S, A.Lwork);
   Reg R;
   Ctx<KMAX> C{S, R, D, K, A, nullptr};
   load_chain(C, CS);
@@ -2427,6 +2521,7 @@ __global__ __launch_bounds__(256) void k_model_op(ModelOpArgs a) {
 template __global__ void k_model_op<64>(ModelOpArgs);
 template __global__ void k_model_op<128>(ModelOpArgs);
 template __global__ void k_model_op<160>(ModelOpArgs);
+template __global__ void k_model_op<256>(ModelOpArgs);
 
 namespace bmg {
 

@@ -20,7 +20,7 @@
 namespace bmg {
 
 #define LEARN_RING (1u << 19)
-#define LEARN_MAX_ROWS (16 * 160)   // label rows of one RB group: chains x model indices
+#define LEARN_MAX_ROWS (16 * 256)   // label rows of one RB group: chains x model indices
 #define LEARN_CHUNK (1 << 20)
 
 struct LearnView {

@@ -217,7 +217,9 @@ typedef struct {
   int32_t learn_window;        /* learned: partners within this many SNPs */
   int32_t learn_steps;         /* learned: Adam steps per training round */
   int32_t learn_batch;         /* learned: rows per Adam step */
-  int32_t learn_pad;
+  int32_t capacity;            /* model columns compiled for: 64, 128, 160 (factor
+                                  in LDS) or 256 (factor in HBM); 0: from the
+                                  prior, e_qg + 6 sd (AH: twice) */
   double  p_learned;           /* learned: share of the drawn swaps that are learned */
   double  learn_rate;          /* learned: Adam learning rate */
 } bmg_batch_cfg;
@@ -302,7 +304,9 @@ int bmg_batch_graph_stats(bmg_batch* b, double out[3]);
  * additions: x.y, X^T x, the triangular solve, the rest; then the number of
  * additions, the number of removals, ms in removals, ms in the alpha /
  * tau2 draws of the refreshes; then within removals of an inner column: the
- * Cholesky downdate, the triangular solve; 0, 0 (device clock) */
+ * Cholesky downdate, the triangular solve; then delayed-rejection second
+ * stages, and within them the scoring of the sub-models (walk, proposal
+ * probabilities, draw) (device clock) */
 int bmg_batch_profile(bmg_batch* b, int chain, double out[20]);
 
 /* --------------------------------------- learned-proposal network ----- */

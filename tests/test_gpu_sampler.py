@@ -91,6 +91,48 @@ def test_trajectory_parity_delayed_rejection(bmg, types, ld, mms, dr):
     assert np.array_equal(np.sort(b.loci(0)), np.sort(s.loci()))
 
 
+@pytest.mark.parametrize("types,ld,dr", [("A", (0.9, 0.9, 10, 30), 8), ("AH", (0.8, 0.9, 10, 20), 6)])
+def test_factor_in_hbm_matches_oracle(bmg, types, ld, dr):
+    """Capacity 256 keeps the working Cholesky factor in a per-chain global
+    array instead of LDS; forced on small models, the trajectory (delayed
+    rejection included) is still the oracle's."""
+    n, p, n_iter = 300, 400, 3000
+    body, y, causal = dataset(n, p, ld, 5, 4321, types)
+    cfg = synth.sampler_cfg(n, p, 5, types=types, seed=91, n_rao=100, n_rao_burnin=5,
+                            thin=10, n_sample_tau2_and_missing=10, max_move_size=8,
+                            delay_rejection=dr, do_n_iter=n_iter)
+    s = oracle_run(cfg, body, y, n_iter)
+    ctx = bmg.Ctx(n, p, body, y, types=(types,))
+    b = bmg.Batch([ctx], [0], [cfg], dict(trace_every_iter=1, capacity=256))
+    b.setup()
+    b.run(n_iter)
+    compare(b.trace(0), s.trace(), n_iter)
+    assert np.allclose(b.p_rao(0), s.p_rao(), rtol=1e-8, atol=1e-10)
+
+
+def test_models_beyond_the_lds_capacity(bmg):
+    """220 causal SNPs at h2 = 0.9 (n = 2,000, p = 3,000): the models grow past
+    the 160 columns an LDS-resident factor holds, the batch picks the
+    256-column capacity from the prior, and the trajectory over 3,000
+    iterations is the oracle's."""
+    n, p, nc, n_iter = 2000, 3000, 220, 3000
+    thresh, rho, _ = synth.snp_layout(p, (0.05, 0.5), None, 77)
+    body = oracle.synth_genotypes(77, n, p, thresh, rho)
+    dec = lambda snps: np.stack([oracle.decode(body, n, int(j), 0) for j in snps])
+    y, causal, _ = synth.phenotype(dec, n, p, nc, 0.9, 77)
+    cfg = synth.sampler_cfg(n, p, nc, seed=3, n_rao=100, n_rao_burnin=5, thin=10, do_n_iter=n_iter,
+                            delay_rejection=4)
+    s = oracle_run(cfg, body, y, n_iter)
+    ctx = bmg.Ctx(n, p, body, y)
+    b = bmg.Batch([ctx], [0], [cfg], dict(trace_every_iter=1, trace_capacity=256 * (n_iter // 10 + 8)))
+    b.setup()
+    b.run(n_iter)
+    tr = b.trace(0)
+    compare(tr, s.trace(), n_iter)
+    assert tr["size"].max() > 160
+    assert np.allclose(b.p_rao(0), s.p_rao(), rtol=1e-8, atol=1e-10)
+
+
 def test_batch_of_chains_each_match_their_oracle_run(bmg):
     n, p, n_iter = 256, 300, 1500
     body, y, _ = dataset(n, p, (0.9, 0.9, 10, 30), 4, 99)
