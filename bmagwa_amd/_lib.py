@@ -56,13 +56,14 @@ class BatchCfg(C.Structure):
         ("trace_capacity", C.c_uint64),
         ("learn_window", C.c_int32), ("learn_steps", C.c_int32), ("learn_batch", C.c_int32),
         ("capacity", C.c_int32), ("p_learned", C.c_double), ("learn_rate", C.c_double),
+        ("max_model_size", C.c_int32), ("cfg_pad", C.c_int32),
     ]
 
 
 BATCH_CFG_DEFAULT = dict(trace_every_iter=0, proposal=0, learn_every=500, learn_freeze_iter=0,
                          scan_digits=5, use_graph=0, trace_capacity=0, learn_window=20,
                          learn_steps=100, learn_batch=4096, capacity=0, p_learned=0.9,
-                         learn_rate=1e-3)
+                         learn_rate=1e-3, max_model_size=0, cfg_pad=0)
 
 OPT_DEFAULTS = dict(max_move_size=20, max_SNP_neighborhood_size=10, p_move_size=0.2,
                     p_move_size_nbs=0.2, p_move_size_nbc=0.2, adapt_p_move_size=1,

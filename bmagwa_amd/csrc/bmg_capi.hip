@@ -678,8 +678,12 @@ static int batch_create(bmg_ctx** ctxs, int n_ctx, int n_chains, const int32_t* 
   hipSetDevice(b->device);
   // capacity: from the prior's model size (e_qg + 6 sd, AH doubles the columns)
   double need = 0;
+  if (b->cfg.max_model_size < 0) { delete b; set_error("max_model_size must be >= 0"); return BMG_E_ARG; }
   for (int c = 0; c < n_chains; ++c) {
     double cols = (opts[c].e_qg + 6.0 * std::sqrt(opts[c].var_qg) + 8.0);
+    // with a model-size limit a proposal reaches at most limit + max_move_size
+    if (b->cfg.max_model_size > 0)
+      cols = std::min(cols, (double)b->cfg.max_model_size + (double)opts[c].max_move_size + 1.0);
     for (int t = 0; t < opts[c].n_types; ++t) if (opts[c].types[t] == BMG_AH) cols *= 2;
     need = std::max(need, cols + 1);
   }
@@ -789,6 +793,7 @@ static int batch_create(bmg_ctx** ctxs, int n_ctx, int n_chains, const int32_t* 
     Kc.max_ms = (uint32_t)std::min<uint64_t>(std::min<uint64_t>(x->p, 255), o.max_move_size);
     Kc.max_nbh = o.max_SNP_neighborhood_size;
     Kc.dr = (int32_t)std::min<uint64_t>(Kc.max_ms, o.delay_rejection);
+    Kc.max_size = b->cfg.max_model_size > 0 ? b->cfg.max_model_size : INT32_MAX;
     Kc.adapt_p_ms = o.adapt_p_move_size && Kc.max_ms > 1;
     Kc.adaptation = o.adaptation; Kc.flat = o.flat_proposal_dist;
     Kc.trace_iter = b->cfg.trace_every_iter;

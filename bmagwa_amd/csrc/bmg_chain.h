@@ -50,6 +50,7 @@ struct ChainConst {
   uint64_t tr_cap, th_cap, th_loci_cap;
   double p_learned;      // probability of a learned swap per iteration
   int32_t dr;            // delayed-rejection move-size limit (0: none)
+  int32_t max_size;      // largest model (SNPs) with nonzero prior
 };
 
 // Mutable per-chain scalars (device global memory).

@@ -959,9 +959,15 @@ __device__ __forceinline__ void restore(Ctx<KMAX>& C) {
 
 // --------------------------------------------------------------- prior ----
 // src/prior.hpp:144-158 (single effect type, Ns[type] == n_loci)
-__device__ __forceinline__ double prior_add(const ChainConst& K, const int (&Ns)[5], int n_old, int type) {
+__device__ __forceinline__ double prior_add_untruncated(const ChainConst& K, const int (&Ns)[5], int n_old,
+                                                        int type) {
   return (log(K.g_a + n_old) - log(K.g_b + K.m_g - n_old - 1)) +
          (log(K.tp_type + sel(Ns, type)) - log(K.tp_sum + n_old));
+}
+// with a model-size limit the prior of larger models is 0
+__device__ __forceinline__ double prior_add(const ChainConst& K, const int (&Ns)[5], int n_old, int type) {
+  if (n_old >= K.max_size) return -INFINITY;
+  return prior_add_untruncated(K, Ns, n_old, type);
 }
 __device__ __forceinline__ double prior_rem(const ChainConst& K, const int (&Ns)[5], int n_old, int type) {
   return (log(K.g_b + K.m_g - n_old) - log(K.g_a + n_old - 1)) +
@@ -1494,7 +1500,7 @@ __device__ __forceinline__ double exhaustive_modelset(Ctx<KMAX>& C, int cl, int 
     const int q = threadIdx.x, type = C.D.type;
     int Ns[5] = {0, 0, 0, 0, 0};
     put(Ns, type, cl + q);
-    S.ex_pa[q] = prior_add(C.K, Ns, cl + q, type);
+    S.ex_pa[q] = prior_add_untruncated(C.K, Ns, cl + q, type);
     S.ex_pr[q] = q > 0 || cl > 0 ? prior_rem(C.K, Ns, cl + q, type) : 0.0;
   }
   double vv = 0.0, ldl = 0.0, ldq = 0.0;
@@ -1519,8 +1525,23 @@ __device__ __forceinline__ double exhaustive_modelset(Ctx<KMAX>& C, int cl, int 
     if (lane_id() == 0) S.bcd[0] = mx;
   }
   __syncthreads();
-  const double mx = S.bcd[0];
+  double mx = S.bcd[0];
   __syncthreads();
+  if (cl + n > C.K.max_size) {
+    // the walk scores with the untruncated prior (its steps pass through
+    // larger models); sub-models above the limit get prior 0 afterwards
+    double m = -INFINITY;
+    for (uint64_t i = threadIdx.x; i < (1ull << n); i += BMG_BLOCK) {
+      if (cl + __popcll(i) > C.K.max_size) lp[i] = -INFINITY;
+      else m = fmax(m, lp[i]);
+    }
+    m = wave_max(m);
+    if (lane_id() == 0) S.red[wave_id()] = m;
+    __syncthreads();
+    mx = -INFINITY;
+    for (int w = 0; w < BMG_BLOCK / 64; ++w) mx = fmax(mx, S.red[w]);
+    __syncthreads();
+  }
   return mx;
 }
 
@@ -1580,7 +1601,9 @@ __device__ __forceinline__ uint64_t dr_second_stage(Ctx<KMAX>& C, int n, double 
     const uint64_t j = (~i) & mask;
     const double pi = p[i], pj = p[j];
     const double a = pj - pi;
-    if (a > 0) { p[i] = exp(pj - max_log) * (1 - exp(-a)); p[j] = -1.0; }
+    // a pair of two zero-prior sub-models (model-size limit) has weight 0
+    if (pi == -INFINITY && pj == -INFINITY) { p[i] = 0.0; p[j] = 1.0; }
+    else if (a > 0) { p[i] = exp(pj - max_log) * (1 - exp(-a)); p[j] = -1.0; }
     else { p[i] = exp(pi - max_log) * (1 - exp(a)); p[j] = 1.0; }
   }
   __syncthreads();

@@ -56,7 +56,7 @@ __global__ void k_rb_finalize(const DevData* __restrict__ data, const ChainConst
   const double s2x2 = 2.0 * CS.sigma2;
   const double nl = (double)CS.nloci;
   // tabulated prior changes (src/sampler.cpp:51-76), single type
-  const double lmp_add = rb_prior_add(K, nl, nl);
+  const double lmp_add = CS.nloci >= K.max_size ? -INFINITY : rb_prior_add(K, nl, nl);
   const double lmp_rem = rb_prior_add(K, nl - 1, nl - 1);
   double dv[4] = {CS.inv_tau2_alpha2[0], CS.inv_tau2_alpha2[1], CS.inv_tau2_alpha2[2],
                   CS.inv_tau2_alpha2[3]};

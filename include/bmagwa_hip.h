@@ -222,6 +222,11 @@ typedef struct {
                                   prior, e_qg + 6 sd (AH: twice) */
   double  p_learned;           /* learned: share of the drawn swaps that are learned */
   double  learn_rate;          /* learned: Adam learning rate */
+  int32_t max_model_size;      /* SNPs; 0: none.  Larger models get prior
+                                  probability 0 (the beta-binomial prior
+                                  truncated there), so moves past it are
+                                  rejected and RB draws respect it */
+  int32_t cfg_pad;
 } bmg_batch_cfg;
 
 int bmg_batch_create(bmg_ctx** ctxs, int n_ctx, int n_chains, const int32_t* chain_ctx,

@@ -270,3 +270,66 @@ def test_reference_cpu_configuration_trajectory(bmg, dr):
     s = oracle_run(cfg, body, y, n_iter)
     compare(b.trace(0), s.trace(), n_iter)
     assert np.allclose(b.p_rao(0), s.p_rao(), rtol=1e-8, atol=1e-10)
+
+
+@pytest.mark.parametrize("dr", [0, 3])
+def test_model_size_limit_matches_exact_enumeration(bmg, dr):
+    """max_model_size = 2 on the 7-SNP small-modelspace fixture
+    (src/tests/testdata/small_modelspace_tau2.ini) with tau2 and alpha held
+    at their set-up draw: no sample exceeds the limit, and the device run's
+    RB PIPs match exact enumeration over the 29 models of at most 2 SNPs
+    (the beta-binomial prior truncated at 2) to 0.01, with and without
+    delayed rejection.  Sample frequencies are held to 0.03: on this fixture
+    they sit up to 0.018 from the exact PIPs without any limit, in the
+    oracle's runs as in the device's, while the RB PIPs agree to 0.001."""
+    from fixtures import small_modelspace, SMALL_MODELSPACE_TAU2_INI
+    body, y = small_modelspace()
+    n_iter, cap = 400000, 2
+    cfg = dict(SMALL_MODELSPACE_TAU2_INI)
+    cfg.update(n_sample_tau2_and_missing=10 ** 9, n_rao=50, n_rao_burnin=100, thin=1,
+               delay_rejection=dr, max_move_size=4, seed=77, do_n_iter=n_iter)
+    d = oracle.Data(30, 7, body, y, recode=True)
+    s = oracle.Sampler(cfg, d)
+    s.init_flat()
+    assert s.setup() == 0
+    inv_a = s.prior()["inv_tau2_alpha2_A"]
+    base = oracle.Model(s, oracle.lib().orc_sampler_new_model(s.h))
+    empty = s.current()
+    lps, subs = [], []
+    for mask in range(1 << 7):
+        sub = [j for j in range(7) if (mask >> j) & 1]
+        base.copy_from(empty)
+        for j in sub:
+            base.add_term(j, 0, [inv_a])
+        lps.append(base.log_likelihood() + s.prior_log_model([len(sub), 0, 0, 0, 0]))
+        subs.append(sub)
+    lps = np.array(lps)
+    size = np.array([len(sub) for sub in subs])
+
+    def pips(keep):
+        w = np.where(keep, np.exp(lps - lps[keep].max()), 0.0)
+        w /= w.sum()
+        out = np.zeros(7)
+        for wi, sub in zip(w, subs):
+            out[sub] += wi
+        return out
+
+    exact = pips(size <= cap)
+    # the limit matters here: about a third of the untruncated posterior
+    # lies on larger models
+    assert np.max(np.abs(pips(size >= 0) - exact)) > 0.1
+
+    ctx = bmg.Ctx(30, 7, body, y, recode=True)
+    b = bmg.Batch([ctx], [0], [cfg], dict(max_model_size=cap, trace_capacity=cap * (n_iter + 16)))
+    b.setup()
+    b.run(n_iter)
+    th = b.thinned(0)
+    sizes, loci = th["sizes"], th["loci"]
+    assert sizes.max() <= cap
+    burn = 20000
+    off = int(sizes[:burn].sum())
+    counts = np.bincount(loci[off:].astype(np.int64), minlength=7)
+    freq = counts / (len(sizes) - burn)
+    assert np.max(np.abs(freq - exact)) < 0.03, (freq, exact)
+    p_rao = b.p_rao(0)
+    assert np.max(np.abs(p_rao - exact)) < 0.01, (p_rao, exact)
