@@ -58,6 +58,11 @@ sys.path.insert(0, HERE)
 
 N, P, N_CAUSAL, H2 = 10000, 500000, 100, 0.5
 DELAY_REJECTION = 0      # --delay-rejection: the sampler option of every run
+MAX_MODEL_SIZE = 0       # model-size limit of every run (0: none)
+# --config large: n=50,000, p=1,000,000, MAF ~ U(0.01, 0.5), the same LD
+# blocks, 300 causal SNPs, h2 = 0.5, 128 chains, models of at most 200 SNPs
+LARGE = dict(N=50000, P=1000000, N_CAUSAL=300, H2=0.5, MAF=(0.01, 0.5), SEED=20261202 + 5000,
+             chains=128, max_model_size=200)
 MAF, LD = (0.02, 0.5), (0.7, 0.95, 20, 100)
 SEED = 20261202 + 4000
 HBM_PEAK_GBS = 8000.0
@@ -95,6 +100,7 @@ def run_batch(bmg, synth, ctx, n_chains, steps, warmup, ips, seed0, batch_cfg=No
             for c in range(n_chains)]
     bc = dict(batch_cfg or {})
     bc["trace_capacity"] = (total // thin + 16) * 200    # thinned loci: up to 200 per sample
+    bc["max_model_size"] = MAX_MODEL_SIZE
     bc.setdefault("use_graph", 1)     # after burn-in: segments replayed from a captured hipGraph
     t0 = time.perf_counter()
     b = bmg.Batch([ctx], [0] * n_chains, cfgs, bc)
@@ -110,6 +116,8 @@ def run_batch(bmg, synth, ctx, n_chains, steps, warmup, ips, seed0, batch_cfg=No
     t0 = time.perf_counter()
     for s in range(steps):
         b.run(ips)
+        if (s + 1) % 10 == 0 and s + 1 < steps:
+            log("%d chains timed step %d/%d" % (n_chains, s + 1, steps))
     b.sync()
     wall = time.perf_counter() - t0
     tm1 = b.timing()
@@ -204,7 +212,7 @@ def phase_profile(bmg, synth, ctx, args):
     cfgs = [synth.sampler_cfg(N, P, N_CAUSAL, types="A", seed=100 + c, chain=c, do_n_iter=total,
                               n_rao_burnin=4, thin=args.thin, delay_rejection=DELAY_REJECTION)
             for c in range(n)]
-    b = bmg.Batch([ctx], [0] * n, cfgs, dict(proposal=0, use_graph=0,
+    b = bmg.Batch([ctx], [0] * n, cfgs, dict(proposal=0, use_graph=0, max_model_size=MAX_MODEL_SIZE,
                                               trace_capacity=(total // args.thin + 16) * 200))
     b.setup()
     b.run(args.warmup * ips)
@@ -272,7 +280,12 @@ def main():
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--iters-per-step", type=int, default=1000)
-    ap.add_argument("--chains", type=int, default=16)
+    ap.add_argument("--chains", type=int, default=None,
+                    help="chains of the batched runs (16; 128 with --config large)")
+    ap.add_argument("--config", choices=("default", "large"), default="default",
+                    help="default: n=10,000, p=500,000, 100 causal, 16 chains (the headline "
+                         "workload); large: n=50,000, p=1,000,000, 300 causal, 128 chains, models of "
+                         "at most 200 SNPs")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
@@ -301,8 +314,13 @@ def main():
     ap.add_argument("--pip-ref-steps", type=int, default=30,
                     help="timed steps of the PIP reference run")
     args = ap.parse_args()
-    global DELAY_REJECTION
+    global DELAY_REJECTION, MAX_MODEL_SIZE, N, P, N_CAUSAL, H2, MAF, SEED
     DELAY_REJECTION = args.delay_rejection
+    if args.config == "large":
+        N, P, N_CAUSAL, H2 = LARGE["N"], LARGE["P"], LARGE["N_CAUSAL"], LARGE["H2"]
+        MAF, SEED, MAX_MODEL_SIZE = LARGE["MAF"], LARGE["SEED"], LARGE["max_model_size"]
+    if args.chains is None:
+        args.chains = LARGE["chains"] if args.config == "large" else 16
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU only: chains and phenotypes are independent, so more "
@@ -451,9 +469,10 @@ def main():
         "ms_per_step": round(wall / args.steps * 1e3, 2),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic (generated on device, seeded)",
-        "config": {"workload": "PMV MCMC over SNP inclusion, n=10000 p=500000 AR(1) LD, "
-                               "100 causal, h2=0.5, additive",
+        "config": {"workload": "PMV MCMC over SNP inclusion, n=%d p=%d AR(1) LD, MAF U(%g, %g), "
+                               "%d causal, h2=%g, additive" % (N, P, MAF[0], MAF[1], N_CAUSAL, H2),
                    "chains": args.chains, "iters_per_step": ips,
+                   "max_model_size": MAX_MODEL_SIZE or None,
                    "delay_rejection": DELAY_REJECTION,
                    "proposal": ("plain moves + learned swap (MLP 16-256-256-1, bf16)" if args.learned
                                 else "plain PMV moves (learned swap not run at this size)")},
