@@ -676,18 +676,26 @@ static int batch_create(bmg_ctx** ctxs, int n_ctx, int n_chains, const int32_t* 
   int kmax = 128;
   b->device = ctxs[0]->device;
   hipSetDevice(b->device);
-  // capacity: from the prior's model size (e_qg + 6 sd, AH doubles the columns)
-  double need = 0;
+  // capacity: the prior's model size bound (e_qg + 6 sd, AH doubles the
+  // columns), or the model-size limit plus one move; the factor leaves LDS
+  // (256) only when a typical model (prior mean plus one move) needs it
+  double need = 0, typical = 0;
   if (b->cfg.max_model_size < 0) { delete b; set_error("max_model_size must be >= 0"); return BMG_E_ARG; }
   for (int c = 0; c < n_chains; ++c) {
     double cols = (opts[c].e_qg + 6.0 * std::sqrt(opts[c].var_qg) + 8.0);
-    // with a model-size limit a proposal reaches at most limit + max_move_size
-    if (b->cfg.max_model_size > 0)
-      cols = std::min(cols, (double)b->cfg.max_model_size + (double)opts[c].max_move_size + 1.0);
-    for (int t = 0; t < opts[c].n_types; ++t) if (opts[c].types[t] == BMG_AH) cols *= 2;
+    double typ = opts[c].e_qg + (double)opts[c].max_move_size + 8.0;
+    if (b->cfg.max_model_size > 0) {
+      // with a model-size limit a proposal reaches at most limit + max_move_size
+      const double lim = (double)b->cfg.max_model_size + (double)opts[c].max_move_size + 1.0;
+      cols = std::min(cols, lim);
+      typ = std::min(typ, lim);
+    }
+    for (int t = 0; t < opts[c].n_types; ++t)
+      if (opts[c].types[t] == BMG_AH) { cols *= 2; typ *= 2; }
     need = std::max(need, cols + 1);
+    typical = std::max(typical, typ + 1);
   }
-  kmax = need <= 64 ? 64 : (need <= 128 ? 128 : (need <= 160 ? 160 : 256));
+  kmax = need <= 64 ? 64 : (need <= 128 ? 128 : ((need <= 160 || typical <= 160) ? 160 : 256));
   if (model_api) kmax = 160;
   if (b->cfg.capacity) {
     if (b->cfg.capacity != 64 && b->cfg.capacity != 128 && b->cfg.capacity != 160 && b->cfg.capacity != 256) {

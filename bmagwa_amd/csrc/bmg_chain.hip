@@ -16,6 +16,7 @@
 
 #include <algorithm>
 
+#include <type_traits>
 #include "bmg_chain.h"
 #include "bmg_device.h"
 #include "../../include/bmagwa_hip.h"
@@ -55,25 +56,18 @@ __device__ __forceinline__ void put(T (&a)[N], int i, T v) {
 }
 
 // ------------------------------------------------------------- state -----
-// The proposed model's factor (packed upper): in LDS up to 160 columns; a
-// larger capacity keeps it in a per-chain global array (L2/HBM), whose
-// address thread 0 stores here at kernel entry.
+// The proposed model's factor (packed upper): an LDS array up to 160
+// columns; a larger capacity keeps it in a per-chain global array (L2/HBM),
+// whose address thread 0 stores in the LDS member at kernel entry.  The
+// member is a plain array in the LDS case (as a base-class member the
+// compiler's code for the triangular loops was a fifth slower).
 constexpr int KMAX_LDS = 160;
-template <int KMAX, bool IN_LDS = (KMAX <= KMAX_LDS)>
-struct FactorStore { double L[KMAX * (KMAX + 1) / 2]; };
 template <int KMAX>
-struct FactorStore<KMAX, false> { double* L; };
+using FactorT = std::conditional_t<(KMAX <= KMAX_LDS), double[KMAX * (KMAX + 1) / 2], double*>;
 
 template <int KMAX>
-__device__ __forceinline__ void bind_factor(FactorStore<KMAX, true>&, double*) {}
-template <int KMAX>
-__device__ __forceinline__ void bind_factor(FactorStore<KMAX, false>& F, double* g) {
-  if (threadIdx.x == 0) F.L = g;
-  __syncthreads();
-}
-
-template <int KMAX>
-struct Smem : FactorStore<KMAX> {
+struct Smem {
+  FactorT<KMAX> L;                       // proposed model's factor (packed upper)
   double xy[KMAX], v[KMAX], d[KMAX];     // proposed model vectors
   double beta[KMAX], mu[KMAX];           // the chain's beta / mu_beta
   double work[KMAX + 64];
@@ -1162,13 +1156,55 @@ __device__ __forceinline__ void sample_alpha_and_tau2(Ctx<KMAX>& C) {
   // sample_tau2
   double a2s2 = R.alpha2 * R.sigma2;
   if (K.use_ind) {
-    for (int i = 1; i < R.beta_len; ++i) {
-      int t = S.xtype[i];
-      double nu = K.nu_tau2[t] + 1.0;
-      double s2 = (K.nus2_tau2[t] + S.beta[i] * S.beta[i] / a2s2) / nu;
-      double dv = 1.0 / (R.alpha2 * rng_sinvchi2(R.rng, nu, s2));
-      if (threadIdx.x == 0) S.d[i] = dv;
+    // one draw per term, in term order on the chain's stream, computed a
+    // term per thread: a thread starts its term where the stream would be
+    // if every earlier term of the round took one trial (two Philox
+    // blocks); the first term after which that does not hold ends the
+    // round, and the next round restarts from the stream position it left.
+    // The draws are the sequential loop's, value for value.
+    uint64_t base = (uint64_t)R.rng.c0 | ((uint64_t)R.rng.c1 << 32);
+    int first = 1;
+    while (first < R.beta_len) {
+      const int i = first + (int)threadIdx.x;
+      long long stop = R.beta_len;     // first term whose stream use differs
+      uint64_t next = 0;
+      if (i < R.beta_len) {
+        const uint64_t start = base + 2ull * (uint64_t)threadIdx.x;
+        Rng r = R.rng;
+        r.c0 = (uint32_t)start; r.c1 = (uint32_t)(start >> 32);
+        const int t = S.xtype[i];
+        const double nu = K.nu_tau2[t] + 1.0;
+        const double s2 = (K.nus2_tau2[t] + S.beta[i] * S.beta[i] / a2s2) / nu;
+        const double dv = 1.0 / (R.alpha2 * rng_sinvchi2(r, nu, s2));
+        next = (uint64_t)r.c0 | ((uint64_t)r.c1 << 32);
+        if (next - start != 2) stop = i;
+        S.work[i] = dv;
+        if (i == first + BMG_BLOCK - 1 && stop == R.beta_len) stop = i;
+      }
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        const long long w = __shfl_xor(stop, o, 64);
+        stop = w < stop ? w : stop;
+      }
+      __syncthreads();
+      if (lane_id() == 0) S.redi[wave_id()] = stop;
+      __syncthreads();
+      stop = S.redi[0];
+      for (int w = 1; w < BMG_BLOCK / 64; ++w) stop = S.redi[w] < stop ? S.redi[w] : stop;
+      // terms first..stop are final; the stream continues after term stop
+      if (i <= stop && i < R.beta_len) S.d[i] = S.work[i];
+      if (i == stop) S.bci[3] = (long long)next;
+      __syncthreads();
+      if (stop >= R.beta_len) {
+        base += 2ull * (uint64_t)(R.beta_len - first);
+        first = R.beta_len;
+      } else {
+        base = (uint64_t)S.bci[3];
+        first = (int)stop + 1;
+      }
+      __syncthreads();
     }
+    R.rng.c0 = (uint32_t)base; R.rng.c1 = (uint32_t)(base >> 32);
   } else {
     double b2[4] = {0, 0, 0, 0}; double mt[4] = {0, 0, 0, 0};
     for (int i = 1; i < R.beta_len; ++i) {
@@ -1411,10 +1447,11 @@ __device__ __forceinline__ double exh_moveleft(Smem<KMAX>& S, const ChainConst& 
 // The walk over all 2^n sub-models (src/sampler.cpp:882-980): additions,
 // left moves and "two new + swap" steps in the reference's order, so that
 // every log-probability is reached by the same sequence of updates.  Wave 0
-// only (lane 0 stores); returns the largest log-probability.
+// only (lane 0 stores); returns the largest log-probability.  Out of line:
+// inlined, its registers crowd the kernel's hot loops.
 template <int KMAX>
-__device__ __forceinline__ double exh_walk(Smem<KMAX>& S, const ChainConst& K, Exh& X, int n,
-                                           double ll0, double* lp) {
+__device__ __noinline__ double exh_walk(Smem<KMAX>& S, const ChainConst& K, Exh X, int n,
+                                        double ll0, double* lp) {
   const bool st = lane_id() == 0;
   uint32_t ci[BMG_DR_MAX + 2];   // walk order; indexed only through `sel`/`put`
   for (int i = 0; i < BMG_DR_MAX + 2; ++i) ci[i] = (uint32_t)i;
@@ -2456,7 +2493,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   const ChainConst& K = args.consts[c];
   const DevData& D = args.data[K.ds];
   const ChainArrays& A = args.arrays[c];
-  bind_factor<KMAX>(S, A.Lwork);
+  if constexpr (KMAX > KMAX_LDS) {
+    if (threadIdx.x == 0) S.L = A.Lwork;
+    __syncthreads();
+  }
   Reg R;
   Ctx<KMAX> C{S, R, D, K, A, CS.prof};
   if (args.mode == 1) {
@@ -2521,7 +2561,10 @@ __global__ __launch_bounds__(256) void k_model_op(ModelOpArgs a) {
   const ChainConst& K = a.consts[0];
   const DevData& D = a.data[K.ds];
   const ChainArrays& A = a.arrays[0];
-  bind_factor<KMAX>(S, A.Lwork);
+  if constexpr (KMAX > KMAX_LDS) {
+    if (threadIdx.x == 0) S.L = A.Lwork;
+    __syncthreads();
+  }
   Reg R;
   Ctx<KMAX> C{S, R, D, K, A, nullptr};
   load_chain(C, CS);
