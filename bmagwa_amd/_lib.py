@@ -114,6 +114,7 @@ SIGNATURES = {
     "bmg_ctx_create_synthetic": (C.c_int, C.c_int, C.c_uint32, C.c_uint32, C.c_uint64, _D, _D, _I32,
                                  C.c_int, _PP),
     "bmg_ctx_set_y": (C.c_int, _P, _D),
+    "bmg_ctx_create_subset": (C.c_int, _P, _U32, C.c_uint32, _D, _I32, C.c_int, _PP),
     "bmg_ctx_destroy": (C.c_int, _P),
     "bmg_ctx_read_body": (C.c_int, _P, _U8),
     "bmg_ctx_stats": (C.c_int, _P, _D),
@@ -134,6 +135,7 @@ SIGNATURES = {
     "bmg_dev_alloc": (C.c_int, C.c_int, C.c_uint64, _PP),
     "bmg_dev_free": (C.c_int, C.c_int, _P),
     "bmg_memcpy_h2d": (C.c_int, C.c_int, _P, _P, C.c_uint64),
+    "bmg_memcpy_d2h": (C.c_int, C.c_int, _P, _P, C.c_uint64),
     "bmg_copy_timed": (C.c_int, C.c_int, _P, _P, C.c_uint64, C.c_int, _D),
     "bmg_read_timed": (C.c_int, C.c_int, _P, C.c_uint64, _P, C.c_int, _D),
     "bmg_mlp_nparams": (C.c_int,),
@@ -149,6 +151,13 @@ SIGNATURES = {
     "bmg_model_kmax": (C.c_int,),
     "bmg_model_create": (C.c_int, _P, _D, _PP),
     "bmg_model_destroy": (C.c_int, _P),
+    "bmg_model_create_sampler": (C.c_int, _P, C.POINTER(Opts), _PP),
+    "bmg_model_sample_beta_sigma2": (C.c_int, _P),
+    "bmg_model_compute_pve": (C.c_int, _P, _D),
+    "bmg_model_beta": (C.c_int, _P, _D, _D, _D, C.POINTER(C.c_int)),
+    "bmg_model_rb": (C.c_int, _P, _D, _D),
+    "bmg_model_commit": (C.c_int, _P, _P),
+    "bmg_model_rollback": (C.c_int, _P, _P),
     "bmg_model_add_term": (C.c_int, _P, C.c_uint32, C.c_int, _D),
     "bmg_model_remove_term": (C.c_int, _P, C.c_int),
     "bmg_model_compute_log_likelihood": (C.c_int, _P),
@@ -232,6 +241,22 @@ class Ctx:
                                        len(tv), 1 if recode else 0, C.byref(h)))
         self.h = h
 
+    @classmethod
+    def subset(cls, pool, individuals, y, types=("A",)):
+        """The dataset of a phenotype measured on pool individuals
+        `individuals` (gathered on the device) with its own y."""
+        self = cls.__new__(cls)
+        ind = np.ascontiguousarray(individuals, dtype=np.uint32)
+        self.n, self.p = len(ind), pool.p
+        self.types = np.array(sorted(EF[t] for t in types), dtype=np.int32)
+        yy = np.ascontiguousarray(y, dtype=np.float64)
+        assert len(yy) == self.n
+        h = C.c_void_p()
+        check(lib().bmg_ctx_create_subset(pool.h, _p(ind, C.c_uint32), self.n, _p(yy),
+                                          _p(self.types, C.c_int32), len(self.types), C.byref(h)))
+        self.h = h
+        return self
+
     def set_y(self, y):
         yy = np.ascontiguousarray(y, dtype=np.float64)
         check(lib().bmg_ctx_set_y(self.h, _p(yy)))
@@ -309,6 +334,12 @@ class Ctx:
                                        C.c_void_p(stream) if stream else None))
         return ms.value
 
+    def scan_xtr_dev(self, r_dev, nvec, out_dev, coding=0, digits=5, stream=None):
+        """X^T R on device buffers r_dev (nvec x n_pad doubles) -> out_dev
+        (nvec x p doubles)."""
+        check(lib().bmg_scan_xtr_dev(self.h, C.c_void_p(r_dev), nvec, coding, digits, C.c_void_p(out_dev),
+                                     C.c_void_p(stream) if stream else None))
+
     def n_pad(self):
         v = C.c_uint32()
         check(lib().bmg_ctx_n_pad(self.h, C.byref(v)))
@@ -341,6 +372,14 @@ class DevBuf:
             raise ValueError("upload larger than the buffer")
         check(lib().bmg_memcpy_h2d(self.device, C.c_void_p(self.ptr), a.ctypes.data_as(C.c_void_p),
                                    a.nbytes))
+
+    def download(self, dtype, count):
+        out = np.empty(count, dtype=dtype)
+        if out.nbytes > self.nbytes:
+            raise ValueError("download larger than the buffer")
+        check(lib().bmg_memcpy_d2h(self.device, out.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr),
+                                   out.nbytes))
+        return out
 
     def free(self):
         if self.ptr:
@@ -425,12 +464,49 @@ def read_timed(src, nbytes, sink, reps=5, device=0):
 class Model:
     """Drop-in Model (src/model.hpp:46-582) on the device."""
 
-    def __init__(self, ctx, n_plus_nu, nus2_plus_yy, inv_tau2_e0=0.0):
-        pr = np.array([n_plus_nu, -0.5 * n_plus_nu, nus2_plus_yy, inv_tau2_e0])
+    def __init__(self, ctx, n_plus_nu=None, nus2_plus_yy=None, inv_tau2_e0=0.0, cfg=None):
+        """Model(data, data_model, prior): either the solved prior terms, or
+        the options of a Sampler (cfg: INI-style dict), whose prior and
+        random stream the model then carries."""
         h = C.c_void_p()
-        check(lib().bmg_model_create(ctx.h, _p(pr), C.byref(h)))
+        if cfg is not None:
+            self._opts = opts_from_cfg(cfg)
+            check(lib().bmg_model_create_sampler(ctx.h, C.byref(self._opts), C.byref(h)))
+        else:
+            pr = np.array([n_plus_nu, -0.5 * n_plus_nu, nus2_plus_yy, inv_tau2_e0])
+            check(lib().bmg_model_create(ctx.h, _p(pr), C.byref(h)))
         self.h = h
         self.ctx = ctx
+
+    def sample_beta_sigma2(self):
+        check(lib().bmg_model_sample_beta_sigma2(self.h))
+
+    def compute_pve(self):
+        out = np.zeros(3)
+        check(lib().bmg_model_compute_pve(self.h, _p(out)))
+        return out
+
+    def beta(self):
+        """beta, sigma2 and y_hat of the last draws / PVE."""
+        k = C.c_int()
+        check(lib().bmg_model_beta(self.h, None, None, None, C.byref(k)))
+        b = np.zeros(max(k.value, 1))
+        s2 = C.c_double()
+        yh = np.zeros(self.ctx.n)
+        check(lib().bmg_model_beta(self.h, _p(b), C.byref(s2), _p(yh), C.byref(k)))
+        return b[:k.value], s2.value, yh
+
+    def rb(self, y_hat):
+        out = np.zeros(self.ctx.p)
+        yh = np.ascontiguousarray(y_hat, dtype=np.float64)
+        check(lib().bmg_model_rb(self.h, _p(yh), _p(out)))
+        return out
+
+    def commit_from(self, proposed):
+        check(lib().bmg_model_commit(self.h, proposed.h))
+
+    def rollback_from(self, current):
+        check(lib().bmg_model_rollback(self.h, current.h))
 
     def add_term(self, ind, t_ind, d):
         dd = np.zeros(2)

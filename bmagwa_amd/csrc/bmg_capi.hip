@@ -152,6 +152,32 @@ extern "C" int bmg_ctx_create_synthetic(int device, uint32_t seed, uint32_t n, u
   return BMG_OK;
 }
 
+extern "C" int bmg_ctx_create_subset(const bmg_ctx* pool, const uint32_t* individuals, uint32_t n_sub,
+                                     const double* y, const int32_t* types, int n_types, bmg_ctx** out) {
+  if (!pool || !individuals || !y || !types || !out) { set_error("null argument"); return BMG_E_ARG; }
+  for (uint32_t i = 0; i < n_sub; ++i)
+    if (individuals[i] >= pool->n) { set_error("individual index past the pool"); return BMG_E_ARG; }
+  bmg_ctx* c = nullptr;
+  int rc = ctx_alloc(pool->device, n_sub, pool->p, out, &c);
+  if (rc) return rc;
+  if ((rc = init_types(c, types, n_types))) { bmg_ctx_destroy(c); *out = nullptr; return rc; }
+  uint32_t* d_sub = nullptr;
+  hipError_t e = hipMalloc(&d_sub, sizeof(uint32_t) * std::max<uint32_t>(n_sub, 1));
+  if (e != hipSuccess) { bmg_ctx_destroy(c); *out = nullptr; return hip_fail(e, "hipMalloc(subset)"); }
+  e = hipMemcpy(d_sub, individuals, sizeof(uint32_t) * n_sub, hipMemcpyHostToDevice);
+  // the pool's genotypes are final (its own stream has finished with them)
+  if (e == hipSuccess) e = hipStreamSynchronize(pool->stream);
+  if (e == hipSuccess) {
+    rc = launch_subset(pool, d_sub, c, c->stream);
+    e = hipStreamSynchronize(c->stream);
+  }
+  hipFree(d_sub);
+  if (rc || e != hipSuccess) { bmg_ctx_destroy(c); *out = nullptr; return rc ? rc : hip_fail(e, "subset"); }
+  if ((rc = ctx_prepare(c, 0))) { bmg_ctx_destroy(c); *out = nullptr; return rc; }
+  if ((rc = bmg_ctx_set_y(c, y))) { bmg_ctx_destroy(c); *out = nullptr; return rc; }
+  return BMG_OK;
+}
+
 extern "C" int bmg_ctx_destroy(bmg_ctx* c) {
   if (!c) return BMG_OK;
   hipSetDevice(c->device);
@@ -325,6 +351,16 @@ extern "C" int bmg_memcpy_h2d(int device, void* dst, const void* src, uint64_t b
   return BMG_OK;
 }
 
+extern "C" int bmg_memcpy_d2h(int device, void* dst, const void* src, uint64_t bytes) {
+  if (!dst || !src) { set_error("null argument"); return BMG_E_ARG; }
+  BMG_HIP(hipSetDevice(device));
+  // the engine's streams do not synchronise with the null stream: wait for
+  // every queued kernel before reading
+  BMG_HIP(hipDeviceSynchronize());
+  BMG_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return BMG_OK;
+}
+
 extern "C" int bmg_copy_timed(int device, const void* src_dev, void* dst_dev, uint64_t bytes, int reps,
                               double* ms_per_launch) {
   if (!src_dev || !dst_dev || !ms_per_launch || reps < 1) { set_error("bad argument"); return BMG_E_ARG; }
@@ -487,8 +523,14 @@ extern "C" __global__ void k_init_chain_arrays(const DevData* data, const ChainC
   }
 }
 
+// Below 256 columns an iterating segment is two launches on the stream: the
+// batch capacity's kernel, which hands a chain whose model comes within one
+// iteration's growth of its capacity to the 256-column kernel at that
+// iteration, then the 256-column kernel, which runs those chains to the end
+// of the segment (its other workgroups return at once).
 static int launch_chain_kernel(bmg_batch* b, uint64_t n_iter, int mode) {
-  KernelArgs a{b->d_data, b->d_consts, b->d_scal, b->d_arrays, b->n_chains, n_iter, mode};
+  const int two = mode == 0 && b->kmax < BMG_KMAXG;
+  KernelArgs a{b->d_data, b->d_consts, b->d_scal, b->d_arrays, b->n_chains, n_iter, mode, two ? 1 : 0, 0};
   switch (b->kmax) {
     case 64: hipLaunchKernelGGL(k_chains<64>, dim3(b->n_chains), dim3(256), 0, b->stream, a); break;
     case 128: hipLaunchKernelGGL(k_chains<128>, dim3(b->n_chains), dim3(256), 0, b->stream, a); break;
@@ -496,6 +538,13 @@ static int launch_chain_kernel(bmg_batch* b, uint64_t n_iter, int mode) {
     default: hipLaunchKernelGGL(k_chains<256>, dim3(b->n_chains), dim3(256), 0, b->stream, a); break;
   }
   BMG_HIP(hipGetLastError());
+  if (two) {
+    KernelArgs a2 = a;
+    a2.role = 2;
+    a2.small_kmax = b->kmax;
+    hipLaunchKernelGGL(k_chains<BMG_KMAXG>, dim3(b->n_chains), dim3(256), 0, b->stream, a2);
+    BMG_HIP(hipGetLastError());
+  }
   return BMG_OK;
 }
 
@@ -528,12 +577,12 @@ static int rb_block_ds(bmg_batch* b, int ds, const std::vector<int>& chains, int
       if (rc) return rc;
     }
     b->n_scans += (t == BMG_AH) ? 2 : 1;
-    if (launch_rb_finalize(b->d_data, b->d_consts, b->d_scal, b->d_arrays, b->d_list, nv, b->kmax, D.p,
+    if (launch_rb_finalize(b->d_data, b->d_consts, b->d_scal, b->d_arrays, b->d_list, nv, BMG_KMAXG, D.p,
                            b->xa, b->xh, b->rsum, b->stream)) {
       set_error("rb finalize launch failed"); return BMG_E_HIP;
     }
     if (learn_now) {
-      rc = learn_labels(b->learn[ds], b->d_data, b->d_consts, b->d_scal, b->d_arrays, b->d_list, nv, b->kmax,
+      rc = learn_labels(b->learn[ds], b->d_data, b->d_consts, b->d_scal, b->d_arrays, b->d_list, nv, BMG_KMAXG,
                         b->xa, b->rsum, b->stream);
       if (rc) return rc;
     }
@@ -583,6 +632,15 @@ static int numeric_error(const std::vector<ChainScal>& cs) {
   for (size_t c = 0; c < cs.size(); ++c)
     if (cs[c].err & ERR_TRACE) {
       set_error("chain " + std::to_string(c) + ": trace buffer full (raise trace_capacity or do_n_iter)");
+      return BMG_E_CAPACITY;
+    }
+  for (size_t c = 0; c < cs.size(); ++c)
+    if (cs[c].err & ERR_CAPACITY) {
+      // the reference stops the same way at its compiled limit
+      // (src/model.hpp:247-262, MAXMODELSIZE)
+      set_error("chain " + std::to_string(c) + " stopped at iteration " + std::to_string(cs[c].n_iter) +
+                ": a proposal needed more than " + std::to_string(BMG_KMAXG) +
+                " model columns, the device limit (set max_model_size)");
       return BMG_E_CAPACITY;
     }
   for (size_t c = 0; c < cs.size(); ++c)
@@ -676,26 +734,21 @@ static int batch_create(bmg_ctx** ctxs, int n_ctx, int n_chains, const int32_t* 
   int kmax = 128;
   b->device = ctxs[0]->device;
   hipSetDevice(b->device);
-  // capacity: the prior's model size bound (e_qg + 6 sd, AH doubles the
-  // columns), or the model-size limit plus one move; the factor leaves LDS
-  // (256) only when a typical model (prior mean plus one move) needs it
-  double need = 0, typical = 0;
+  // capacity of the launch: the smallest compiled one that holds a typical
+  // model (the prior mean plus one move, AH: two columns a locus); a chain
+  // whose model comes within one move of it continues in the 256-column
+  // kernel from that iteration, so the capacity changes speed, not results
+  double typical = 0;
   if (b->cfg.max_model_size < 0) { delete b; set_error("max_model_size must be >= 0"); return BMG_E_ARG; }
   for (int c = 0; c < n_chains; ++c) {
-    double cols = (opts[c].e_qg + 6.0 * std::sqrt(opts[c].var_qg) + 8.0);
     double typ = opts[c].e_qg + (double)opts[c].max_move_size + 8.0;
-    if (b->cfg.max_model_size > 0) {
-      // with a model-size limit a proposal reaches at most limit + max_move_size
-      const double lim = (double)b->cfg.max_model_size + (double)opts[c].max_move_size + 1.0;
-      cols = std::min(cols, lim);
-      typ = std::min(typ, lim);
-    }
+    if (b->cfg.max_model_size > 0)
+      typ = std::min(typ, (double)b->cfg.max_model_size + (double)opts[c].max_move_size + 1.0);
     for (int t = 0; t < opts[c].n_types; ++t)
-      if (opts[c].types[t] == BMG_AH) { cols *= 2; typ *= 2; }
-    need = std::max(need, cols + 1);
+      if (opts[c].types[t] == BMG_AH) typ *= 2;
     typical = std::max(typical, typ + 1);
   }
-  kmax = need <= 64 ? 64 : (need <= 128 ? 128 : ((need <= 160 || typical <= 160) ? 160 : 256));
+  kmax = typical <= 64 ? 64 : (typical <= 128 ? 128 : (typical <= 160 ? 160 : 256));
   if (model_api) kmax = 160;
   if (b->cfg.capacity) {
     if (b->cfg.capacity != 64 && b->cfg.capacity != 128 && b->cfg.capacity != 160 && b->cfg.capacity != 256) {
@@ -744,10 +797,13 @@ static int batch_create(bmg_ctx** ctxs, int n_ctx, int n_chains, const int32_t* 
   b->chain_ds.assign(chain_ctx, chain_ctx + n_chains);
   BMG_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
   // per-chain layout
-  const int K = kmax;
-  uint64_t tr_cap = b->cfg.trace_every_iter ? std::max<uint64_t>(opts[0].do_n_iter, 1) + 8 : 1;
-  uint64_t th_cap = std::max<uint64_t>(opts[0].do_n_iter / opts[0].thin, 1) + 8;
-  uint64_t th_loci_cap = b->cfg.trace_capacity ? b->cfg.trace_capacity : th_cap * 64;
+  // trace buffers sized for the longest chain; thinned loci for models of
+  // up to BMG_KMAXG loci on average unless the caller sizes them
+  uint64_t n_iter_max = 1;
+  for (int c = 0; c < n_chains; ++c) n_iter_max = std::max<uint64_t>(n_iter_max, opts[c].do_n_iter);
+  uint64_t tr_cap = b->cfg.trace_every_iter ? n_iter_max + 8 : 1;
+  uint64_t th_cap = n_iter_max / opts[0].thin + 8;
+  uint64_t th_loci_cap = b->cfg.trace_capacity ? b->cfg.trace_capacity : th_cap * BMG_KMAXG;
   uint64_t pmax = 0, npmax = 0, tmax = 0, wmax = 0;
   for (auto& D : b->h_data) {
     pmax = std::max(pmax, D.p); npmax = std::max<uint64_t>(npmax, D.n_pad); tmax = std::max(tmax, D.tree_size);
@@ -758,10 +814,13 @@ static int batch_create(bmg_ctx** ctxs, int n_ctx, int n_chains, const int32_t* 
   size_t o_mi = add(4 * pmax), o_qa = add(8 * pmax), o_qr = add(8 * pmax);
   size_t o_ta = add(8 * tmax), o_tr = add(8 * tmax);
   size_t o_pp = add(8 * pmax), o_pr = add(8 * pmax), o_prr = add(8 * pmax);
-  size_t o_L = add(8 * (size_t)K * (K + 1) / 2), o_xx = add(8 * (size_t)K * K);
-  size_t o_Lw = add(K > 160 ? 8 * (size_t)K * (K + 1) / 2 : 0);
-  size_t o_xc = add(4 * (size_t)K * wmax);
-  size_t o_yh = add(8 * npmax), o_vec = add(8 * 5 * K), o_idx = add(4 * 8 * K), o_ad = add(8 * 6 * 256);
+  // per-column arrays for BMG_KMAXG columns whatever the launch capacity K:
+  // a chain that leaves K continues in the 256-column kernel on the same state
+  const size_t G = BMG_KMAXG;
+  size_t o_L = add(8 * G * (G + 1) / 2), o_xx = add(8 * G * G);
+  size_t o_Lw = add(8 * G * (G + 1) / 2);
+  size_t o_xc = add(4 * G * wmax);
+  size_t o_yh = add(8 * npmax), o_vec = add(8 * 5 * G), o_idx = add(4 * 8 * G), o_ad = add(8 * 6 * 256);
   size_t o_trm = add(tr_cap), o_trj = add(tr_cap), o_trs = add(tr_cap), o_trz = add(4 * tr_cap),
          o_trl = add(8 * tr_cap);
   size_t o_ths = add(4 * th_cap), o_thl = add(4 * th_loci_cap), o_thll = add(8 * th_cap),
@@ -773,8 +832,8 @@ static int batch_create(bmg_ctx** ctxs, int n_ctx, int n_chains, const int32_t* 
   BMG_HIP(hipMemsetAsync(b->pool, 0, sz * n_chains, b->stream));
   std::vector<ChainScal> scal(n_chains);
   std::vector<double> p0(n_chains);
-  std::vector<double> vec(5 * K), adapt(6 * 256);
-  std::vector<int32_t> idx(8 * K);
+  std::vector<double> vec(5 * G), adapt(6 * 256);
+  std::vector<int32_t> idx(8 * G);
   for (int c = 0; c < n_chains; ++c) {
     const bmg_opts& o = opts[c];
     bmg_ctx* x = ctxs[chain_ctx[c]];
@@ -784,7 +843,7 @@ static int batch_create(bmg_ctx** ctxs, int n_ctx, int n_chains, const int32_t* 
     A.tree_add = (double*)(base + o_ta); A.tree_rem = (double*)(base + o_tr);
     A.p_proposal = (double*)(base + o_pp); A.p_rao = (double*)(base + o_pr); A.p_r = (double*)(base + o_prr);
     A.Lcur = (double*)(base + o_L); A.xxs = (double*)(base + o_xx); A.y_hat = (double*)(base + o_yh);
-    A.Lwork = K > 160 ? (double*)(base + o_Lw) : nullptr;
+    A.Lwork = (double*)(base + o_Lw);
     A.xcol = (uint32_t*)(base + o_xc);
     A.vec = (double*)(base + o_vec); A.idx = (int32_t*)(base + o_idx); A.adapt = (double*)(base + o_ad);
     A.tr_move = base + o_trm; A.tr_jd = base + o_trj; A.tr_ms = base + o_trs;
@@ -802,6 +861,9 @@ static int batch_create(bmg_ctx** ctxs, int n_ctx, int n_chains, const int32_t* 
     Kc.max_nbh = o.max_SNP_neighborhood_size;
     Kc.dr = (int32_t)std::min<uint64_t>(Kc.max_ms, o.delay_rejection);
     Kc.max_size = b->cfg.max_model_size > 0 ? b->cfg.max_model_size : INT32_MAX;
+    // a move adds at most max_move_size loci (two columns each for AH);
+    // one column of slack
+    Kc.col_reserve = (int32_t)Kc.max_ms * (x->types[0] == BMG_AH ? 2 : 1) + 1;
     Kc.adapt_p_ms = o.adapt_p_move_size && Kc.max_ms > 1;
     Kc.adaptation = o.adaptation; Kc.flat = o.flat_proposal_dist;
     Kc.trace_iter = b->cfg.trace_every_iter;
@@ -852,12 +914,12 @@ static int batch_create(bmg_ctx** ctxs, int n_ctx, int n_chains, const int32_t* 
     double sy = 0.0;
     for (uint32_t i = 0; i < x->n; ++i) sy += x->y_host[i];
     vec[0] = sy;
-    vec[2 * K] = o.inv_tau2_e_const_val;
-    idx[3 * K] = 0;        // slot of the constant column
-    idx[4 * K] = -1;       // its "SNP"
-    idx[5 * K] = 4;        // covariate marker (src/model.hpp:98-101)
-    BMG_HIP(hipMemcpyAsync(A.vec, vec.data(), sizeof(double) * 5 * K, hipMemcpyHostToDevice, b->stream));
-    BMG_HIP(hipMemcpyAsync(A.idx, idx.data(), sizeof(int32_t) * 8 * K, hipMemcpyHostToDevice, b->stream));
+    vec[2 * G] = o.inv_tau2_e_const_val;
+    idx[3 * G] = 0;        // slot of the constant column
+    idx[4 * G] = -1;       // its "SNP"
+    idx[5 * G] = 4;        // covariate marker (src/model.hpp:98-101)
+    BMG_HIP(hipMemcpyAsync(A.vec, vec.data(), sizeof(double) * 5 * G, hipMemcpyHostToDevice, b->stream));
+    BMG_HIP(hipMemcpyAsync(A.idx, idx.data(), sizeof(int32_t) * 8 * G, hipMemcpyHostToDevice, b->stream));
     double nn = (double)x->n;
     BMG_HIP(hipMemcpyAsync(A.xxs, &nn, sizeof(double), hipMemcpyHostToDevice, b->stream));
     std::fill(adapt.begin(), adapt.end(), 0.0);
@@ -1026,7 +1088,7 @@ static int build_graph(bmg_batch* b) {
       rc = launch_scan(c, b->r, nv, t == BMG_AH ? 0 : t, digits, b->xa, b->stream);
       if (!rc && t == BMG_AH) rc = launch_scan(c, b->r, nv, 1, digits, b->xh, b->stream);
       scans += (t == BMG_AH) ? 2 : 1;
-      if (!rc && launch_rb_finalize(b->d_data, b->d_consts, b->d_scal, b->d_arrays, list, nv, b->kmax, D.p,
+      if (!rc && launch_rb_finalize(b->d_data, b->d_consts, b->d_scal, b->d_arrays, list, nv, BMG_KMAXG, D.p,
                                     b->xa, b->xh, b->rsum, b->stream)) {
         set_error("rb finalize launch failed"); rc = BMG_E_HIP;
       }
@@ -1478,6 +1540,84 @@ extern "C" int bmg_model_create(bmg_ctx* ctx, const bmg_model_prior* prior, bmg_
   return BMG_OK;
 }
 
+extern "C" int bmg_model_create_sampler(bmg_ctx* ctx, const bmg_opts* opts, bmg_model** out) {
+  if (!ctx || !opts || !out) { set_error("null argument"); return BMG_E_ARG; }
+  if (ctx->n_types < 1) { set_error("dataset has no effect type"); return BMG_E_ARG; }
+  // the model of one Sampler: its prior and its random stream (key = seed,
+  // chain) from the options, as a batch chain before set-up
+  bmg_opts o = *opts;
+  o.do_n_iter = 1;
+  int32_t zero = 0;
+  bmg_batch_cfg cfg{};
+  bmg_batch* b = nullptr;
+  int rc = batch_create(&ctx, 1, 1, &zero, &o, &cfg, 1, &b);
+  if (rc) return rc;
+  bmg_model* m = new bmg_model();
+  m->b = b;
+  ModelOpArgs a{};
+  a.op = 2;
+  rc = launch_model_op(b, a);
+  if (rc) { bmg_batch_destroy(b); delete m; return rc; }
+  *out = m;
+  return BMG_OK;
+}
+
+extern "C" int bmg_model_sample_beta_sigma2(bmg_model* m) {
+  if (!m) { set_error("null model"); return BMG_E_ARG; }
+  ModelOpArgs a{};
+  a.op = 4;
+  int rc = launch_model_op(m->b, a);
+  if (rc) return rc;
+  ChainScal s;
+  BMG_HIP(hipMemcpy(&s, m->b->d_scal, sizeof(ChainScal), hipMemcpyDeviceToHost));
+  if (s.err & ERR_SIGMA2) { set_error("sigma2 draw did not finish (non-finite residual sum)"); return BMG_E_NUMERIC; }
+  return BMG_OK;
+}
+
+extern "C" int bmg_model_compute_pve(bmg_model* m, double pves[3]) {
+  if (!m || !pves) { set_error("null argument"); return BMG_E_ARG; }
+  ModelOpArgs a{};
+  a.op = 5;
+  int rc = launch_model_op(m->b, a);
+  if (rc) return rc;
+  ChainScal s;
+  BMG_HIP(hipMemcpy(&s, m->b->d_scal, sizeof(ChainScal), hipMemcpyDeviceToHost));
+  for (int i = 0; i < 3; ++i) pves[i] = s.pves[i];
+  return BMG_OK;
+}
+
+extern "C" int bmg_model_beta(bmg_model* m, double* beta, double* sigma2, double* y_hat, int* len) {
+  if (!m) { set_error("null model"); return BMG_E_ARG; }
+  bmg_batch* b = m->b;
+  ChainScal s;
+  BMG_HIP(hipMemcpy(&s, b->d_scal, sizeof(ChainScal), hipMemcpyDeviceToHost));
+  const ChainArrays& A = b->h_arrays[0];
+  if (beta && s.beta_len > 0)
+    BMG_HIP(hipMemcpy(beta, A.vec + 3 * BMG_KMAXG, sizeof(double) * s.beta_len, hipMemcpyDeviceToHost));
+  if (sigma2) *sigma2 = s.sigma2;
+  if (y_hat) BMG_HIP(hipMemcpy(y_hat, A.y_hat, sizeof(double) * b->h_data[0].n, hipMemcpyDeviceToHost));
+  if (len) *len = s.beta_len;
+  return BMG_OK;
+}
+
+extern "C" int bmg_model_rb(bmg_model* m, const double* y_hat, double* p_r) {
+  if (!m || !y_hat || !p_r) { set_error("null argument"); return BMG_E_ARG; }
+  const bmg_ctx* x = m->b->ctxs[0];
+  if (x->n_types != 1) {
+    set_error("the RB scan of a model takes one effect type (several: the type softmax is not built)");
+    return BMG_E_UNSUPPORTED;
+  }
+  return bmg_batch_rb(m->b, 0, y_hat, p_r);
+}
+
+extern "C" int bmg_model_commit(bmg_model* current, const bmg_model* proposed) {
+  return bmg_model_copy(current, proposed);
+}
+
+extern "C" int bmg_model_rollback(bmg_model* proposed, const bmg_model* current) {
+  return bmg_model_copy(proposed, current);
+}
+
 extern "C" int bmg_model_destroy(bmg_model* m) {
   if (!m) return BMG_OK;
   bmg_batch_destroy(m->b);
@@ -1547,7 +1687,7 @@ extern "C" int bmg_model_set_d(bmg_model* m, int x_ind, double val) {
   if (!m) { set_error("null model"); return BMG_E_ARG; }
   const int K = m->b->kmax;
   if (x_ind < 0 || x_ind >= K) { set_error("x_ind out of range"); return BMG_E_ARG; }
-  BMG_HIP(hipMemcpy(m->b->h_arrays[0].vec + 2 * K + x_ind, &val, sizeof(double), hipMemcpyHostToDevice));
+  BMG_HIP(hipMemcpy(m->b->h_arrays[0].vec + 2 * BMG_KMAXG + x_ind, &val, sizeof(double), hipMemcpyHostToDevice));
   return BMG_OK;
 }
 
@@ -1566,7 +1706,7 @@ extern "C" int bmg_model_get(bmg_model* m, double* xx, double* l, double* xy, do
                              double scal[6]) {
   if (!m) { set_error("null model"); return BMG_E_ARG; }
   bmg_batch* b = m->b;
-  const int K = b->kmax;
+  const int K = BMG_KMAXG;   // per-column arrays are laid out for BMG_KMAXG columns
   ChainScal s;
   BMG_HIP(hipMemcpy(&s, b->d_scal, sizeof(ChainScal), hipMemcpyDeviceToHost));
   int k = s.cols;

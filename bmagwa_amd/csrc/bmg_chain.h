@@ -51,6 +51,7 @@ struct ChainConst {
   double p_learned;      // probability of a learned swap per iteration
   int32_t dr;            // delayed-rejection move-size limit (0: none)
   int32_t max_size;      // largest model (SNPs) with nonzero prior
+  int32_t col_reserve;   // most columns one iteration can add to the current model
 };
 
 // Mutable per-chain scalars (device global memory).
@@ -85,21 +86,28 @@ struct ChainScal {
   int32_t err;
   uint64_t tr_count, th_count, th_loci_len;
   double tot_add, tot_rem;   // sum-tree totals
+  // capacity: a chain whose model comes within col_reserve columns of the
+  // launch's capacity continues in the 256-column kernel (big = 1) from the
+  // same iteration; seg_end is the iteration the current segment ends at
+  int32_t big;
+  uint64_t seg_end;
 };
 
-// Per-chain device arrays.
+// Per-chain device arrays.  Every per-column array is laid out for
+// BMG_KMAXG columns whatever the launch's capacity, so a chain's state is
+// read the same way by every compiled capacity.
 struct ChainArrays {
   int32_t* model_inds;       // p
   double* q_add; double* q_rem;          // p
   double* tree_add; double* tree_rem;    // tree_size each
   double* p_proposal; double* p_rao; double* p_r;  // p
-  double* Lcur;              // KMAX (KMAX+1) / 2
-  double* Lwork;             // KMAX (KMAX+1) / 2 working factor when it is not in LDS (else null)
-  double* xxs;               // KMAX x KMAX slot matrix
-  uint32_t* xcol;            // KMAX x W32: packed genotypes of each slot's column
+  double* Lcur;              // packed upper factor, BMG_KMAXG (BMG_KMAXG+1) / 2
+  double* Lwork;             // the same size: working factor of the 256-column kernel
+  double* xxs;               // BMG_KMAXG x BMG_KMAXG slot matrix
+  uint32_t* xcol;            // BMG_KMAXG x W32: packed genotypes of each slot's column
   double* y_hat;             // n_pad
-  double* vec;               // 5 x KMAX: xy, v, d, beta, mu
-  int32_t* idx;              // 8 x KMAX: loci, x_ind1, x_ind2, slot, snp, xtype, tind, -
+  double* vec;               // 5 x BMG_KMAXG: xy, v, d, beta, mu
+  int32_t* idx;              // 8 x BMG_KMAXG: loci, x_ind1, x_ind2, slot, snp, xtype, tind, -
   double* adapt;             // 6 x 256: q_pms, q_pms_nbs, q_pms_nbc, r_ms_sum, r_ms_n, q_p0_ms
   uint8_t* tr_move; uint8_t* tr_jd; uint8_t* tr_ms; uint32_t* tr_size; double* tr_ll;
   uint32_t* th_size; uint32_t* th_loci; double* th_ll; double* th_s2; double* th_alpha;
@@ -107,14 +115,15 @@ struct ChainArrays {
   double* dr_lp;         // 2^dr second-stage log-probabilities (null: dr == 0)
 };
 
-// error flags of a chain; every flag from ERR_NONFINITE on stops the chain
+// error flags of a chain; the flags in ERR_STOP stop the chain
 enum {
-  ERR_CAPACITY = 1, ERR_TRACE = 2, ERR_TREE = 4,
+  ERR_CAPACITY = 1,      // a proposal needed more than BMG_KMAXG columns
+  ERR_TRACE = 2, ERR_TREE = 4,
   ERR_NONFINITE = 8,     // tau2/alpha refresh left a non-finite log-likelihood
   ERR_SIGMA2 = 16,       // sigma2 draw not finite and positive
   ERR_ALPHA = 32,        // alpha draw did not finish
   ERR_WALK = 64,         // a neighbourhood walk left the genome
-  ERR_STOP = ERR_NONFINITE | ERR_SIGMA2 | ERR_ALPHA | ERR_WALK
+  ERR_STOP = ERR_CAPACITY | ERR_NONFINITE | ERR_SIGMA2 | ERR_ALPHA | ERR_WALK
 };
 
 }  // namespace bmg
@@ -128,6 +137,11 @@ struct KernelArgs {
   int n_chains;
   uint64_t n_iter;     // iterations to run in this launch
   int mode;            // 0 iterate, 1 set-up
+  // 0: the only chain kernel of the segment; 1: the launch capacity's kernel,
+  // followed by 2: the 256-column kernel, which runs the chains with big = 1
+  // up to their seg_end
+  int role;
+  int small_kmax;      // role 2: the capacity of the role-1 kernel
 };
 
 // one Model method on a one-chain state (the drop-in Model API)
@@ -136,7 +150,8 @@ struct ModelOpArgs {
   const bmg::ChainConst* consts;
   bmg::ChainScal* scal;
   const bmg::ChainArrays* arrays;
-  int op;              // 0 add, 1 remove, 2 compute_log_likelihood, 3 swapadj
+  int op;              // 0 add, 1 remove, 2 compute_log_likelihood, 3 swapadj,
+                       // 4 sample_beta_sigma2, 5 compute_pve
   uint32_t ind;
   int t_ind, model_ind;
   double d0, d1;

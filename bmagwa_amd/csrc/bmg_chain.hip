@@ -178,6 +178,15 @@ __device__ __forceinline__ double dot_y_block(const uint32_t* __restrict__ w, ui
   return block_sum(s, red);
 }
 
+// The Makefile compiles this file once per model capacity (BMG_KMAX_ONLY =
+// 64, 128, 160, 256) so the four instantiations build in parallel; the
+// non-template code lives in the 64-column unit.
+#ifndef BMG_KMAX_ONLY
+#define BMG_KMAX_ONLY 0
+#endif
+#define BMG_INST(K) (BMG_KMAX_ONLY == 0 || BMG_KMAX_ONLY == (K))
+
+#if BMG_INST(64)
 __global__ __launch_bounds__(BMG_BLOCK) void k_xy_all(DevData D, int coding, double* __restrict__ out) {
   __shared__ double red[8];
   for (uint64_t j = blockIdx.x; j < D.p; j += gridDim.x) {
@@ -185,6 +194,7 @@ __global__ __launch_bounds__(BMG_BLOCK) void k_xy_all(DevData D, int coding, dou
     if (threadIdx.x == 0) out[j] = v;
   }
 }
+#endif
 
 template <int KMAX>
 __device__ __forceinline__ double snp_dot_y(Ctx<KMAX>& C, uint64_t snp, int coding) {
@@ -420,11 +430,17 @@ __device__ __forceinline__ void drotg(double& a, double& b, double& c, double& s
 // (rows below j + 1 are not rotated yet) and stores only the finished row,
 // and the owner of column j hands the rotation's inputs to all lanes with a
 // cross-lane read, so no step waits on an LDS round trip.
+//
+// v (U^T v = X^T y) goes through the same rotations and sign fix as the
+// factor's rows, so after the removal it solves the new factor's system
+// without a second triangular solve (the reference re-solves,
+// src/model.hpp:530-545; the two agree to rounding).
 template <int KMAX>
 __device__ __forceinline__ void chol_downdate(Smem<KMAX>& S, int m, int cr) {
   constexpr int NT = (KMAX + 63) / 64;
   if (wave_id() == 0) {
     const int lane = lane_id();
+    double vt = S.v[cr], vb = S.v[cr + 1];   // rows (j, j + 1) of v, all lanes
     double top[NT], nxt[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
@@ -441,9 +457,16 @@ __device__ __forceinline__ void chol_downdate(Smem<KMAX>& S, int m, int cr) {
         const int owner = j & 63;
         // every lane generates the rotation from the same two values
         double a = readlane_d(top[ti], owner), b = readlane_d(nxt[ti], owner);
+        const double vbn = j + 2 < m ? S.v[j + 2] : 0.0;   // next row of v, read ahead
         double cc, ss;
         drotg(a, b, cc, ss);
         if (lane == owner) S.L[P(j + 1) + j] = a;
+        {
+          const double tv = cc * vt + ss * vb;
+          vt = cc * vb - ss * vt;
+          if (lane == 0) S.v[j] = tv;
+          vb = vbn;
+        }
 #pragma unroll
         for (int t = ti; t < NT; ++t) {
           const int jl = lane + 64 * t;
@@ -466,6 +489,7 @@ __device__ __forceinline__ void chol_downdate(Smem<KMAX>& S, int m, int cr) {
     double* col = S.L + P(jl + 1);
     for (int i = cr; i <= jl; ++i)
       if (S.work[i] < 0) col[i] = -col[i];
+    if (S.work[jl] < 0) S.v[jl] = -S.v[jl];
   }
   __syncthreads();
   // compaction: logical column jl rows 0..jl <- physical column jl+1
@@ -493,27 +517,35 @@ __device__ __forceinline__ void chol_downdate(Smem<KMAX>& S, int m, int cr) {
 }
 
 // Full factorisation of S.L (m x m, in place), right-looking; returns 0 if
-// not positive definite.
+// not positive definite.  Step k: every thread takes the pivot, the thread
+// of column i > k scales its row-k entry (kept in S.work for the others),
+// then walks its own column down from row k + 1 subtracting work[j] * u_ki.
+// Per element the subtractions come in k order, each a multiply and a
+// subtract, as in the sequential loop; two barriers a step, no index
+// arithmetic beyond the packed offsets.
 template <int KMAX>
 __device__ __forceinline__ int chol_full(Smem<KMAX>& S, int m) {
-  int ok = 1;
+  __syncthreads();
   for (int k = 0; k < m; ++k) {
-    __syncthreads();
-    double dk = S.L[P(k) + k];
-    if (!(dk > 0.0)) { ok = 0; break; }
-    dk = sqrt(dk);
+    const double akk = S.L[P(k) + k];
+    if (!(akk > 0.0)) { __syncthreads(); return 0; }
+    const double dk = sqrt(akk);
+    const int i = k + 1 + (int)threadIdx.x;   // this thread's column (m <= 256)
+    double u = 0.0;
+    if (i < m) {
+      u = S.L[P(i) + k] / dk;
+      S.L[P(i) + k] = u;
+      S.work[i] = u;
+    }
     __syncthreads();
     if (threadIdx.x == 0) S.L[P(k) + k] = dk;
-    for (int i = k + 1 + threadIdx.x; i < m; i += BMG_BLOCK) S.L[P(i) + k] = S.L[P(i) + k] / dk;
-    __syncthreads();
-    int w = m - k - 1;
-    for (int idx = threadIdx.x; idx < w * w; idx += BMG_BLOCK) {
-      int j = k + 1 + idx / w, i = k + 1 + idx % w;
-      if (j <= i) S.L[P(i) + j] = S.L[P(i) + j] - S.L[P(j) + k] * S.L[P(i) + k];
+    if (i < m) {
+      double* col = S.L + P(i);
+      for (int j = k + 1; j <= i; ++j) col[j] = col[j] - S.work[j] * u;
     }
+    __syncthreads();
   }
-  __syncthreads();
-  return ok;
+  return 1;
 }
 
 // ------------------------------------------------------------- sum trees --
@@ -677,8 +709,8 @@ __device__ __forceinline__ void update_on_add(Ctx<KMAX>& C, uint32_t ind, int mo
   // record the new X^T X row/column in the slot matrix
   for (int c = threadIdx.x; c <= col; c += BMG_BLOCK) {
     int sc = c == col ? slot : S.slot[c];
-    C.A.xxs[(size_t)slot * KMAX + sc] = S.work[c];
-    C.A.xxs[(size_t)sc * KMAX + slot] = S.work[c];
+    C.A.xxs[(size_t)slot * BMG_KMAXG + sc] = S.work[c];
+    C.A.xxs[(size_t)sc * BMG_KMAXG + slot] = S.work[c];
   }
   if (threadIdx.x == 0) S.xy[col] = xy;
   // bordered update (src/symmmatrix.cpp:144-164)
@@ -793,12 +825,9 @@ __device__ __forceinline__ void update_on_remove_impl(Ctx<KMAX>& C, int x_ind) {
     R.syx += vx * vx;
   } else {
     const uint64_t ta = wall_clock64();
+    // v follows the downdate's rotations (no re-solve)
     chol_downdate(S, k, x_ind);
     const uint64_t tb = wall_clock64();
-    for (int i = threadIdx.x; i < R.cols; i += BMG_BLOCK) S.v[i] = S.xy[i];
-    __syncthreads();
-    if (wave_id() == 0) trsv_ut_w0(S, S.v, R.cols);
-    __syncthreads();
     if (threadIdx.x == 0 && C.prof) { C.prof[16] += tb - ta; C.prof[17] += wall_clock64() - tb; }
     double ld = 0.0, vv = 0.0;
     for (int i = threadIdx.x; i < R.cols; i += BMG_BLOCK) {
@@ -869,7 +898,7 @@ __device__ __forceinline__ void compute_log_likelihood(Ctx<KMAX>& C) {
     while (P(j + 1) <= e) ++j;
     while (P(j) > e) --j;
     int i = e - P(j);
-    double v = C.A.xxs[(size_t)S.slot[i] * KMAX + S.slot[j]];
+    double v = C.A.xxs[(size_t)S.slot[i] * BMG_KMAXG + S.slot[j]];
     if (i == j) v += S.d[i];
     S.L[e] = v;
   }
@@ -2247,22 +2276,22 @@ __device__ __forceinline__ void load_chain(Ctx<KMAX>& C, const ChainScal& CS) {
   for (int e = threadIdx.x; e < P(R.cols); e += BMG_BLOCK) S.L[e] = C.A.Lcur[e];
   for (int i = threadIdx.x; i < KMAX; i += BMG_BLOCK) {
     S.xy[i] = S.c_xy[i] = vec[i];
-    S.v[i] = S.c_v[i] = vec[KMAX + i];
-    S.d[i] = S.c_d[i] = vec[2 * KMAX + i];
-    S.beta[i] = vec[3 * KMAX + i];
-    S.mu[i] = vec[4 * KMAX + i];
+    S.v[i] = S.c_v[i] = vec[BMG_KMAXG + i];
+    S.d[i] = S.c_d[i] = vec[2 * BMG_KMAXG + i];
+    S.beta[i] = vec[3 * BMG_KMAXG + i];
+    S.mu[i] = vec[4 * BMG_KMAXG + i];
     S.loci[i] = S.c_loci[i] = (uint32_t)idx[i];
-    S.xi1[i] = S.c_xi1[i] = idx[KMAX + i];
-    S.xi2[i] = S.c_xi2[i] = idx[2 * KMAX + i];
-    S.slot[i] = S.c_slot[i] = idx[3 * KMAX + i];
-    S.snp[i] = S.c_snp[i] = idx[4 * KMAX + i];
-    S.xtype[i] = S.c_xtype[i] = (int8_t)idx[5 * KMAX + i];
-    S.tind[i] = S.c_tind[i] = (int8_t)idx[6 * KMAX + i];
+    S.xi1[i] = S.c_xi1[i] = idx[BMG_KMAXG + i];
+    S.xi2[i] = S.c_xi2[i] = idx[2 * BMG_KMAXG + i];
+    S.slot[i] = S.c_slot[i] = idx[3 * BMG_KMAXG + i];
+    S.snp[i] = S.c_snp[i] = idx[4 * BMG_KMAXG + i];
+    S.xtype[i] = S.c_xtype[i] = (int8_t)idx[5 * BMG_KMAXG + i];
+    S.tind[i] = S.c_tind[i] = (int8_t)idx[6 * BMG_KMAXG + i];
   }
   if (threadIdx.x < (KMAX + 63) / 64) {
     uint64_t m = 0;
     for (int i = 0; i < R.cols; ++i)
-      if ((idx[3 * KMAX + i] >> 6) == (int)threadIdx.x) m |= 1ull << (idx[3 * KMAX + i] & 63);
+      if ((idx[3 * BMG_KMAXG + i] >> 6) == (int)threadIdx.x) m |= 1ull << (idx[3 * BMG_KMAXG + i] & 63);
     S.slots_cur[threadIdx.x] = S.slots_prop[threadIdx.x] = m;
   }
   // sum-tree levels >= 2
@@ -2284,11 +2313,11 @@ __device__ __forceinline__ void store_chain(Ctx<KMAX>& C, ChainScal& CS) {
   double* vec = C.A.vec;
   int32_t* idx = C.A.idx;
   for (int i = threadIdx.x; i < KMAX; i += BMG_BLOCK) {
-    vec[i] = S.c_xy[i]; vec[KMAX + i] = S.c_v[i]; vec[2 * KMAX + i] = S.c_d[i];
-    vec[3 * KMAX + i] = S.beta[i]; vec[4 * KMAX + i] = S.mu[i];
-    idx[i] = (int32_t)S.c_loci[i]; idx[KMAX + i] = S.c_xi1[i]; idx[2 * KMAX + i] = S.c_xi2[i];
-    idx[3 * KMAX + i] = S.c_slot[i]; idx[4 * KMAX + i] = S.c_snp[i];
-    idx[5 * KMAX + i] = S.c_xtype[i]; idx[6 * KMAX + i] = S.c_tind[i];
+    vec[i] = S.c_xy[i]; vec[BMG_KMAXG + i] = S.c_v[i]; vec[2 * BMG_KMAXG + i] = S.c_d[i];
+    vec[3 * BMG_KMAXG + i] = S.beta[i]; vec[4 * BMG_KMAXG + i] = S.mu[i];
+    idx[i] = (int32_t)S.c_loci[i]; idx[BMG_KMAXG + i] = S.c_xi1[i]; idx[2 * BMG_KMAXG + i] = S.c_xi2[i];
+    idx[3 * BMG_KMAXG + i] = S.c_slot[i]; idx[4 * BMG_KMAXG + i] = S.c_snp[i];
+    idx[5 * BMG_KMAXG + i] = S.c_xtype[i]; idx[6 * BMG_KMAXG + i] = S.c_tind[i];
   }
   if (C.D.top >= 2) {
     uint64_t o2 = C.D.lev_off[2];
@@ -2351,6 +2380,17 @@ __device__ __forceinline__ void run_iterations(Ctx<KMAX>& C, ChainScal& CS, uint
     // a chain whose current model turned non-finite stops here; the host
     // reports it (the oracle stops with an error at the same point)
     if (R.err & ERR_STOP) break;
+    // a model within one iteration's growth of this kernel's capacity
+    // continues in the 256-column kernel from this iteration, before any
+    // draw of it, so the trajectory is the one a 256-column launch gives
+    if (KMAX < BMG_KMAXG && R.c_cols + K.col_reserve > KMAX) {
+      if (threadIdx.x == 0) {
+        CS.big = 1;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) CS.prof[q] += pr[q];
+      }
+      break;
+    }
     const uint64_t t_it = wall_clock64();
     uint64_t t0 = t_it;
     if ((iter + 1) % K.n_st == 0) {
@@ -2510,17 +2550,42 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     store_chain(C, CS);
     return;
   }
+  // role 1 fixes the segment's end for every chain; role 2 runs the chains
+  // that left the role-1 kernel's capacity, up to that end
+  if (args.role == 1 && threadIdx.x == 0) CS.seg_end = CS.n_iter + args.n_iter;
   if (CS.rb_pending) return;   // the RB block must run first
+  uint64_t n_iter = args.n_iter;
+  if (args.role == 1 && CS.big) return;
+  if (args.role == 2) {
+    if (!CS.big || CS.seg_end <= CS.n_iter) return;
+    n_iter = CS.seg_end - CS.n_iter;
+  }
   load_chain(C, CS);
-  run_iterations(C, CS, args.n_iter);
+  run_iterations(C, CS, n_iter);
   store_chain(C, CS);
+  if (args.role == 2 && threadIdx.x == 0) {
+    // back to the launch capacity once the model is well inside it and
+    // every column's slot exists there
+    int top = 0;
+    for (int i = 0; i < R.c_cols; ++i) top = max(top, (int)S.c_slot[i]);
+    if (!(R.err & ERR_STOP) && R.c_cols + K.col_reserve + 8 <= args.small_kmax && top < args.small_kmax)
+      CS.big = 0;
+  }
 }
 
 // explicit instantiations used by the host
+#if BMG_INST(64)
 template __global__ void k_chains<64>(KernelArgs);
+#endif
+#if BMG_INST(128)
 template __global__ void k_chains<128>(KernelArgs);
+#endif
+#if BMG_INST(160)
 template __global__ void k_chains<160>(KernelArgs);
+#endif
+#if BMG_INST(256)
 template __global__ void k_chains<256>(KernelArgs);
+#endif
 
 // ------------------------------------------------------- model op kernel --
 
@@ -2568,6 +2633,7 @@ __global__ __launch_bounds__(256) void k_model_op(ModelOpArgs a) {
   Reg R;
   Ctx<KMAX> C{S, R, D, K, A, nullptr};
   load_chain(C, CS);
+  R.err &= ~ERR_CAPACITY;   // a capacity error belongs to the call that hit it
   if (a.op == 0) {
     double d2[2] = {a.d0, a.d1};
     add_term(C, a.ind, a.t_ind, d2);
@@ -2575,20 +2641,35 @@ __global__ __launch_bounds__(256) void k_model_op(ModelOpArgs a) {
     remove_term(C, a.model_ind);
   } else if (a.op == 2) {
     compute_log_likelihood(C);
-  } else {
+  } else if (a.op == 3) {
     swapadj_w0(S, R.cols, a.model_ind);
     R.lmin = a.model_ind;
+  } else if (a.op == 4) {
+    sample_beta_sigma2(C);   // from the model's own stream
+  } else if (a.op == 5) {
+    double pv[3];
+    compute_pve(C, pv);      // leaves y_hat = X beta
+    if (threadIdx.x == 0) { CS.pves[0] = pv[0]; CS.pves[1] = pv[1]; CS.pves[2] = pv[2]; }
   }
   R.lmin = 0;
   commit(C);
   store_chain(C, CS);
 }
 
+#if BMG_INST(64)
 template __global__ void k_model_op<64>(ModelOpArgs);
+#endif
+#if BMG_INST(128)
 template __global__ void k_model_op<128>(ModelOpArgs);
+#endif
+#if BMG_INST(160)
 template __global__ void k_model_op<160>(ModelOpArgs);
+#endif
+#if BMG_INST(256)
 template __global__ void k_model_op<256>(ModelOpArgs);
+#endif
 
+#if BMG_INST(64)
 namespace bmg {
 
 int launch_xy_all(const DevData& D, int coding, double* out, hipStream_t s) {
@@ -2598,3 +2679,4 @@ int launch_xy_all(const DevData& D, int coding, double* out, hipStream_t s) {
 }
 
 }  // namespace bmg
+#endif

@@ -473,6 +473,36 @@ int ensure(void** ptr, size_t* have, size_t need) {
   return BMG_OK;
 }
 
+// out SNP j, word w, field f <- pool SNP j, individual sub[16 w + f] (code
+// 00 past n_sub); one thread per output word, the pool row's words read
+// through L2 (a gather of 2-bit fields within one SNP's row)
+__global__ void k_subset(const uint32_t* __restrict__ pool, uint32_t W32p, const uint32_t* __restrict__ sub,
+                         uint32_t n_sub, uint64_t p, uint32_t W32, uint32_t* __restrict__ out) {
+  const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= W32) return;
+  for (uint64_t j = blockIdx.y; j < p; j += gridDim.y) {
+    const uint32_t* row = pool + j * W32p;
+    uint32_t v = 0;
+#pragma unroll
+    for (int f = 0; f < 16; ++f) {
+      const uint32_t i = w * 16 + f;
+      if (i < n_sub) {
+        const uint32_t k = sub[i];
+        v |= ((row[k >> 4] >> (2 * (k & 15))) & 3u) << (2 * f);
+      }
+    }
+    out[j * W32 + w] = v;
+  }
+}
+
+int launch_subset(const bmg_ctx* pool, const uint32_t* sub_dev, bmg_ctx* c, hipStream_t s) {
+  const unsigned gy = (unsigned)std::min<uint64_t>(c->p, 65535);
+  hipLaunchKernelGGL(k_subset, dim3((c->W32 + 255) / 256, gy), dim3(256), 0, s, pool->g, pool->W32, sub_dev,
+                     c->n, c->p, c->W32, c->g);
+  BMG_HIP(hipGetLastError());
+  return BMG_OK;
+}
+
 int ctx_prepare(bmg_ctx* c, int recode) {
   hipStream_t s = c->stream;
   if (recode) {
@@ -540,22 +570,42 @@ int launch_synth(bmg_ctx* c, uint32_t seed, const double* thresh_dev, const doub
   return BMG_OK;
 }
 
-static int scan_prepare(bmg_ctx* c, const double* r_dev, int nvec, int digits, hipStream_t s) {
+// the scan's scratch: digit planes (NC x n_pad int8), the int32
+// accumulator ((p + 1024) x 16) and the per-vector scales (16 doubles)
+struct ScanBufs {
+  int8_t* dig;
+  int32_t* acc;
+  double* scale;
+};
+
+static int scan_check(int nvec, int digits) {
   if (nvec < 1 || digits < 1 || digits > 6 || nvec * digits > 16) {
     set_error("scan: need 1 <= nvec*digits <= 16, digits <= 6");
     return BMG_E_ARG;
   }
+  return BMG_OK;
+}
+
+static int scan_digits(bmg_ctx* c, const double* r_dev, int nvec, int digits, const ScanBufs& B,
+                       hipStream_t s) {
+  hipLaunchKernelGGL(k_absmax, dim3(nvec), dim3(256), 0, s, r_dev, c->n_pad, nvec, B.scale, digits);
+  hipLaunchKernelGGL(k_digits, dim3((c->n_pad + 255) / 256, nvec), dim3(256), 0, s, r_dev,
+                     c->n_pad, nvec, digits, B.scale, B.dig);
+  BMG_HIP(hipGetLastError());
+  return BMG_OK;
+}
+
+static int scan_prepare(bmg_ctx* c, const double* r_dev, int nvec, int digits, hipStream_t s, ScanBufs* B) {
+  int rc = scan_check(nvec, digits);
+  if (rc) return rc;
   int NC = nvec * digits;
   if (ensure((void**)&c->dig, &c->dig_bytes, (size_t)NC * c->n_pad)) return BMG_E_HIP;
   // p rows plus one block of the largest scan shape (16 waves x 4 tiles x 16
   // SNPs) for the unguarded stores of the last block
   if (ensure((void**)&c->acc, &c->acc_bytes, (size_t)(c->p + 1024) * 16 * sizeof(int32_t))) return BMG_E_HIP;
   if (!c->scale) BMG_HIP(hipMalloc(&c->scale, sizeof(double) * 16));
-  hipLaunchKernelGGL(k_absmax, dim3(nvec), dim3(256), 0, s, r_dev, c->n_pad, nvec, c->scale, digits);
-  hipLaunchKernelGGL(k_digits, dim3((c->n_pad + 255) / 256, nvec), dim3(256), 0, s, r_dev,
-                     c->n_pad, nvec, digits, c->scale, c->dig);
-  BMG_HIP(hipGetLastError());
-  return BMG_OK;
+  *B = ScanBufs{c->dig, c->acc, c->scale};
+  return scan_digits(c, r_dev, nvec, digits, *B, s);
 }
 
 struct ScanShape {
@@ -589,52 +639,53 @@ static uint32_t tile_for(int NC, size_t bytes) {
 
 // zero the accumulator when several individual tiles add into it (a fill
 // kernel, so the sequence stays valid inside captured graphs)
-static void scan_clear(bmg_ctx* c, const ScanShape& sh, hipStream_t s) {
+static void scan_clear(bmg_ctx* c, const ScanShape& sh, const ScanBufs& B, hipStream_t s) {
   if (sh.atomic)
-    hipLaunchKernelGGL(k_fill_i32, dim3(2048), dim3(256), 0, s, c->acc, (uint64_t)c->p * 16, 0);
+    hipLaunchKernelGGL(k_fill_i32, dim3(2048), dim3(256), 0, s, B.acc, (uint64_t)c->p * 16, 0);
 }
 
 template <int NT, int CODING, bool ATOMIC>
-static void scan_mfma_w(bmg_ctx* c, int NC, const ScanShape& sh, hipStream_t s) {
+static void scan_mfma_w(bmg_ctx* c, int NC, const ScanShape& sh, const ScanBufs& B, hipStream_t s) {
   dim3 grid(sh.gx, sh.n_tiles);
   switch (sh.waves) {
     case 16:
       hipLaunchKernelGGL((k_scan_mfma<NT, CODING, 16, ATOMIC>), grid, dim3(1024), sh.lds, s, c->g, c->p, c->W32,
-                         c->dig, NC, c->n_pad, sh.tile_words, c->acc);
+                         B.dig, NC, c->n_pad, sh.tile_words, B.acc);
       break;
     case 8:
       hipLaunchKernelGGL((k_scan_mfma<NT, CODING, 8, ATOMIC>), grid, dim3(512), sh.lds, s, c->g, c->p, c->W32,
-                         c->dig, NC, c->n_pad, sh.tile_words, c->acc);
+                         B.dig, NC, c->n_pad, sh.tile_words, B.acc);
       break;
     default:
       hipLaunchKernelGGL((k_scan_mfma<NT, CODING, 4, ATOMIC>), grid, dim3(256), sh.lds, s, c->g, c->p, c->W32,
-                         c->dig, NC, c->n_pad, sh.tile_words, c->acc);
+                         B.dig, NC, c->n_pad, sh.tile_words, B.acc);
   }
 }
 
 template <int CODING>
-static void scan_mfma_c(bmg_ctx* c, int NC, const ScanShape& sh, hipStream_t s) {
+static void scan_mfma_c(bmg_ctx* c, int NC, const ScanShape& sh, const ScanBufs& B, hipStream_t s) {
   if (sh.atomic) {
-    if (sh.nt == 4) scan_mfma_w<4, CODING, true>(c, NC, sh, s);
-    else scan_mfma_w<2, CODING, true>(c, NC, sh, s);
+    if (sh.nt == 4) scan_mfma_w<4, CODING, true>(c, NC, sh, B, s);
+    else scan_mfma_w<2, CODING, true>(c, NC, sh, B, s);
   } else {
-    if (sh.nt == 4) scan_mfma_w<4, CODING, false>(c, NC, sh, s);
-    else scan_mfma_w<2, CODING, false>(c, NC, sh, s);
+    if (sh.nt == 4) scan_mfma_w<4, CODING, false>(c, NC, sh, B, s);
+    else scan_mfma_w<2, CODING, false>(c, NC, sh, B, s);
   }
 }
 
-static void scan_mfma(bmg_ctx* c, int NC, int coding, const ScanShape& sh, hipStream_t s) {
+static void scan_mfma(bmg_ctx* c, int NC, int coding, const ScanShape& sh, const ScanBufs& B, hipStream_t s) {
   switch (coding) {
-    case 0: scan_mfma_c<0>(c, NC, sh, s); break;
-    case 1: scan_mfma_c<1>(c, NC, sh, s); break;
-    case 2: scan_mfma_c<2>(c, NC, sh, s); break;
-    default: scan_mfma_c<3>(c, NC, sh, s); break;
+    case 0: scan_mfma_c<0>(c, NC, sh, B, s); break;
+    case 1: scan_mfma_c<1>(c, NC, sh, B, s); break;
+    case 2: scan_mfma_c<2>(c, NC, sh, B, s); break;
+    default: scan_mfma_c<3>(c, NC, sh, B, s); break;
   }
 }
 
-static int scan_kernel(bmg_ctx* c, int nvec, int coding, int digits, const ScanShape& sh, hipStream_t s) {
-  scan_clear(c, sh, s);
-  scan_mfma(c, nvec * digits, coding, sh, s);
+static int scan_kernel(bmg_ctx* c, int nvec, int coding, int digits, const ScanShape& sh, const ScanBufs& B,
+                       hipStream_t s) {
+  scan_clear(c, sh, B, s);
+  scan_mfma(c, nvec * digits, coding, sh, B, s);
   BMG_HIP(hipGetLastError());
   return BMG_OK;
 }
@@ -644,7 +695,8 @@ static int scan_kernel(bmg_ctx* c, int nvec, int coding, int digits, const ScanS
 // prepared digits -- HIP events on the scan's stream, best of two launches,
 // the accumulator clear included -- and keeps the fastest for the context's
 // lifetime.  A candidate the runtime refuses to launch is skipped.
-static int scan_tuned_shape(bmg_ctx* c, int nvec, int coding, int digits, hipStream_t s, ScanShape* out) {
+static int scan_tuned_shape(bmg_ctx* c, int nvec, int coding, int digits, const ScanBufs& B, hipStream_t s,
+                            ScanShape* out) {
   const int NC = nvec * digits;
   if (c->scan_tw[NC]) {
     *out = make_shape(c, NC, c->scan_nt[NC], c->scan_wv[NC], c->scan_tw[NC], c->scan_gx[NC]);
@@ -679,8 +731,8 @@ static int scan_tuned_shape(bmg_ctx* c, int nvec, int coding, int digits, hipStr
     bool ok = true;
     for (int r = 0; r < 2 && ok; ++r) {
       (void)hipEventRecord(e0, s);
-      scan_clear(c, cand[i], s);
-      scan_mfma(c, NC, coding, cand[i], s);
+      scan_clear(c, cand[i], B, s);
+      scan_mfma(c, NC, coding, cand[i], B, s);
       if (hipGetLastError() != hipSuccess) { ok = false; break; }
       (void)hipEventRecord(e1, s);
       if (hipEventSynchronize(e1) != hipSuccess) { ok = false; break; }
@@ -706,25 +758,27 @@ static int scan_tuned_shape(bmg_ctx* c, int nvec, int coding, int digits, hipStr
 
 int launch_scan(bmg_ctx* c, const double* r_dev, int nvec, int coding, int digits,
                 double* out_dev, hipStream_t s) {
-  int rc = scan_prepare(c, r_dev, nvec, digits, s);
+  ScanBufs B;
+  int rc = scan_prepare(c, r_dev, nvec, digits, s, &B);
   if (rc) return rc;
   ScanShape sh;
-  rc = scan_tuned_shape(c, nvec, coding, digits, s, &sh);
+  rc = scan_tuned_shape(c, nvec, coding, digits, B, s, &sh);
   if (rc) return rc;
-  rc = scan_kernel(c, nvec, coding, digits, sh, s);
+  rc = scan_kernel(c, nvec, coding, digits, sh, B, s);
   if (rc) return rc;
   hipLaunchKernelGGL(k_scan_combine, dim3((unsigned)((c->p + 255) / 256), nvec), dim3(256), 0, s,
-                     c->acc, c->p, nvec, digits, c->scale, out_dev);
+                     B.acc, c->p, nvec, digits, B.scale, out_dev);
   BMG_HIP(hipGetLastError());
   return BMG_OK;
 }
 
 int launch_scan_timed(bmg_ctx* c, const double* r_dev, int nvec, int coding, int digits,
                       double* out_dev, int reps, double* ms, hipStream_t s) {
-  int rc = scan_prepare(c, r_dev, nvec, digits, s);
+  ScanBufs B;
+  int rc = scan_prepare(c, r_dev, nvec, digits, s, &B);
   if (rc) return rc;
   ScanShape sh;
-  rc = scan_tuned_shape(c, nvec, coding, digits, s, &sh);
+  rc = scan_tuned_shape(c, nvec, coding, digits, B, s, &sh);
   if (rc) return rc;
   hipEvent_t e0, e1;
   BMG_HIP(hipEventCreate(&e0));
@@ -733,9 +787,9 @@ int launch_scan_timed(bmg_ctx* c, const double* r_dev, int nvec, int coding, int
   // the clear stays outside the timed region: the events bracket the MFMA
   // kernel alone, as a kernel trace reports it
   for (int r = 0; r < reps && !rc; ++r) {
-    scan_clear(c, sh, s);
+    scan_clear(c, sh, B, s);
     BMG_HIP(hipEventRecord(e0, s));
-    scan_mfma(c, nvec * digits, coding, sh, s);
+    scan_mfma(c, nvec * digits, coding, sh, B, s);
     BMG_HIP(hipGetLastError());
     BMG_HIP(hipEventRecord(e1, s));
     BMG_HIP(hipEventSynchronize(e1));
@@ -746,7 +800,7 @@ int launch_scan_timed(bmg_ctx* c, const double* r_dev, int nvec, int coding, int
   hipEventDestroy(e0); hipEventDestroy(e1);
   *ms = total / (reps > 0 ? reps : 1);
   hipLaunchKernelGGL(k_scan_combine, dim3((unsigned)((c->p + 255) / 256), nvec), dim3(256), 0, s,
-                     c->acc, c->p, nvec, digits, c->scale, out_dev);
+                     B.acc, c->p, nvec, digits, B.scale, out_dev);
   BMG_HIP(hipGetLastError());
   return BMG_OK;
 }

@@ -49,6 +49,8 @@ int ctx_prepare(bmg_ctx* c, int recode);
 int ctx_self_moments(bmg_ctx* c);
 int launch_decode(bmg_ctx* c, const uint64_t* snps_dev, uint64_t count, int coding,
                   double* out_dev, hipStream_t s);
+// genotypes of a subset of a pool dataset's individuals into c (same p)
+int launch_subset(const bmg_ctx* pool, const uint32_t* sub_dev, bmg_ctx* c, hipStream_t s);
 int launch_synth(bmg_ctx* c, uint32_t seed, const double* thresh_dev, const double* rho_dev,
                  const uint64_t* blk_start_dev, uint64_t n_blocks, hipStream_t s);
 int launch_scan(bmg_ctx* c, const double* r_dev, int nvec, int coding, int digits,

@@ -85,6 +85,13 @@ int bmg_ctx_create(int device, const uint8_t* bed_body, uint32_t n, uint64_t p,
 int bmg_ctx_create_synthetic(int device, uint32_t seed, uint32_t n, uint64_t p,
                              const double* thresh, const double* rho,
                              const int32_t* types, int n_types, bmg_ctx** out);
+/* A phenotype measured on a subset of a pool's individuals (no counterpart
+ * in the reference, where each such phenotype is its own .bed/.fam pair,
+ * src/data.hpp:45-72): a new dataset holding the genotypes of
+ * individuals[0..n_sub) of the pool (gathered on the device, in that
+ * order) with its own y (n_sub) and types; the pool stays usable. */
+int bmg_ctx_create_subset(const bmg_ctx* pool, const uint32_t* individuals, uint32_t n_sub,
+                          const double* y, const int32_t* types, int n_types, bmg_ctx** out);
 int bmg_ctx_set_y(bmg_ctx* ctx, const double* y);
 int bmg_ctx_destroy(bmg_ctx* ctx);
 /* packed body back to the host, reference layout (p * ceil(n/4) bytes) */
@@ -154,6 +161,8 @@ int bmg_band_xtx_timed(bmg_ctx* ctx, uint32_t W, int coding_a, int coding_b, int
 int bmg_dev_alloc(int device, uint64_t bytes, void** ptr);
 int bmg_dev_free(int device, void* ptr);
 int bmg_memcpy_h2d(int device, void* dst, const void* src, uint64_t bytes);
+/* device-to-host copy after every kernel queued on the device has finished */
+int bmg_memcpy_d2h(int device, void* dst, const void* src, uint64_t bytes);
 int bmg_copy_timed(int device, const void* src_dev, void* dst_dev, uint64_t bytes, int reps,
                    double* ms_per_launch);
 /* Streaming read of `bytes` (no counterpart in the reference): the best
@@ -174,7 +183,32 @@ typedef struct {
  * without covariates; kmax = column capacity (<= bmg_model_kmax()). */
 int bmg_model_kmax(void);
 int bmg_model_create(bmg_ctx* ctx, const bmg_model_prior* prior, bmg_model** out);
+/* The Model of one Sampler with its Prior and random stream
+ * (src/sampler.hpp:171-203, src/prior.hpp:45-122, src/rand.hpp:36-58): the
+ * prior is solved from the options as for a batch chain, and the stream is
+ * the chain's (key = seed, chain).  The empty model's log-likelihood is
+ * computed. */
+int bmg_model_create_sampler(bmg_ctx* ctx, const bmg_opts* opts, bmg_model** out);
 int bmg_model_destroy(bmg_model* m);
+/* Model::sample_beta_sigma2(rng) (src/model.hpp:314-342) from the model's
+ * stream; BMG_E_NUMERIC if the sigma2 draw cannot finish */
+int bmg_model_sample_beta_sigma2(bmg_model* m);
+/* Model::compute_pve(pves) (src/model.hpp:345-392, no covariates): leaves
+ * y_hat = X beta on the device for bmg_model_rb */
+int bmg_model_compute_pve(bmg_model* m, double pves[3]);
+/* beta (*len values: the constant column first), sigma2 and y_hat (n) of
+ * the last draws; any pointer may be NULL */
+int bmg_model_beta(bmg_model* m, double* beta, double* sigma2, double* y_hat, int* len);
+/* RaoBlackwellizer::p_raoblackwell(Sampler*, y_hat, p_r, p_r_types)
+ * (src/sampler.hpp:73-74, src/sampler.cpp:32-261) for a caller-held model:
+ * p_r[p] from the model's state and prior and the given y_hat[n]; one
+ * effect type */
+int bmg_model_rb(bmg_model* m, const double* y_hat, double* p_r);
+/* *current_model = *new_model after an accepted move, and *new_model =
+ * *current_model after a rejected one (Model::operator=, src/model.hpp:
+ * 115-162; src/sampler.cpp:592,634 and src/sampler.hpp:604-608,737-741) */
+int bmg_model_commit(bmg_model* current, const bmg_model* proposed);
+int bmg_model_rollback(bmg_model* proposed, const bmg_model* current);
 /* Model::add_term(ind, t_ind, inv_tau2_alpha2) (src/model.hpp:239-268) */
 int bmg_model_add_term(bmg_model* m, uint32_t ind, int t_ind, const double* d);
 /* Model::remove_term(model_ind) (src/model.hpp:270-312) */
@@ -217,9 +251,16 @@ typedef struct {
   int32_t learn_window;        /* learned: partners within this many SNPs */
   int32_t learn_steps;         /* learned: Adam steps per training round */
   int32_t learn_batch;         /* learned: rows per Adam step */
-  int32_t capacity;            /* model columns compiled for: 64, 128, 160 (factor
-                                  in LDS) or 256 (factor in HBM); 0: from the
-                                  prior, e_qg + 6 sd (AH: twice) */
+  int32_t capacity;            /* model columns of the launch: 64, 128, 160
+                                  (factor in LDS) or 256 (factor in HBM); 0:
+                                  the smallest that holds the prior mean plus
+                                  one move (AH: twice).  A chain whose model
+                                  comes within one move of it continues in the
+                                  256-column kernel from that iteration (same
+                                  results); a proposal past 256 columns stops
+                                  the chain and bmg_batch_run fails with
+                                  BMG_E_CAPACITY (the reference's MAXMODELSIZE
+                                  error, src/model.hpp:247-262) */
   double  p_learned;           /* learned: share of the drawn swaps that are learned */
   double  learn_rate;          /* learned: Adam learning rate */
   int32_t max_model_size;      /* SNPs; 0: none.  Larger models get prior

@@ -11,7 +11,9 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+# ORACLE_LIB selects another build of the same sources (the sanitizer build
+# of `make -C oracle asan`)
+LIB_PATH = os.environ.get("ORACLE_LIB") or os.path.join(HERE, "_build", "liboracle.so")
 
 EF = {"A": 0, "H": 1, "D": 2, "R": 3, "AH": 4}
 
@@ -81,6 +83,37 @@ _U8 = C.POINTER(C.c_uint8)
 _U32 = C.POINTER(C.c_uint32)
 _I32 = C.POINTER(C.c_int32)
 _I64 = C.POINTER(C.c_int64)
+
+
+class Rng(C.Structure):
+    """orc_rng: Philox4x32-10 key and counter of one stream."""
+    _fields_ = [("key", C.c_uint32 * 2), ("ctr", C.c_uint32 * 4)]
+
+
+def mvnormal_invchol(seed, mu, U, scale, count):
+    """count draws of x = scale U^-1 z + mu (src/rand.hpp:144-168) from one
+    stream; U upper triangular (d x d)."""
+    r = Rng()
+    lib().orc_rng_init(C.byref(r), seed, 0, 0, 0)
+    d = len(mu)
+    Uf = np.asfortranarray(U, dtype=np.float64)
+    m = np.ascontiguousarray(mu, dtype=np.float64)
+    out = np.zeros((count, d))
+    x = np.zeros(d)
+    for i in range(count):
+        lib().orc_rng_mvnormal_invchol(C.byref(r), ptr(m), Uf.ctypes.data_as(C.POINTER(C.c_double)), d, d,
+                                       scale, ptr(x))
+        out[i] = x
+    return out
+
+
+def adapt_jd_mb(p_ms, r_ms, r_n, q_p0):
+    """One move-size adaptation step (src/sampler.hpp:1728-1769); q_p0 is
+    updated in place."""
+    a = np.ascontiguousarray(r_ms, dtype=np.float64)
+    b = np.ascontiguousarray(r_n, dtype=np.float64)
+    assert q_p0.dtype == np.float64 and q_p0.flags.c_contiguous
+    return lib().orc_adapt_jd_mb(p_ms, ptr(a), ptr(b), ptr(q_p0), len(a))
 
 
 def _sig(lib, name, res, *args):
@@ -182,6 +215,11 @@ def load():
     _sig(lib, "orc_geometric_dist_cdf", None, C.c_int, C.c_double, _D)
     _sig(lib, "orc_sample_discrete_naive", C.c_int, _D, C.c_int, C.c_double)
     _sig(lib, "orc_ess_geyer", C.c_double, _D, C.c_int64)
+    _sig(lib, "orc_rng_init", None, C.POINTER(Rng), C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32)
+    _sig(lib, "orc_rng_mvnormal_invchol", None, C.POINTER(Rng), _D, _D, C.c_int, C.c_int, C.c_double, _D)
+    _sig(lib, "orc_model_draw_beta_sigma2", C.c_int64, _P, _P, C.c_int64, _D, _D)
+    _sig(lib, "orc_sampler_refresh_draws", C.c_int64, _P, C.c_int64, _D, _D)
+    _sig(lib, "orc_adapt_jd_mb", C.c_double, C.c_double, _D, _D, _D, C.c_int)
     return lib
 
 
@@ -434,6 +472,26 @@ class Sampler:
     def p_proposal(self):
         return np.ctypeslib.as_array(lib().orc_sampler_p_proposal(self.h),
                                      shape=(self.data.p,)).copy()
+
+    def refresh_draws(self, count):
+        """count independent tau2/alpha refreshes of the current model from
+        its present state (src/prior.cpp:30-141): alpha (count,) and the
+        terms' inv_tau2_alpha2 (count, k)."""
+        k = int(self.current().cols()) - 1
+        al = np.zeros(count)
+        dv = np.zeros(count * max(k, 1))
+        k2 = lib().orc_sampler_refresh_draws(self.h, count, ptr(al), ptr(dv))
+        return al, dv[:count * k2].reshape(count, k2)
+
+    def beta_sigma2_draws(self, count):
+        """count draws of (sigma2, beta) on the current model (src/model.hpp:
+        314-342) from the chain's stream."""
+        k = int(self.current().cols())
+        s2 = np.zeros(count)
+        b = np.zeros(count * k)
+        m = lib().orc_sampler_current_model(self.h)
+        k2 = lib().orc_model_draw_beta_sigma2(self.h, m, count, ptr(s2), ptr(b))
+        return s2, b[:count * k2].reshape(count, k2)
 
     def rb(self, y_hat):
         out = np.zeros(self.data.p)

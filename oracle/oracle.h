@@ -44,6 +44,10 @@ double orc_rng_u01(orc_rng* r);
 double orc_rng_normal(orc_rng* r);
 double orc_rng_gamma(orc_rng* r, double a);
 double orc_rng_sinvchi2(orc_rng* r, double nu, double s2);
+/* x = scale * U^-1 z + mu, z standard normal, U upper triangular
+ * (column-major, leading dimension ld), src/rand.hpp:144-168 */
+void   orc_rng_mvnormal_invchol(orc_rng* r, const double* mu, const double* U, int ld, int d,
+                                double scale, double* x);
 /* fills out[0..cnt) with draws of one kind: 0 u01, 1 normal, 2 gamma(a),
  * 3 sinvchi2(a, b) */
 void   orc_rng_fill(uint32_t seed, uint32_t chain, uint32_t c2, uint32_t c3,
@@ -184,6 +188,15 @@ void   orc_model_get(const orc_model* m, double* xx, double* l, double* xy,
                      double* v, double* d, double scal[6]);
 void   orc_model_set_d(orc_model* m, int x_ind, double val);
 void   orc_model_sample_beta_sigma2(orc_sampler* s, orc_model* m);
+/* N draws of (sigma2, beta) from the sampler's stream on a fixed model:
+ * s2_out[N], beta_out[N x k]; returns k */
+int64_t orc_model_draw_beta_sigma2(orc_sampler* s, orc_model* m, int64_t N, double* s2_out,
+                                   double* beta_out);
+/* N independent tau2/alpha refreshes (src/prior.cpp:30-141) of the current
+ * model, each from the same starting state: alpha_out[N], and the terms'
+ * inv_tau2_alpha2 after each, dv_out[N x k]; returns k.  The state is
+ * restored afterwards. */
+int64_t orc_sampler_refresh_draws(orc_sampler* s, int64_t N, double* alpha_out, double* dv_out);
 void   orc_model_compute_pve(orc_model* m, double pves[3]);
 const double* orc_model_beta(const orc_model* m);
 const double* orc_model_y_hat(const orc_model* m);
@@ -229,6 +242,9 @@ void   orc_synth_genotypes(uint32_t seed, uint32_t n, uint64_t p,
 
 /* ------------------------------------------------------- utilities */
 void   orc_geometric_dist_cdf(int maxsize, double p, double* values);
+/* move-size adaptation by expected jump distance, src/sampler.hpp:1728-1769
+ * (updates q_p0[max_ms], returns the new p_move_size) */
+double orc_adapt_jd_mb(double p_ms, const double* r_ms, const double* r_n, double* q_p0, int max_ms);
 int    orc_sample_discrete_naive(const double* cumsum, int m, double u01);
 double orc_ess_geyer(const double* x, int64_t n);
 

@@ -412,10 +412,18 @@ void orc__model_sample_beta_sigma2(orc_model* m, orc_rng* rng)
   m->sigma2 = val;
   compute_mu_beta(m);
   m->beta_len = m->mu_len;
-  for (int i = 0; i < m->beta_len; ++i) m->beta[i] = orc_rng_normal(rng);
-  orc_trsv_un(m->l, m->kmax, m->beta_len, m->beta);
-  double sc = sqrt(m->sigma2);
-  for (int i = 0; i < m->beta_len; ++i) m->beta[i] = sc * m->beta[i] + m->mu_beta[i];
+  orc_rng_mvnormal_invchol(rng, m->mu_beta, m->l, m->kmax, m->beta_len, sqrt(m->sigma2), m->beta);
+}
+
+/* src/rand.hpp:144-168 (Rand::rand_mvnormal_invchol): x = scale * U^-1 z + mu
+ * with z standard normal, U upper triangular (column-major, ld), i.e. x ~
+ * N(mu, scale^2 (U'U)^-1) */
+void orc_rng_mvnormal_invchol(orc_rng* rng, const double* mu, const double* U, int ld, int d,
+                              double scale, double* x)
+{
+  for (int i = 0; i < d; ++i) x[i] = orc_rng_normal(rng);
+  orc_trsv_un(U, ld, d, x);
+  for (int i = 0; i < d; ++i) x[i] = scale * x[i] + mu[i];
 }
 
 static double vvar(const double* x, size_t n)
@@ -426,7 +434,7 @@ static double vvar(const double* x, size_t n)
 }
 
 /* src/model.hpp:345-392 */
-void orc_model_compute_pve(orc_model* m, double* pves)
+void orc_model_compute_pve(orc_model* m, double pves[3])
 {
   size_t n = m->n; int me = m->m_e;
   int ce = (me > 1);

@@ -1296,20 +1296,21 @@ static void adapt_acptrate(orc_sampler* s, double* p_ms, size_t* na, size_t* nm,
   *na = 1; *nm = 2;
 }
 
-/* src/sampler.hpp:1728-1769 */
-static void adapt_jd_mb(orc_sampler* s, double* p_ms, const double* r_ms,
-                        const double* r_n, int max_ms)
+/* src/sampler.hpp:1728-1769: adds the current truncated geometric move-size
+ * distribution to q_p0 and returns the grid point (50 points from 0.01 to
+ * 0.99) of largest importance-weighted expected jump distance */
+double orc_adapt_jd_mb(double p_ms, const double* r_ms, const double* r_n, double* q_p0, int max_ms)
 {
-  double q = 1 - *p_ms;
+  double q = 1 - p_ms;
   double zp = 1 - pow(q, max_ms);
-  double q_p = *p_ms / zp;
-  for (int i = 0; i < max_ms; ++i) { s->q_p0_ms[i] += q_p; q_p *= q; }
+  double q_p = p_ms / zp;
+  for (int i = 0; i < max_ms; ++i) { q_p0[i] += q_p; q_p *= q; }
   double p_best = 0.01, p = 0.01, h_best = -INFINITY, h;
   double step = 0.98 / 49.0, ratio;
   for (int j = 0; j < 50; ++j) {
     h = 0.0; zp = 0.0; q = 1.0 - p; q_p = p;
     for (int i = 0; i < max_ms; ++i) {
-      ratio = q_p / s->q_p0_ms[i];
+      ratio = q_p / q_p0[i];
       h += (i + 1) * r_ms[i] * ratio;
       zp += r_n[i] * ratio;
       q_p *= q;
@@ -1318,7 +1319,13 @@ static void adapt_jd_mb(orc_sampler* s, double* p_ms, const double* r_ms,
     if (h > h_best) { h_best = h; p_best = p; }
     p += step;
   }
-  *p_ms = p_best;
+  return p_best;
+}
+
+static void adapt_jd_mb(orc_sampler* s, double* p_ms, const double* r_ms,
+                        const double* r_n, int max_ms)
+{
+  *p_ms = orc_adapt_jd_mb(*p_ms, r_ms, r_n, s->q_p0_ms, max_ms);
 }
 
 static void trace_push(orc_sampler* s, uint8_t mv, uint8_t jd)
@@ -1353,7 +1360,7 @@ static void thinned_push(orc_sampler* s)
     s->th_loci_cap = s->th_loci_cap ? 2 * s->th_loci_cap : 4096;
     s->th_loci = (uint32_t*)realloc(s->th_loci, s->th_loci_cap * sizeof(uint32_t));
   }
-  memcpy(s->th_loci + s->th_loci_len, s->cur->loci, sizeof(uint32_t) * k);
+  if (k > 0) memcpy(s->th_loci + s->th_loci_len, s->cur->loci, sizeof(uint32_t) * k);
   s->th_loci_len += k;
   s->th_size[s->th_len] = (uint32_t)k;
   s->th_ll[s->th_len] = s->cur->loglik;
@@ -1542,6 +1549,45 @@ int orc_sampler_loci(const orc_sampler* s, uint32_t* loci)
 
 void orc_model_sample_beta_sigma2(orc_sampler* s, orc_model* m)
 { orc__model_sample_beta_sigma2(m, &s->rng); }
+
+int64_t orc_model_draw_beta_sigma2(orc_sampler* s, orc_model* m, int64_t N, double* s2_out,
+                                   double* beta_out)
+{
+  int64_t k = 0;
+  for (int64_t i = 0; i < N; ++i) {
+    orc__model_sample_beta_sigma2(m, &s->rng);
+    k = m->beta_len;
+    s2_out[i] = m->sigma2;
+    for (int64_t j = 0; j < k; ++j) beta_out[i * k + j] = m->beta[j];
+  }
+  return k;
+}
+
+int64_t orc_sampler_refresh_draws(orc_sampler* s, int64_t N, double* alpha_out, double* dv_out)
+{
+  orc_model* m = s->cur;
+  orc_prior* p = &s->pr;
+  const double a0 = p->alpha, a20 = p->alpha2;
+  double ita0[4];
+  for (int t = 0; t < 4; ++t) ita0[t] = p->inv_tau2_alpha2[t];
+  const int k = m->beta_len - m->m_e;
+  double* dv0 = (double*)malloc(sizeof(double) * (k > 0 ? k : 1));
+  for (int j = 0; j < k; ++j) dv0[j] = m->dv[m->m_e + j];
+  for (int64_t i = 0; i < N; ++i) {
+    p->alpha = a0; p->alpha2 = a20;
+    for (int t = 0; t < 4; ++t) p->inv_tau2_alpha2[t] = ita0[t];
+    for (int j = 0; j < k; ++j) m->dv[m->m_e + j] = dv0[j];
+    prior_sample_alpha_and_tau2(s, m);
+    alpha_out[i] = p->alpha;
+    for (int j = 0; j < k; ++j) dv_out[i * k + j] = m->dv[m->m_e + j];
+  }
+  p->alpha = a0; p->alpha2 = a20;
+  for (int t = 0; t < 4; ++t) p->inv_tau2_alpha2[t] = ita0[t];
+  for (int j = 0; j < k; ++j) m->dv[m->m_e + j] = dv0[j];
+  m->mu_beta_computed = 0;
+  free(dv0);
+  return k;
+}
 
 void orc_exhaustive_modelset(orc_model* full, int const_loci, int n_inds,
                              double* log_model_probs, double* max_log)

@@ -124,3 +124,140 @@ def test_swapadj_matches_fresh_factor(bmg):
     A = before["xx"] + np.triu(before["xx"], 1).T + np.diag(before["d"])
     ref = np.linalg.cholesky(A[np.ix_(perm, perm)]).T
     assert np.allclose(np.triu(after["l"]), ref, atol=1e-10)
+
+
+def _sampler_dataset(n, p, seed):
+    from bmagwa_amd import synth
+    thresh, rho, _ = synth.snp_layout(p, (0.05, 0.5), (0.8, 0.9, 10, 30), seed)
+    body = oracle.synth_genotypes(seed, n, p, thresh, rho)
+    dec = lambda snps: np.stack([oracle.decode(body, n, int(j), 0) for j in snps])
+    y, causal, _ = synth.phenotype(dec, n, p, 6, 0.6, seed)
+    cfg = synth.sampler_cfg(n, p, 6, seed=29, chain=3)
+    return body, y, cfg
+
+
+OPS = [(0, 17, 0), (0, 120, 0), (0, 121, 0), (0, 333, 0), (0, 480, 0), (1, 2, 0), (0, 250, 0),
+       (2, 0, 0), (3, 0, 0), (2, 0, 0), (3, 0, 0)]
+
+
+def _oracle_ops(cfg, body, y, n, p, d):
+    s = oracle.Sampler(cfg, oracle.Data(n, p, body, y))
+    m = s.current()
+    m.compute_log_likelihood()
+    h = oracle.lib().orc_sampler_current_model(s.h)
+    out = []
+    for op, arg, _ in OPS:
+        if op == 0:
+            m.add_term(arg, 0, [d, d])
+            out.append(("add", m.size(), m.log_likelihood()))
+        elif op == 1:
+            m.remove_term(arg)
+            out.append(("remove", m.size(), m.log_likelihood()))
+        elif op == 2:
+            oracle.lib().orc_model_sample_beta_sigma2(s.h, h)
+            k = m.cols()
+            b = np.ctypeslib.as_array(oracle.lib().orc_model_beta(h), shape=(k,)).copy()
+            out.append(("beta", m.get()["sigma2"], b))
+        else:
+            pv = np.zeros(3)
+            oracle.lib().orc_model_compute_pve(h, oracle.ptr(pv))
+            yh = np.ctypeslib.as_array(oracle.lib().orc_model_y_hat(h), shape=(n,)).copy()
+            out.append(("pve", pv, s.rb(yh)))
+    return out
+
+
+def _compare_ops(got, want):
+    for g, w in zip(got, want):
+        assert g[0] == w[0]
+        if g[0] in ("add", "remove"):
+            assert g[1] == w[1]
+            assert g[2] == pytest.approx(w[2], rel=1e-10)
+        elif g[0] == "beta":
+            assert g[1] == pytest.approx(w[1], rel=1e-10)
+            assert np.allclose(g[2], w[2], rtol=1e-9, atol=1e-12)
+        else:
+            assert np.allclose(g[1], w[1], rtol=1e-10, atol=1e-14)
+            assert np.allclose(g[2], w[2], rtol=1e-8, atol=1e-12)
+
+
+def test_sampler_model_draws_pve_and_rb_match_oracle(bmg):
+    """A model created from a Sampler's options (bmg_model_create_sampler)
+    carries its prior and random stream: add / remove, sample_beta_sigma2,
+    compute_pve and the RB scan of a caller-held model (bmg_model_rb)
+    reproduce the oracle's Model and RaoBlackwellizer."""
+    n, p = 400, 600
+    body, y, cfg = _sampler_dataset(n, p, 41)
+    s0 = oracle.Sampler(cfg, oracle.Data(n, p, body, y))
+    d = s0.prior()["inv_tau2_alpha2_A"]
+    want = _oracle_ops(cfg, body, y, n, p, d)
+    ctx = bmg.Ctx(n, p, body, y)
+    m = bmg.Model(ctx, cfg=cfg)
+    got = []
+    for op, arg, _ in OPS:
+        if op == 0:
+            m.add_term(arg, 0, [d, d])
+            got.append(("add", m.size(), m.log_likelihood()))
+        elif op == 1:
+            m.remove_term(arg)
+            got.append(("remove", m.size(), m.log_likelihood()))
+        elif op == 2:
+            m.sample_beta_sigma2()
+            b, s2, _ = m.beta()
+            got.append(("beta", s2, b))
+        else:
+            pv = m.compute_pve()
+            _, _, yh = m.beta()
+            got.append(("pve", pv, m.rb(yh)))
+    _compare_ops(got, want)
+    # commit / rollback: a proposal copied back and forth
+    prop = bmg.Model(ctx, cfg=cfg)
+    prop.rollback_from(m)
+    prop.add_term(5, 0, [d, d])
+    assert prop.size() == m.size() + 1
+    prop.rollback_from(m)
+    assert prop.log_likelihood() == m.log_likelihood()
+    prop.add_term(5, 0, [d, d])
+    m.commit_from(prop)
+    assert m.size() == prop.size() and m.log_likelihood() == prop.log_likelihood()
+
+
+def test_cpp_caller_matches_oracle(bmg, tmp_path):
+    """tests/cpp/model_caller: a C++ HipModel (the binding a maintainer adds
+    to the reference) driven through include/bmagwa_hip.h reproduces the
+    oracle's log-likelihoods, draws, PVE and RB probabilities."""
+    import ctypes
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "_build", "model_caller")
+    assert os.path.exists(exe), "build the caller first (__graft_entry__.build())"
+    n, p = 400, 600
+    body, y, cfg = _sampler_dataset(n, p, 43)
+    s0 = oracle.Sampler(cfg, oracle.Data(n, p, body, y))
+    d = s0.prior()["inv_tau2_alpha2_A"]
+    want = _oracle_ops(cfg, body, y, n, p, d)
+    o = bmg.opts_from_cfg(cfg)
+    blob = (np.uint32(n).tobytes() + np.uint64(p).tobytes() + body.tobytes() + y.astype(np.float64).tobytes()
+            + ctypes.string_at(ctypes.byref(o), ctypes.sizeof(o)) + np.uint32(len(OPS)).tobytes())
+    for op, arg, dd in OPS:
+        blob += np.int32(op).tobytes() + np.uint32(arg).tobytes() + np.float64(d).tobytes()
+    inp = tmp_path / "in.bin"
+    inp.write_bytes(blob)
+    print("model_caller n=%d p=%d ops=%d" % (n, p, len(OPS)), flush=True)
+    res = subprocess.run([exe, str(inp)], capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    lines = res.stdout.split("\n")
+    got, i = [], 0
+    while i < len(lines) and lines[i]:
+        f = lines[i].split()
+        if f[0] in ("add", "remove"):
+            got.append((f[0], int(f[2]), float(f[3])))
+        elif f[0] == "beta":
+            got.append(("beta", float(f[1]), np.array([float(v) for v in f[2:]])))
+        else:
+            pv = np.array([float(v) for v in f[1:4]])
+            i += 1
+            pr = np.array([float(v) for v in lines[i].split()[1:]])
+            got.append(("pve", pv, pr))
+        i += 1
+    assert len(got) == len(OPS)
+    _compare_ops(got, want)

@@ -133,6 +133,53 @@ def test_models_beyond_the_lds_capacity(bmg):
     assert np.allclose(b.p_rao(0), s.p_rao(), rtol=1e-8, atol=1e-10)
 
 
+@pytest.mark.parametrize("capacity", [64, 160])
+def test_chains_leave_a_small_capacity_with_the_same_trajectory(bmg, capacity):
+    """The same growing models (n = 2,000, p = 3,000, 220 causal, h2 = 0.9)
+    in a launch of 64 or 160 columns: a chain whose model comes within one
+    move of the capacity continues in the 256-column kernel from that
+    iteration, and the trajectory over 2,000 iterations is still the
+    oracle's (nothing is rejected for want of room)."""
+    n, p, nc, n_iter = 2000, 3000, 220, 2000
+    thresh, rho, _ = synth.snp_layout(p, (0.05, 0.5), None, 77)
+    body = oracle.synth_genotypes(77, n, p, thresh, rho)
+    dec = lambda snps: np.stack([oracle.decode(body, n, int(j), 0) for j in snps])
+    y, causal, _ = synth.phenotype(dec, n, p, nc, 0.9, 77)
+    cfg = synth.sampler_cfg(n, p, nc, seed=3, n_rao=100, n_rao_burnin=5, thin=10, do_n_iter=n_iter,
+                            delay_rejection=4)
+    s = oracle_run(cfg, body, y, n_iter)
+    ctx = bmg.Ctx(n, p, body, y)
+    b = bmg.Batch([ctx], [0], [cfg], dict(trace_every_iter=1, capacity=capacity))
+    b.setup()
+    b.run(n_iter)
+    tr = b.trace(0)
+    compare(tr, s.trace(), n_iter)
+    assert tr["size"].max() + 1 > capacity - 21
+    assert np.allclose(b.p_rao(0), s.p_rao(), rtol=1e-8, atol=1e-10)
+
+
+def test_models_past_the_device_limit_stop_with_an_error(bmg):
+    """AH coding (two columns a locus) with 220 causal SNPs at h2 = 0.9:
+    the models grow past the 256 columns the device holds, and the run stops
+    with BMG_E_CAPACITY naming the chain instead of rejecting the moves that
+    do not fit (the reference stops the same way at its MAXMODELSIZE)."""
+    n, p, nc = 2000, 3000, 220
+    thresh, rho, _ = synth.snp_layout(p, (0.05, 0.5), None, 78)
+    body = oracle.synth_genotypes(78, n, p, thresh, rho)
+    dec = lambda snps: np.stack([oracle.decode(body, n, int(j), 0) for j in snps])
+    y, causal, _ = synth.phenotype(dec, n, p, nc, 0.9, 78, dominance=True)
+    cfg = synth.sampler_cfg(n, p, nc, types="AH", seed=5, n_rao=100, n_rao_burnin=5, thin=10,
+                            do_n_iter=20000)
+    ctx = bmg.Ctx(n, p, body, y, types=("AH",))
+    b = bmg.Batch([ctx], [0], [cfg], dict(trace_every_iter=1))
+    b.setup()
+    with pytest.raises(bmg.BmgError, match="device limit"):
+        for _ in range(20):
+            b.run(1000)
+    st = b.state(0)
+    assert 2 * st["size"] + 1 + 2 * 20 + 1 > 256
+
+
 def test_batch_of_chains_each_match_their_oracle_run(bmg):
     n, p, n_iter = 256, 300, 1500
     body, y, _ = dataset(n, p, (0.9, 0.9, 10, 30), 4, 99)
