@@ -127,6 +127,7 @@ SIGNATURES = {
     "bmg_band_xtx_dev": (C.c_int, _P, C.c_uint32, C.c_int, C.c_int, _P, _P),
     "bmg_scan_xtr": (C.c_int, _P, _D, C.c_int, C.c_int, C.c_int, _D),
     "bmg_scan_xtr_dev": (C.c_int, _P, _P, C.c_int, C.c_int, C.c_int, _P, _P),
+    "bmg_scan_xtr_dev_noalloc": (C.c_int, _P, _P, C.c_int, C.c_int, C.c_int, _P, _P),
     "bmg_ctx_n_pad": (C.c_int, _P, _U32),
     "bmg_scan_xtr_timed": (C.c_int, _P, _P, C.c_int, C.c_int, C.c_int, _P, C.c_int, _D, _P),
     "bmg_scan_shape": (C.c_int, _P, C.c_int, C.c_int, _P),
@@ -334,11 +335,12 @@ class Ctx:
                                        C.c_void_p(stream) if stream else None))
         return ms.value
 
-    def scan_xtr_dev(self, r_dev, nvec, out_dev, coding=0, digits=5, stream=None):
+    def scan_xtr_dev(self, r_dev, nvec, out_dev, coding=0, digits=5, stream=None, noalloc=False):
         """X^T R on device buffers r_dev (nvec x n_pad doubles) -> out_dev
-        (nvec x p doubles)."""
-        check(lib().bmg_scan_xtr_dev(self.h, C.c_void_p(r_dev), nvec, coding, digits, C.c_void_p(out_dev),
-                                     C.c_void_p(stream) if stream else None))
+        (nvec x p doubles); noalloc: the allocation-free entry point."""
+        f = lib().bmg_scan_xtr_dev_noalloc if noalloc else lib().bmg_scan_xtr_dev
+        check(f(self.h, C.c_void_p(r_dev), nvec, coding, digits, C.c_void_p(out_dev),
+                C.c_void_p(stream) if stream else None))
 
     def n_pad(self):
         v = C.c_uint32()

@@ -285,6 +285,13 @@ extern "C" int bmg_scan_xtr_dev(bmg_ctx* c, const double* r_dev, int nvec, int c
   return launch_scan(c, r_dev, nvec, coding, digits, out_dev, s);
 }
 
+extern "C" int bmg_scan_xtr_dev_noalloc(bmg_ctx* c, const double* r_dev, int nvec, int coding, int digits,
+                                        double* out_dev, void* stream) {
+  if (!c || !r_dev || !out_dev || coding < 0 || coding > 3) { set_error("bad argument"); return BMG_E_ARG; }
+  hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  return launch_scan_noalloc(c, r_dev, nvec, coding, digits, out_dev, s);
+}
+
 extern "C" int bmg_scan_xtr_timed(bmg_ctx* c, const double* r_dev, int nvec, int coding, int digits,
                                   double* out_dev, int reps, double* ms_per_launch, void* stream) {
   if (!c || !r_dev || !out_dev || !ms_per_launch || coding < 0 || coding > 3 || reps < 1) {

@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <mutex>
 #include <vector>
 
 #include "bmg_device.h"
@@ -799,6 +800,69 @@ int launch_scan_timed(bmg_ctx* c, const double* r_dev, int nvec, int coding, int
   }
   hipEventDestroy(e0); hipEventDestroy(e1);
   *ms = total / (reps > 0 ? reps : 1);
+  hipLaunchKernelGGL(k_scan_combine, dim3((unsigned)((c->p + 255) / 256), nvec), dim3(256), 0, s,
+                     B.acc, c->p, nvec, digits, B.scale, out_dev);
+  BMG_HIP(hipGetLastError());
+  return BMG_OK;
+}
+
+// ------------------------------------------------ allocation-free scan --
+// The scan's scratch for callers that cannot allocate (graph capture, tight
+// serving loops): one file-scope device array per GPU, held until the
+// process ends.  Host code never passes `scan_scratch` itself to a kernel
+// (on the host the name is a stand-in, not a device address): the device
+// address is looked up once per device with hipGetSymbolAddress and kept.
+// Layout: [accumulator (p + 1024) x 16 int32 | digits 16 x n_pad int8 |
+// scales 16 doubles], at most SCAN_SCRATCH_BYTES.
+constexpr size_t SCAN_SCRATCH_ACC_ROWS = size_t(1) << 20;    // p + 1024 <= 2^20 SNP rows
+constexpr size_t SCAN_SCRATCH_DIG = size_t(16) << 16;        // 16 columns x n_pad <= 65,536
+constexpr size_t SCAN_SCRATCH_BYTES = SCAN_SCRATCH_ACC_ROWS * 16 * 4 + SCAN_SCRATCH_DIG + 16 * 8;
+static __device__ __align__(256) uint8_t scan_scratch[SCAN_SCRATCH_BYTES];
+
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[64] = {nullptr};
+
+static int scratch_ptr(int device, void** out) {
+  if (device < 0 || device >= 64) { set_error("scan_noalloc: device index out of range"); return BMG_E_ARG; }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    int cur = 0;
+    BMG_HIP(hipGetDevice(&cur));
+    if (cur != device) BMG_HIP(hipSetDevice(device));
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scan_scratch));
+    if (cur != device) (void)hipSetDevice(cur);
+    if (e != hipSuccess || !ptr) return hip_fail(e, "hipGetSymbolAddress(scan_scratch)");
+    g_scratch_ptr[device] = ptr;
+  }
+  *out = g_scratch_ptr[device];
+  return BMG_OK;
+}
+
+int launch_scan_noalloc(bmg_ctx* c, const double* r_dev, int nvec, int coding, int digits,
+                        double* out_dev, hipStream_t s) {
+  int rc = scan_check(nvec, digits);
+  if (rc) return rc;
+  if (c->p + 1024 > SCAN_SCRATCH_ACC_ROWS || (size_t)16 * c->n_pad > SCAN_SCRATCH_DIG) {
+    set_error("scan_noalloc: the allocation-free scan holds p <= 1,047,552 SNPs and n <= 65,536 "
+              "individuals; use bmg_scan_xtr_dev");
+    return BMG_E_CAPACITY;
+  }
+  void* base = nullptr;
+  rc = scratch_ptr(c->device, &base);
+  if (rc) return rc;
+  uint8_t* b8 = (uint8_t*)base;
+  ScanBufs B{(int8_t*)(b8 + SCAN_SCRATCH_ACC_ROWS * 16 * 4), (int32_t*)b8,
+             (double*)(b8 + SCAN_SCRATCH_ACC_ROWS * 16 * 4 + SCAN_SCRATCH_DIG)};
+  rc = scan_digits(c, r_dev, nvec, digits, B, s);
+  if (rc) return rc;
+  // the shape the tuner picked for this width if it ran, else the fixed one
+  // (no timing here: tuning creates events)
+  const int NC = nvec * digits;
+  ScanShape sh = c->scan_tw[NC] ? make_shape(c, NC, c->scan_nt[NC], c->scan_wv[NC], c->scan_tw[NC], c->scan_gx[NC])
+                                : make_shape(c, NC, 2, 8, tile_for(NC, 163840), 256);
+  rc = scan_kernel(c, nvec, coding, digits, sh, B, s);
+  if (rc) return rc;
   hipLaunchKernelGGL(k_scan_combine, dim3((unsigned)((c->p + 255) / 256), nvec), dim3(256), 0, s,
                      B.acc, c->p, nvec, digits, B.scale, out_dev);
   BMG_HIP(hipGetLastError());

@@ -55,6 +55,9 @@ int launch_synth(bmg_ctx* c, uint32_t seed, const double* thresh_dev, const doub
                  const uint64_t* blk_start_dev, uint64_t n_blocks, hipStream_t s);
 int launch_scan(bmg_ctx* c, const double* r_dev, int nvec, int coding, int digits,
                 double* out_dev, hipStream_t s);
+// the same scan with its scratch in a file-scope device array (no allocation)
+int launch_scan_noalloc(bmg_ctx* c, const double* r_dev, int nvec, int coding, int digits,
+                        double* out_dev, hipStream_t s);
 int launch_scan_timed(bmg_ctx* c, const double* r_dev, int nvec, int coding, int digits,
                       double* out_dev, int reps, double* ms, hipStream_t s);
 int launch_band(bmg_ctx* c, uint32_t W, int ca, int cb, int32_t* out_dev, hipStream_t s);

@@ -130,6 +130,18 @@ int bmg_scan_xtr(bmg_ctx* ctx, const double* r, int nvec, int coding, int digits
 /* device-pointer form for timing: r_dev nvec x n_pad (double), out_dev nvec x p */
 int bmg_scan_xtr_dev(bmg_ctx* ctx, const double* r_dev, int nvec, int coding,
                      int digits, double* out_dev, void* stream);
+/* No counterpart in the reference: bmg_scan_xtr_dev for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments and results, bit for
+ * bit; the scratch (digit planes, accumulator, scales) is one file-scope
+ * device array of 65.0 MiB per GPU, held until the process ends, so it
+ * takes p <= 1,047,552 SNPs and n <= 65,536 individuals (BMG_E_CAPACITY
+ * beyond).  The scan shape is the one the autotuner picked for this width
+ * on the context if it ran, else a fixed shape (no tuning, no events).
+ * Calls that overlap on one GPU share the array and race: serialise them
+ * (one stream, or events). */
+int bmg_scan_xtr_dev_noalloc(bmg_ctx* ctx, const double* r_dev, int nvec, int coding, int digits,
+                             double* out_dev, void* stream);
 int bmg_ctx_n_pad(bmg_ctx* ctx, uint32_t* n_pad);
 /* Timing form of bmg_scan_xtr_dev: prepares the digit planes once, then
  * launches the int8-MFMA scan kernel `reps` times on `stream`, bracketing
