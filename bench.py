@@ -233,10 +233,12 @@ def phase_profile(bmg, synth, ctx, args):
     p1 = [b.profile(c) for c in range(n)]
     t1 = b.timing()
     it = args.steps * ips
+    busy = []
     for c in range(n):
         d = {k: p1[c][k] - p0[c][k] for k in p0[c]}
         na, nr = max(d["n_add"], 1), max(d["n_rem"], 1)
         us = lambda k, per: round(d[k] / per * 1e3, 2)
+        busy.append(d["total"] / it * 1e3)
         print(json.dumps(dict(chain=c, size=b.state(c)["size"], us_per_iter=us("total", it),
                               adds_per_iter=round(d["n_add"] / it, 2), removes_per_iter=round(d["n_rem"] / it, 2),
                               add_xy=us("add_xy", na), add_xtx=us("add_xtx", na), add_solve=us("add_trsv", na),
@@ -247,7 +249,12 @@ def phase_profile(bmg, synth, ctx, args):
                               addrem_moves=us("addrem", it), statechange=us("statechange", it),
                               swap=us("swap", it), second_stage=us("dr", it),
                               second_stage_scoring=us("dr_score", it))), flush=True)
-    print(json.dumps(dict(kernel_us_per_iter=round((t1["chain_ms"] - t0["chain_ms"]) / it * 1e3, 1))), flush=True)
+    # a launch lasts as long as its slowest chain: the share of the chains'
+    # workgroup-time spent waiting for it is the launch's idle fraction
+    print(json.dumps(dict(kernel_us_per_iter=round((t1["chain_ms"] - t0["chain_ms"]) / it * 1e3, 1),
+                          slowest_chain_us_per_iter=round(max(busy), 1),
+                          mean_chain_us_per_iter=round(float(np.mean(busy)), 1),
+                          idle_fraction=round(1.0 - float(np.mean(busy)) / max(busy), 3))), flush=True)
     b.close()
 
 

@@ -114,6 +114,7 @@ SIGNATURES = {
     "bmg_ctx_create_synthetic": (C.c_int, C.c_int, C.c_uint32, C.c_uint32, C.c_uint64, _D, _D, _I32,
                                  C.c_int, _PP),
     "bmg_ctx_set_y": (C.c_int, _P, _D),
+    "bmg_ctx_set_covariates": (C.c_int, _P, _D, C.c_uint32),
     "bmg_ctx_create_subset": (C.c_int, _P, _U32, C.c_uint32, _D, _I32, C.c_int, _PP),
     "bmg_ctx_destroy": (C.c_int, _P),
     "bmg_ctx_read_body": (C.c_int, _P, _U8),
@@ -186,6 +187,7 @@ SIGNATURES = {
     "bmg_batch_p_proposal": (C.c_int, _P, C.c_int, _D),
     "bmg_batch_trace": (C.c_int, _P, C.c_int, _U8, _U8, _U8, _U32, _D, C.c_int64, _I64),
     "bmg_batch_write_outputs": (C.c_int, _P, C.c_int, C.c_char_p),
+    "bmg_batch_thinned_beta": (C.c_int, _P, C.c_int, _D, C.c_int64, _I64),
     "bmg_batch_thinned": (C.c_int, _P, C.c_int, _U32, C.c_int64, _U32, C.c_int64, _D, _D, _D, _I64, _I64),
     "bmg_batch_rb": (C.c_int, _P, C.c_int, _D, _D),
     "bmg_batch_timing": (C.c_int, _P, _D),
@@ -224,7 +226,7 @@ class Ctx:
     """One dataset in HBM: Data + DataModel + PrecomputedSNPCovariances."""
 
     def __init__(self, n, p, body=None, y=None, types=("A",), recode=False, device=0,
-                 synth=None):
+                 synth=None, e_cov=None):
         self.n, self.p = n, p
         tv = np.array(sorted(EF[t] for t in types), dtype=np.int32)
         self.types = tv
@@ -241,6 +243,13 @@ class Ctx:
             check(lib().bmg_ctx_create(device, _p(b, C.c_uint8), n, p, _p(yy), _p(tv, C.c_int32),
                                        len(tv), 1 if recode else 0, C.byref(h)))
         self.h = h
+        if e_cov is not None:
+            self.set_covariates(e_cov)
+
+    def set_covariates(self, e_cov):
+        """Covariates n x m_e (the constant column is added by the engine)."""
+        e = np.asfortranarray(np.asarray(e_cov, dtype=np.float64).reshape(self.n, -1))
+        check(lib().bmg_ctx_set_covariates(self.h, e.ctypes.data_as(C.POINTER(C.c_double)), e.shape[1]))
 
     @classmethod
     def subset(cls, pool, individuals, y, types=("A",)):
@@ -684,6 +693,14 @@ class Batch:
                                       _p(loci, C.c_uint32), nl.value, _p(ll), _p(s2), _p(al),
                                       C.byref(ns), C.byref(nl)))
         return dict(sizes=sizes, loci=loci[:nl.value], loglik=ll, sigma2=s2, alpha=al)
+
+    def thinned_beta(self, c):
+        """save_beta: the beta of every thinned sample, concatenated."""
+        n = C.c_int64()
+        check(lib().bmg_batch_thinned_beta(self.h, c, None, 0, C.byref(n)))
+        out = np.zeros(max(n.value, 1))
+        check(lib().bmg_batch_thinned_beta(self.h, c, _p(out), n.value, C.byref(n)))
+        return out[:n.value]
 
     def rb(self, c, y_hat):
         yh = np.ascontiguousarray(y_hat, dtype=np.float64)

@@ -97,6 +97,22 @@ extern "C" int bmg_ctx_set_y(bmg_ctx* c, const double* y) {
   return BMG_OK;
 }
 
+extern "C" int bmg_ctx_set_covariates(bmg_ctx* c, const double* e, uint32_t m_e) {
+  if (!c || (m_e > 0 && !e)) { set_error("null argument"); return BMG_E_ARG; }
+  if (m_e > 32) { set_error("at most 32 covariates on the device"); return BMG_E_UNSUPPORTED; }
+  BMG_HIP(hipSetDevice(c->device));
+  if (c->E) { BMG_HIP(hipFree(c->E)); c->E = nullptr; }
+  c->m_e = m_e;
+  c->e_host.assign(e, e + (size_t)c->n * m_e);
+  if (m_e == 0) return BMG_OK;
+  std::vector<double> padded((size_t)m_e * c->n_pad, 0.0);
+  for (uint32_t k = 0; k < m_e; ++k)
+    for (uint32_t i = 0; i < c->n; ++i) padded[(size_t)k * c->n_pad + i] = e[(size_t)k * c->n + i];
+  BMG_HIP(hipMalloc(&c->E, sizeof(double) * padded.size()));
+  BMG_HIP(hipMemcpy(c->E, padded.data(), sizeof(double) * padded.size(), hipMemcpyHostToDevice));
+  return BMG_OK;
+}
+
 extern "C" int bmg_ctx_create(int device, const uint8_t* bed_body, uint32_t n, uint64_t p,
                               const double* y, const int32_t* types, int n_types, int recode,
                               bmg_ctx** out) {
@@ -182,7 +198,7 @@ extern "C" int bmg_ctx_destroy(bmg_ctx* c) {
   if (!c) return BMG_OK;
   hipSetDevice(c->device);
   if (c->stream) hipStreamSynchronize(c->stream);
-  hipFree(c->g); hipFree(c->y); hipFree(c->pre);
+  hipFree(c->g); hipFree(c->y); hipFree(c->pre); hipFree(c->E);
   hipFree(c->dig); hipFree(c->acc); hipFree(c->scale); hipFree(c->tmp_r); hipFree(c->tmp_out);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
@@ -484,6 +500,8 @@ static DevData make_devdata(bmg_ctx* c) {
   D.type = c->types[0];
   D.n_types = c->n_types;
   for (int i = 0; i < 5; ++i) D.types[i] = c->types[i];
+  D.me = 1 + (int)c->m_e;
+  D.E = c->E;
   tree_layout(D);
   return D;
 }
@@ -775,7 +793,10 @@ static int batch_create(bmg_ctx** ctxs, int n_ctx, int n_chains, const int32_t* 
       delete b; set_error("the device sampler needs exactly one effect type, equal to the dataset's");
       return BMG_E_UNSUPPORTED;
     }
-    if (o.m_e != 0) { delete b; set_error("covariates are not supported on the device"); return BMG_E_UNSUPPORTED; }
+    if (o.m_e != x->m_e) {
+      delete b; set_error("m_e of the options differs from the dataset's covariates (bmg_ctx_set_covariates)");
+      return BMG_E_ARG;
+    }
     if (!model_api && x->n_missing != 0) { delete b; set_error("missing genotypes are not supported on the device"); return BMG_E_UNSUPPORTED; }
     if (o.thin == 0 || o.n_rao == 0 || o.n_sample_tau2_and_missing == 0 || o.n_rao % o.thin != 0) {
       delete b; set_error("need thin, n_rao, n_sample_tau2_and_missing > 0 and n_rao % thin == 0");
@@ -833,6 +854,12 @@ static int batch_create(bmg_ctx** ctxs, int n_ctx, int n_chains, const int32_t* 
   size_t o_ths = add(4 * th_cap), o_thl = add(4 * th_loci_cap), o_thll = add(8 * th_cap),
          o_ths2 = add(8 * th_cap), o_tha = add(8 * th_cap), o_thp = add(8 * 3 * th_cap);
   size_t o_dr = add(dr_max ? (size_t)8 << dr_max : 0);
+  // save_beta: every thinned sample's beta (the fixed columns plus up to two
+  // per locus)
+  int any_beta = 0;
+  for (int c = 0; c < n_chains; ++c) any_beta |= opts[c].save_beta != 0;
+  const uint64_t th_beta_cap = any_beta ? 2 * th_loci_cap + th_cap * (uint64_t)(1 + 32) : 0;
+  size_t o_thb = add(8 * th_beta_cap);
   b->per_chain = sz;
   hipError_t e = hipMalloc(&b->pool, sz * n_chains);
   if (e != hipSuccess) { bmg_batch_destroy(b); return hip_fail(e, "hipMalloc(chain pool)"); }
@@ -859,6 +886,7 @@ static int batch_create(bmg_ctx** ctxs, int n_ctx, int n_chains, const int32_t* 
     A.th_ll = (double*)(base + o_thll); A.th_s2 = (double*)(base + o_ths2); A.th_alpha = (double*)(base + o_tha);
     A.th_pve = (double*)(base + o_thp);
     A.dr_lp = dr_max ? (double*)(base + o_dr) : nullptr;
+    A.th_beta = o.save_beta ? (double*)(base + o_thb) : nullptr;
     b->h_arrays.push_back(A);
     // constants (src/sampler.hpp:107-203, src/prior.hpp:45-122)
     ChainConst Kc{};
@@ -905,30 +933,47 @@ static int batch_create(bmg_ctx** ctxs, int n_ctx, int n_chains, const int32_t* 
     Kc.use_ind = o.use_individual_tau2;
     Kc.inv_tau2_e0 = o.inv_tau2_e_const_val;
     Kc.tr_cap = tr_cap; Kc.th_cap = th_cap; Kc.th_loci_cap = th_loci_cap;
+    Kc.th_beta_cap = o.save_beta ? th_beta_cap : 0;
     b->h_consts.push_back(Kc);
     // initial scalars: empty model (constant column only)
     S.alpha = alpha; S.alpha2 = alpha2;
     S.n_rao_burnin = o.n_rao_burnin; S.p_rao_n = 0; S.p_proposal_n = 1;
     S.p_ms = o.p_move_size;
     for (int q = 0; q < 3; ++q) { S.n_acpt[q] = 1; S.n_moves[q] = 2; }
-    S.cols = 1; S.nloci = 0; S.beta_len = 0; S.mu_computed = 0;
+    const int me = 1 + (int)x->m_e;   // the constant and the covariates
+    S.cols = me; S.nloci = 0; S.beta_len = 0; S.mu_computed = 0;
     S.ll = NAN;
     // p_proposal flat (src/sampler.hpp:351-362)
     p0[c] = ga / (ga + gb) * m_g / m_g;
     // vectors: xy[0] = sum y, d[0] = inv_tau2_e const
     std::fill(vec.begin(), vec.end(), 0.0);
     std::fill(idx.begin(), idx.end(), 0);
-    double sy = 0.0;
-    for (uint32_t i = 0; i < x->n; ++i) sy += x->y_host[i];
-    vec[0] = sy;
-    vec[2 * G] = o.inv_tau2_e_const_val;
-    idx[3 * G] = 0;        // slot of the constant column
-    idx[4 * G] = -1;       // its "SNP"
-    idx[5 * G] = 4;        // covariate marker (src/model.hpp:98-101)
+    // the fixed columns (src/model.hpp:92-105): E'y, E'E and the covariate
+    // precisions, each dot summed in individual order; column 0 is the
+    // constant (its dots are sums)
+    const uint32_t nn_ = x->n;
+    auto ecol = [&](int c, uint32_t i) { return c == 0 ? 1.0 : x->e_host[(size_t)(c - 1) * nn_ + i]; };
+    for (int c = 0; c < me; ++c) {
+      double sy = 0.0;
+      for (uint32_t i = 0; i < nn_; ++i) sy += ecol(c, i) * x->y_host[i];
+      vec[c] = sy;
+      vec[2 * G + c] = c == 0 ? o.inv_tau2_e_const_val : o.inv_tau2_e_val;
+      idx[3 * G + c] = c;     // slot
+      idx[4 * G + c] = -1;    // no SNP
+      idx[5 * G + c] = 4;     // covariate marker (src/model.hpp:98-101)
+    }
     BMG_HIP(hipMemcpyAsync(A.vec, vec.data(), sizeof(double) * 5 * G, hipMemcpyHostToDevice, b->stream));
     BMG_HIP(hipMemcpyAsync(A.idx, idx.data(), sizeof(int32_t) * 8 * G, hipMemcpyHostToDevice, b->stream));
-    double nn = (double)x->n;
-    BMG_HIP(hipMemcpyAsync(A.xxs, &nn, sizeof(double), hipMemcpyHostToDevice, b->stream));
+    std::vector<double> ee((size_t)me * me);
+    for (int r = 0; r < me; ++r)
+      for (int c = r; c < me; ++c) {
+        double sum = 0.0;
+        for (uint32_t i = 0; i < nn_; ++i) sum += ecol(r, i) * ecol(c, i);
+        ee[(size_t)r * me + c] = ee[(size_t)c * me + r] = sum;
+      }
+    for (int r = 0; r < me; ++r)
+      BMG_HIP(hipMemcpyAsync(A.xxs + (size_t)r * G, ee.data() + (size_t)r * me, sizeof(double) * me,
+                             hipMemcpyHostToDevice, b->stream));
     std::fill(adapt.begin(), adapt.end(), 0.0);
     geometric_cdf_h((int)Kc.max_ms, o.p_move_size, adapt.data());
     geometric_cdf_h((int)(Kc.max_ms + 1) / 2, o.p_move_size_nbs, adapt.data() + 256);
@@ -1473,6 +1518,26 @@ extern "C" int bmg_batch_write_outputs(bmg_batch* b, int chain, const char* base
     rc = write_file(base + f.name, f.p, f.bytes);
     if (rc) return rc;
   }
+  if (K.th_beta_cap) {
+    // src/sampler.cpp:710-713: beta of every thinned sample, concatenated
+    std::vector<double> beta(s.th_beta_len);
+    if (!beta.empty())
+      BMG_HIP(hipMemcpy(beta.data(), A.th_beta, 8 * beta.size(), hipMemcpyDeviceToHost));
+    rc = write_file(base + "_beta.dat", beta.data(), 8 * beta.size());
+    if (rc) return rc;
+  }
+  return BMG_OK;
+}
+
+extern "C" int bmg_batch_thinned_beta(bmg_batch* b, int chain, double* out, int64_t cap, int64_t* len) {
+  ChainScal s;
+  int rc = get_scal(b, chain, &s);
+  if (rc) return rc;
+  if (!b->h_consts[chain].th_beta_cap) { set_error("the chain does not save beta (save_beta = 0)"); return BMG_E_STATE; }
+  if (len) *len = (int64_t)s.th_beta_len;
+  const int64_t k = std::min<int64_t>(cap, (int64_t)s.th_beta_len);
+  if (out && k > 0)
+    BMG_HIP(hipMemcpy(out, b->h_arrays[chain].th_beta, 8 * k, hipMemcpyDeviceToHost));
   return BMG_OK;
 }
 
@@ -1514,6 +1579,10 @@ static int launch_model_op(bmg_batch* b, ModelOpArgs a) {
 extern "C" int bmg_model_create(bmg_ctx* ctx, const bmg_model_prior* prior, bmg_model** out) {
   if (!ctx || !prior || !out) { set_error("null argument"); return BMG_E_ARG; }
   if (ctx->n_types < 1) { set_error("dataset has no effect type"); return BMG_E_ARG; }
+  if (ctx->m_e > 0) {
+    set_error("a dataset with covariates needs their precisions: use bmg_model_create_sampler");
+    return BMG_E_ARG;
+  }
   // a model is a one-chain batch whose constants carry the given prior
   bmg_opts o{};
   o.n = ctx->n; o.m_g = ctx->p; o.do_n_iter = 1; o.n_rao = 1; o.thin = 1; o.n_sample_tau2_and_missing = 1;

@@ -32,6 +32,8 @@ struct DevData {
   int lH;                // learned-swap window (SNPs each side, <= 32)
   const double* xy[4];   // x_j . y of every SNP per coding (null: computed per add)
   int mixed;             // columns may carry codings other than `type` (Model API)
+  int me;                // fixed columns: the constant plus the covariates
+  const double* E;       // covariate c >= 1 at E + (c - 1) * n_pad (zero past n)
 };
 
 // Per-chain constants (options + solved prior, src/sampler.hpp:107-203,
@@ -48,6 +50,7 @@ struct ChainConst {
   int32_t use_ind;
   double inv_tau2_e0;
   uint64_t tr_cap, th_cap, th_loci_cap;
+  uint64_t th_beta_cap;  // save_beta: doubles of thinned beta kept (0: not saved)
   double p_learned;      // probability of a learned swap per iteration
   int32_t dr;            // delayed-rejection move-size limit (0: none)
   int32_t max_size;      // largest model (SNPs) with nonzero prior
@@ -84,7 +87,7 @@ struct ChainScal {
   double rao_z1, rao_z2, prop_z1, prop_z2;
   uint32_t rb_call_idx;
   int32_t err;
-  uint64_t tr_count, th_count, th_loci_len;
+  uint64_t tr_count, th_count, th_loci_len, th_beta_len;
   double tot_add, tot_rem;   // sum-tree totals
   // capacity: a chain whose model comes within col_reserve columns of the
   // launch's capacity continues in the 256-column kernel (big = 1) from the
@@ -112,6 +115,7 @@ struct ChainArrays {
   uint8_t* tr_move; uint8_t* tr_jd; uint8_t* tr_ms; uint32_t* tr_size; double* tr_ll;
   uint32_t* th_size; uint32_t* th_loci; double* th_ll; double* th_s2; double* th_alpha;
   double* th_pve;        // 3 per thinned sample (src/model.hpp:345-392)
+  double* th_beta;       // save_beta: beta of every thinned sample, concatenated
   double* dr_lp;         // 2^dr second-stage log-probabilities (null: dr == 0)
 };
 

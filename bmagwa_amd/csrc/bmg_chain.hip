@@ -223,7 +223,7 @@ __device__ __forceinline__ void planes_c(uint32_t w, int coding, uint32_t& p1, u
 //
 // A lane owns one 16-word chunk of the individuals and a group of columns:
 // it decodes the new column's chunk once, then streams its columns' chunks
-// (the next column's words are loaded while the current one is summed);
+// (four columns' words are loaded together, then summed);
 // the chunk sums meet in LDS integer accumulators (exact, so the order is
 // free).  W / 16 lanes cover a column, so 256 / (W / 16) column groups run
 // side by side.  CODING >= 0: every column has that coding (one effect type).
@@ -248,45 +248,64 @@ __device__ __forceinline__ void column_dots(Ctx<KMAX>& C, uint64_t snp_new, int 
 #pragma unroll
         for (int u = 0; u < 4; ++u) planes_c<CODING>(xs[u], coding_new, n1[4 * q + u], n2[4 * q + u]);
       }
+      const int me = C.D.me;
       auto col_ptr = [&](int c) -> const uint32_t* {
-        return (c >= cols || c == 0) ? wn : xcol + (size_t)S.slot[c] * W;
+        return (c >= cols || c < me) ? wn : xcol + (size_t)S.slot[c] * W;
       };
-      int c = grp;
-      uint4 nx[4];
-      if (c <= cols) {
-        const uint32_t* pc = col_ptr(c);
+      // four columns of the group per round: their 16 loads are in flight
+      // together, so a lane waits on memory once per four columns
+      for (int c = grp; c <= cols; c += 4 * groups) {
+        uint4 cur[4][4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) nx[q] = *(const uint4*)(pc + w0 + 4 * q);
-      }
-      for (; c <= cols; c += groups) {
-        uint4 cur[4];
+        for (int u = 0; u < 4; ++u) {
+          const int cu = c + u * groups;
+          const uint32_t* pc = cu <= cols ? col_ptr(cu) : wn;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) cur[q] = nx[q];
-        const int cn = c + groups;
-        if (cn <= cols) {
-          const uint32_t* pc = col_ptr(cn);
-#pragma unroll
-          for (int q = 0; q < 4; ++q) nx[q] = *(const uint4*)(pc + w0 + 4 * q);
+          for (int q = 0; q < 4; ++q) cur[u][q] = *(const uint4*)(pc + w0 + 4 * q);
         }
-        const int cc = (c >= cols || c == 0) ? coding_new : (int)S.xtype[c];
-        int sum = 0;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const uint32_t xs[4] = {cur[q].x, cur[q].y, cur[q].z, cur[q].w};
+        for (int u = 0; u < 4; ++u) {
+          const int cu = c + u * groups;
+          if (cu > cols) break;
+          const int cc = (cu >= cols || cu == 0) ? coding_new : (int)S.xtype[cu];
+          int sum = 0;
 #pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            uint32_t a1, a2;
-            planes_c<CODING>(xs[u], cc, a1, a2);
-            // column 0 is the constant: its planes are all ones
-            if (c == 0) { a1 = BMG_EVEN; a2 = 0u; }
-            const int k = 4 * q + u;
-            sum += (int)(__popc(a1 & n1[k]) + __popc(a1 & n2[k]) + __popc(a2 & n1[k]) + __popc(a2 & n2[k]));
+          for (int q = 0; q < 4; ++q) {
+            const uint32_t xs[4] = {cur[u][q].x, cur[u][q].y, cur[u][q].z, cur[u][q].w};
+#pragma unroll
+            for (int k4 = 0; k4 < 4; ++k4) {
+              uint32_t a1, a2;
+              planes_c<CODING>(xs[k4], cc, a1, a2);
+              // column 0 is the constant: its planes are all ones
+              if (cu == 0) { a1 = BMG_EVEN; a2 = 0u; }
+              const int k = 4 * q + k4;
+              sum += (int)(__popc(a1 & n1[k]) + __popc(a1 & n2[k]) + __popc(a2 & n1[k]) + __popc(a2 & n2[k]));
+            }
           }
+          if (cu == 0 || cu >= me) atomicAdd(&S.iacc[cu], sum);   // covariates: cov_dots
         }
-        atomicAdd(&S.iacc[c], sum);
       }
     }
   }
+}
+
+// S.work[c] = e_c^T x_new for the covariate columns c in [1, me): each
+// thread sums over its individuals, then a fixed-order block reduction
+template <int KMAX>
+__device__ __forceinline__ void cov_dots(Ctx<KMAX>& C, uint64_t snp_new, int coding_new) {
+  const int me = C.D.me;
+  const uint32_t* __restrict__ wn = C.D.g + snp_new * C.D.W32;
+  for (int c = 1; c < me; ++c) {
+    const double* __restrict__ e = C.D.E + (size_t)(c - 1) * C.D.n_pad;
+    double acc = 0.0;
+    for (uint32_t i = threadIdx.x; i < C.D.n; i += BMG_BLOCK) {
+      const int iv = code_ival((wn[i >> 4] >> (2 * (i & 15))) & 3u, coding_new);
+      if (iv > 0) acc += (double)iv * e[i];
+    }
+    acc = block_sum(acc, C.S.red);
+    if (threadIdx.x == 0) C.S.work[c] = acc;
+  }
+  __syncthreads();
 }
 
 template <int KMAX>
@@ -309,6 +328,7 @@ __device__ __forceinline__ void new_column_dots(Ctx<KMAX>& C, uint64_t snp_new, 
   __syncthreads();
   for (int c = threadIdx.x; c <= cols; c += BMG_BLOCK) S.work[c] = (double)S.iacc[c];
   __syncthreads();
+  if (C.D.me > 1) cov_dots(C, snp_new, coding_new);
 }
 
 // ----------------------------------------------------- triangular (wave 0)
@@ -321,13 +341,17 @@ __device__ __forceinline__ double readlane_d(double v, int src) {
 }
 
 // x := U^-T x for x[0..m) in LDS (dtrsv Upper/Trans order).  Wave 0 only.
-// Lane l holds x[l + 64 t] in b[t] and the diagonal entry of that row in
-// dg[t]; the outer loop over t is unrolled so the register arrays are only
-// ever indexed by constants.  Row i: every lane divides its own pair and the
-// owner's quotient is read across lanes (so the serial chain per row is one
-// division and one cross-lane read, with no LDS wait), and the next row's
-// coefficients are loaded while this row's updates run.  Per element the
-// arithmetic is the sequential loop's.
+// Lane l holds x[l + 64 t] in b[t] and the reciprocal of that row's
+// diagonal entry in rd[t] (all computed at once before the sweep); the
+// outer loop over t is unrolled so the register arrays are only ever
+// indexed by constants.  Row i: every lane scales its own entry and the
+// owner's product is read across lanes, so the serial chain per row is one
+// multiply, one cross-lane read and one fused multiply-subtract, with no
+// division and no LDS wait; the next row's coefficients are loaded while
+// this row's updates run.  Per element the subtractions come in row order,
+// as in the sequential loop; the reciprocal and the fused update round
+// differently from its divisions (by an ulp), which the log-likelihoods'
+// 1e-10 agreement with the oracle absorbs.
 template <int KMAX>
 __device__ __forceinline__ void trsv_ut_w0(Smem<KMAX>& S, double* x, int m) {
   constexpr int NT = (KMAX + 63) / 64;
@@ -337,7 +361,7 @@ __device__ __forceinline__ void trsv_ut_w0(Smem<KMAX>& S, double* x, int m) {
   for (int t = 0; t < NT; ++t) {
     int j = lane + 64 * t;
     b[t] = j < m ? x[j] : 0.0;
-    dg[t] = j < m ? S.L[P(j) + j] : 1.0;
+    dg[t] = j < m ? 1.0 / S.L[P(j) + j] : 1.0;
   }
 #pragma unroll
   for (int ti = 0; ti < NT; ++ti) {
@@ -352,7 +376,7 @@ __device__ __forceinline__ void trsv_ut_w0(Smem<KMAX>& S, double* x, int m) {
     }
     for (int i = i0; i < iend; ++i) {
       const int owner = i & 63;
-      const double xi = readlane_d(b[ti] / dg[ti], owner);
+      const double xi = readlane_d(b[ti] * dg[ti], owner);
       double cn[NT];
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
@@ -363,7 +387,7 @@ __device__ __forceinline__ void trsv_ut_w0(Smem<KMAX>& S, double* x, int m) {
 #pragma unroll
       for (int t = ti; t < NT; ++t) {
         int j = lane + 64 * t;
-        if (j > i && j < m) b[t] = b[t] - cf[t] * xi;
+        if (j > i && j < m) b[t] = fma(-cf[t], xi, b[t]);
       }
 #pragma unroll
       for (int t = 0; t < NT; ++t) cf[t] = cn[t];
@@ -373,28 +397,50 @@ __device__ __forceinline__ void trsv_ut_w0(Smem<KMAX>& S, double* x, int m) {
   for (int t = 0; t < NT; ++t) { int j = lane + 64 * t; if (j < m) x[j] = b[t]; }
 }
 
+// x := U^-1 x (dtrsv Upper/NoTrans, back substitution).  Wave 0 only.  Lane
+// l holds x[l + 64 t] and the reciprocal of that row's diagonal; column j's
+// entries for the next row are read from LDS while row j's updates run, so
+// the serial chain per row is a multiply, a cross-lane read and a fused
+// update (rounding within an ulp of the divide-and-subtract loop).
 template <int KMAX>
 __device__ __forceinline__ void trsv_un_w0(Smem<KMAX>& S, double* x, int m) {
   constexpr int NT = (KMAX + 63) / 64;
   const int lane = lane_id();
-  double b[NT];
+  double b[NT], rd[NT], cf[NT];
 #pragma unroll
-  for (int t = 0; t < NT; ++t) { int j = lane + 64 * t; b[t] = j < m ? x[j] : 0.0; }
+  for (int t = 0; t < NT; ++t) {
+    int j = lane + 64 * t;
+    b[t] = j < m ? x[j] : 0.0;
+    rd[t] = j < m ? 1.0 / S.L[P(j) + j] : 1.0;
+  }
+  if (m <= 0) return;
+#pragma unroll
+  for (int q = 0; q < NT; ++q) {
+    int i = lane + 64 * q;
+    cf[q] = i < m - 1 ? S.L[P(m - 1) + i] : 0.0;
+  }
 #pragma unroll
   for (int tj = NT - 1; tj >= 0; --tj) {
     const int jend = m < 64 * (tj + 1) ? m : 64 * (tj + 1);
     for (int j = jend - 1; j >= 64 * tj; --j) {
       const int owner = j & 63;
-      double t = readlane_d(b[tj], owner);
-      if (t != 0.0) {
-        t = t / S.L[P(j) + j];
-        if (lane == owner) b[tj] = t;
+      double cn[NT];
+#pragma unroll
+      for (int q = 0; q < NT; ++q) {
+        int i = lane + 64 * q;
+        cn[q] = (j >= 1 && i < j - 1) ? S.L[P(j - 1) + i] : 0.0;
+      }
+      const double xj = readlane_d(b[tj] * rd[tj], owner);
+      if (xj != 0.0) {
+        if (lane == owner) b[tj] = xj;
 #pragma unroll
         for (int q = 0; q <= tj; ++q) {
           int i = lane + 64 * q;
-          if (i < j) b[q] = b[q] - t * S.L[P(j) + i];
+          if (i < j) b[q] = fma(-xj, cf[q], b[q]);
         }
       }
+#pragma unroll
+      for (int q = 0; q < NT; ++q) cf[q] = cn[q];
     }
   }
 #pragma unroll
@@ -418,6 +464,21 @@ __device__ __forceinline__ void drotg(double& a, double& b, double& c, double& s
     if (fabs(sb) >= fabs(sa) && c != 0.0) z = 1.0 / c;
   }
   a = r; b = z;
+}
+
+// The rotation drotg generates (same sign convention: r takes the sign of
+// the larger input), without its overflow scaling, which a Cholesky factor's
+// entries never need: one square root and one division instead of four
+// divisions, so the downdate's serial chain per step is shorter.  c, s and r
+// agree with drotg's to rounding.
+__device__ __forceinline__ void rot_gen(double& a, double b, double& c, double& s) {
+  const double roe = fabs(a) > fabs(b) ? a : b;
+  if (a == 0.0 && b == 0.0) { c = 1.0; s = 0.0; a = 0.0; return; }
+  double r = sqrt(fma(a, a, b * b));
+  r = roe >= 0.0 ? r : -r;
+  const double ri = 1.0 / r;
+  c = a * ri; s = b * ri;
+  a = r;
 }
 
 // Cholesky downdate removing column cr of the m x m factor (LINPACK dchex
@@ -459,7 +520,7 @@ __device__ __forceinline__ void chol_downdate(Smem<KMAX>& S, int m, int cr) {
         double a = readlane_d(top[ti], owner), b = readlane_d(nxt[ti], owner);
         const double vbn = j + 2 < m ? S.v[j + 2] : 0.0;   // next row of v, read ahead
         double cc, ss;
-        drotg(a, b, cc, ss);
+        rot_gen(a, b, cc, ss);
         if (lane == owner) S.L[P(j + 1) + j] = a;
         {
           const double tv = cc * vt + ss * vb;
@@ -893,14 +954,21 @@ __device__ __forceinline__ void compute_log_likelihood(Ctx<KMAX>& C) {
   Reg& R = C.R;
   int k = R.cols;
   if (k < 1) { R.ll = NAN; return; }
-  for (int e = threadIdx.x; e < P(k); e += BMG_BLOCK) {
-    int j = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
-    while (P(j + 1) <= e) ++j;
-    while (P(j) > e) --j;
-    int i = e - P(j);
-    double v = C.A.xxs[(size_t)S.slot[i] * BMG_KMAXG + S.slot[j]];
-    if (i == j) v += S.d[i];
-    S.L[e] = v;
+  // X'X + D from the slot matrix into the packed factor: a thread per
+  // column j, its rows i <= j read eight at a time (row slot(j) of the slot
+  // matrix holds column j's entries), so the loads are in flight together
+  if ((int)threadIdx.x < k) {
+    const int j = (int)threadIdx.x;
+    const double* __restrict__ row = C.A.xxs + (size_t)S.slot[j] * BMG_KMAXG;
+    double* col = S.L + P(j);
+    for (int i0 = 0; i0 <= j; i0 += 8) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = row[S.slot[i0 + u <= j ? i0 + u : j]];   // past j: a valid slot
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (i0 + u <= j) col[i0 + u] = i0 + u == j ? v[u] + S.d[j] : v[u];
+    }
   }
   double ls = 0.0;
   for (int i = 1 + threadIdx.x; i < k; i += BMG_BLOCK) ls += log(S.d[i]);
@@ -1074,29 +1142,30 @@ __device__ __forceinline__ double coded_times(uint32_t code, double t, double t2
 }
 
 template <int KMAX, int CODING>
-__device__ __forceinline__ void xbeta_impl(Ctx<KMAX>& C, double* out5, int write_yhat) {
+__device__ __forceinline__ void xbeta_impl(Ctx<KMAX>& C, double* out7, int write_yhat) {
   Smem<KMAX>& S = C.S;
   Reg& R = C.R;
   const double b0 = S.beta[0];
+  const int me = C.D.me;
   const uint32_t n = C.D.n, W = C.D.W32;
   const uint32_t* __restrict__ xcol = C.A.xcol;
   const double* __restrict__ y = C.D.y;
   double* __restrict__ yhat = C.A.y_hat;
   const int L = R.beta_len;
-  double sxb = 0, sxb2 = 0, sy = 0, sy2 = 0, syx = 0;
+  double sxb = 0, sxb2 = 0, sy = 0, sy2 = 0, syx = 0, se = 0, se2 = 0;
   for (uint32_t w = threadIdx.x; w * 16 < n; w += BMG_BLOCK) {
     double acc[16];
 #pragma unroll
     for (int q = 0; q < 16; ++q) acc[q] = 0.0;
-    for (int c0 = 1; c0 < L; c0 += 8) {
-      uint32_t wd[8];
+    for (int c0 = me; c0 < L; c0 += 16) {
+      uint32_t wd[16];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
+      for (int u = 0; u < 16; ++u) {
         const int c = c0 + u;
         wd[u] = (c < L && S.beta[c] != 0.0) ? xcol[(size_t)S.slot[c] * W + w] : 0u;
       }
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
+      for (int u = 0; u < 16; ++u) {
         const int c = c0 + u;
         if (c < L) {
           const double t = S.beta[c];
@@ -1124,56 +1193,65 @@ __device__ __forceinline__ void xbeta_impl(Ctx<KMAX>& C, double* out5, int write
       if (i < n) {
         const double a = acc[q];
         sxb += a; sxb2 += a * a;
-        const double yh = R.nloci > 0 ? a + b0 : b0;
+        // E beta_E of this individual in column order (the constant first)
+        double e = b0;
+        for (int c = 1; c < me; ++c) e = e + C.D.E[(size_t)(c - 1) * C.D.n_pad + i] * S.beta[c];
+        se += e; se2 += e * e;
+        // with covariates and no loci the reference adds E beta_E to the
+        // y_hat it holds (src/model.hpp:369-373 leaves the genetic part
+        // unset), and so does this
+        const double yh = R.nloci > 0 ? a + e : (me > 1 && write_yhat ? yhat[i] + e : e);
         sy += yh; sy2 += yh * yh;
-        syx += (b0 - y[i]) * a;
+        syx += (e - y[i]) * a;
         if (write_yhat) yhat[i] = yh;
       }
     }
   }
-  out5[0] = block_sum(sxb, S.red);
-  out5[1] = block_sum(sxb2, S.red);
-  out5[2] = block_sum(sy, S.red);
-  out5[3] = block_sum(sy2, S.red);
-  out5[4] = block_sum(syx, S.red);
+  out7[0] = block_sum(sxb, S.red);
+  out7[1] = block_sum(sxb2, S.red);
+  out7[2] = block_sum(sy, S.red);
+  out7[3] = block_sum(sy2, S.red);
+  out7[4] = block_sum(syx, S.red);
+  out7[5] = block_sum(se, S.red);
+  out7[6] = block_sum(se2, S.red);
 }
 
-// y_hat = X_g beta_g (+ constant) into the chain's y_hat (write_yhat) and
-// out5: sum xb, sum xb^2, sum y_hat, sum y_hat^2, sum (beta_0 - y) xb, with
-// xb = X_g beta_g without the constant.  One packed word (16 individuals)
-// per lane: the columns' words are fetched eight at a time (coalesced across
-// lanes, several loads in flight) and each individual's sum is formed in
-// model-column order, as the per-individual loop does.
+// y_hat = X_g beta_g + E beta_E into the chain's y_hat (write_yhat) and
+// out7: sum xb, sum xb^2, sum y_hat, sum y_hat^2, sum (e - y) xb, sum e,
+// sum e^2, with xb = X_g beta_g and e = E beta_E (the constant and the
+// covariates).  One packed word (16 individuals) per lane: the columns'
+// words are fetched sixteen at a time (coalesced across lanes, all in
+// flight together) and each individual's sum is formed in model-column
+// order, as the per-individual loop does.
 template <int KMAX>
-__device__ __forceinline__ void xbeta(Ctx<KMAX>& C, double* out5, int write_yhat) {
+__device__ __forceinline__ void xbeta(Ctx<KMAX>& C, double* out7, int write_yhat) {
   // the sampler's columns share one coding unless its type is AH (the
   // Model API may mix them)
   if (!C.D.mixed && C.D.type != BMG_AH) {
     switch (C.D.type) {
-      case 0: xbeta_impl<KMAX, 0>(C, out5, write_yhat); return;
-      case 1: xbeta_impl<KMAX, 1>(C, out5, write_yhat); return;
-      case 2: xbeta_impl<KMAX, 2>(C, out5, write_yhat); return;
-      default: xbeta_impl<KMAX, 3>(C, out5, write_yhat); return;
+      case 0: xbeta_impl<KMAX, 0>(C, out7, write_yhat); return;
+      case 1: xbeta_impl<KMAX, 1>(C, out7, write_yhat); return;
+      case 2: xbeta_impl<KMAX, 2>(C, out7, write_yhat); return;
+      default: xbeta_impl<KMAX, 3>(C, out7, write_yhat); return;
     }
   }
-  xbeta_impl<KMAX, -1>(C, out5, write_yhat);
+  xbeta_impl<KMAX, -1>(C, out7, write_yhat);
 }
 
-// src/model.hpp:345-392 (no covariates: pves[2] = 0)
+// src/model.hpp:345-392: shares of var(E beta_E) (covariates beyond the
+// constant), var(X_g beta_g) and var(y_hat) in var(y_hat) + sigma2
 template <int KMAX>
 __device__ __forceinline__ void compute_pve(Ctx<KMAX>& C, double* pves) {
-  double s[5];
+  double s[7];
   xbeta(C, s, 1);
-  double n = (double)C.D.n;
-  if (C.R.nloci > 0) {
-    pves[1] = (s[1] - s[0] * s[0] / n) / (n - 1);
-    pves[2] = 0.0;
-    pves[0] = pves[1];
-    double z = pves[0] + C.R.sigma2;
-    pves[0] /= z; pves[1] /= z; pves[2] /= z;
-  } else {
-    pves[0] = pves[1] = pves[2] = 0.0;
-  }
+  const double n = (double)C.D.n;
+  const bool ce = C.D.me > 1, cg = C.R.nloci > 0;
+  pves[2] = ce ? (s[6] - s[5] * s[5] / n) / (n - 1) : 0.0;
+  pves[1] = cg ? (s[1] - s[0] * s[0] / n) / (n - 1) : 0.0;
+  if (!ce && !cg) { pves[0] = pves[1] = pves[2] = 0.0; return; }
+  pves[0] = !ce ? pves[1] : (!cg ? pves[2] : (s[3] - s[2] * s[2] / n) / (n - 1));
+  const double z = pves[0] + C.R.sigma2;
+  pves[0] /= z; pves[1] /= z; pves[2] /= z;
 }
 
 // src/prior.cpp:30-141 on the working model
@@ -1192,7 +1270,7 @@ __device__ __forceinline__ void sample_alpha_and_tau2(Ctx<KMAX>& C) {
     // round, and the next round restarts from the stream position it left.
     // The draws are the sequential loop's, value for value.
     uint64_t base = (uint64_t)R.rng.c0 | ((uint64_t)R.rng.c1 << 32);
-    int first = 1;
+    int first = C.D.me;   // the covariates' precisions are fixed
     while (first < R.beta_len) {
       const int i = first + (int)threadIdx.x;
       long long stop = R.beta_len;     // first term whose stream use differs
@@ -1236,7 +1314,7 @@ __device__ __forceinline__ void sample_alpha_and_tau2(Ctx<KMAX>& C) {
     R.rng.c0 = (uint32_t)base; R.rng.c1 = (uint32_t)(base >> 32);
   } else {
     double b2[4] = {0, 0, 0, 0}; double mt[4] = {0, 0, 0, 0};
-    for (int i = 1; i < R.beta_len; ++i) {
+    for (int i = C.D.me; i < R.beta_len; ++i) {
       const int xt = S.xtype[i];
       put(b2, xt, sel(b2, xt) + S.beta[i] * S.beta[i]);
       put(mt, xt, sel(mt, xt) + 1.0);
@@ -1249,17 +1327,17 @@ __device__ __forceinline__ void sample_alpha_and_tau2(Ctx<KMAX>& C) {
       R.ita2[t] = 1.0 / (R.alpha2 * rng_sinvchi2(R.rng, nu, s2));
     }
     __syncthreads();
-    for (int i = 1 + threadIdx.x; i < R.beta_len; i += BMG_BLOCK) S.d[i] = sel(R.ita2, (int)S.xtype[i]);
+    for (int i = C.D.me + threadIdx.x; i < R.beta_len; i += BMG_BLOCK) S.d[i] = sel(R.ita2, (int)S.xtype[i]);
   }
   R.mu_computed = 0;
   __syncthreads();
   // sample_alpha
-  int nterms = R.beta_len - 1;
+  int nterms = R.beta_len - C.D.me;
   double a = R.alpha;
   double xbxb = 0, ebxb = 0;
   if (nterms > 0) {
-    // x_b = X_g beta_g, e_b = beta_0 - y
-    double s[5];
+    // x_b = X_g beta_g, e_b = E beta_E - y
+    double s[7];
     xbeta(C, s, 0);
     ebxb = s[4];
     xbxb = s[1];
@@ -1282,11 +1360,11 @@ __device__ __forceinline__ void sample_alpha_and_tau2(Ctx<KMAX>& C) {
   R.alpha = a; R.alpha2 = a * a;
   __syncthreads();
   if (K.use_ind) {
-    for (int i = 1 + threadIdx.x; i < R.beta_len; i += BMG_BLOCK) S.d[i] = S.d[i] * old / R.alpha2;
+    for (int i = C.D.me + threadIdx.x; i < R.beta_len; i += BMG_BLOCK) S.d[i] = S.d[i] * old / R.alpha2;
   } else {
 #pragma unroll
     for (int t = 0; t < 4; ++t) if (K.allow_terms[t]) R.ita2[t] = R.ita2[t] * old / R.alpha2;
-    for (int i = 1 + threadIdx.x; i < R.beta_len; i += BMG_BLOCK) S.d[i] = sel(R.ita2, (int)S.xtype[i]);
+    for (int i = C.D.me + threadIdx.x; i < R.beta_len; i += BMG_BLOCK) S.d[i] = sel(R.ita2, (int)S.xtype[i]);
   }
   __syncthreads();
 }
@@ -1545,7 +1623,7 @@ template <int KMAX>
 __device__ __forceinline__ double exhaustive_modelset(Ctx<KMAX>& C, int cl, int n, double* lp) {
   Smem<KMAX>& S = C.S;
   Reg& R = C.R;
-  int c0 = 1;   // the intercept column
+  int c0 = C.D.me;   // the first column after the constant and the covariates
   if (cl > 0) c0 = (C.D.types[S.tind[cl - 1]] == 4 ? S.xi2[cl - 1] : S.xi1[cl - 1]) + 1;
   const int nT = R.cols - c0;
   for (int c = 0; c < nT; ++c)
@@ -2459,6 +2537,16 @@ __device__ __forceinline__ void run_iterations(Ctx<KMAX>& C, ChainScal& CS, uint
           C.A.th_ll[t] = R.c_ll; C.A.th_s2[t] = R.sigma2; C.A.th_alpha[t] = R.alpha;
           C.A.th_pve[3 * t] = pves[0]; C.A.th_pve[3 * t + 1] = pves[1]; C.A.th_pve[3 * t + 2] = pves[2];
           CS.th_count = t + 1; CS.th_loci_len = ll0 + R.c_nloci;
+          // save_beta (src/sampler.cpp:710-713): the beta drawn for this sample
+          if (K.th_beta_cap) {
+            const uint64_t b0 = CS.th_beta_len;
+            if (b0 + R.beta_len <= K.th_beta_cap) {
+              for (int q = 0; q < R.beta_len; ++q) C.A.th_beta[b0 + q] = S.beta[q];
+              CS.th_beta_len = b0 + R.beta_len;
+            } else {
+              R.err |= ERR_TRACE;
+            }
+          }
         } else {
           R.err |= ERR_TRACE;
         }

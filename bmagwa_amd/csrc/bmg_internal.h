@@ -26,6 +26,11 @@ struct bmg_ctx {
   double yy = 0, var_y = 0, var_x = 0, mean_x = 0;
   uint64_t n_missing = 0;
   std::vector<double> y_host;
+  // covariates (the reference's E without the constant): n x m_e host copy
+  // (column-major) and m_e x n_pad on the device, zero past n
+  uint32_t m_e = 0;
+  std::vector<double> e_host;
+  double* E = nullptr;
   hipStream_t stream = nullptr;
   // scan scratch (grown on demand)
   int8_t* dig = nullptr; size_t dig_bytes = 0;

@@ -73,8 +73,8 @@ typedef struct {
  * the 3-byte header) and y from the .fam/.y file; DataModel(data, types)
  * (src/data_model.hpp:52-89) and PrecomputedSNPCovariances(data_model)
  * (src/precomputed_snp_covariances.hpp:58-87) are built on the device.
- * Covariates (m_e > 0) are not supported by the engine (BMG_E_UNSUPPORTED);
- * the constant column is always present.  recode != 0 applies
+ * The constant column is always present; covariates are added with
+ * bmg_ctx_set_covariates.  recode != 0 applies
  * recode_g_to_minor_allele_count (src/data.cpp:324-339). */
 int bmg_ctx_create(int device, const uint8_t* bed_body, uint32_t n, uint64_t p,
                    const double* y, const int32_t* types, int n_types, int recode,
@@ -93,6 +93,12 @@ int bmg_ctx_create_synthetic(int device, uint32_t seed, uint32_t n, uint64_t p,
 int bmg_ctx_create_subset(const bmg_ctx* pool, const uint32_t* individuals, uint32_t n_sub,
                           const double* y, const int32_t* types, int n_types, bmg_ctx** out);
 int bmg_ctx_set_y(bmg_ctx* ctx, const double* y);
+/* The covariates of Data (src/data.hpp:45-60, read_e; the constant column is
+ * added by the engine as by the reference): e is n x m_e, column-major, m_e
+ * <= 32.  Chains and models on the dataset then carry m_e + 1 fixed columns
+ * with precisions inv_tau2_e_const_val / inv_tau2_e_val (src/model.hpp:
+ * 92-105, src/prior.hpp:185-190); bmg_opts.m_e must equal m_e. */
+int bmg_ctx_set_covariates(bmg_ctx* ctx, const double* e, uint32_t m_e);
 int bmg_ctx_destroy(bmg_ctx* ctx);
 /* packed body back to the host, reference layout (p * ceil(n/4) bytes) */
 int bmg_ctx_read_body(bmg_ctx* ctx, uint8_t* out);
@@ -249,7 +255,7 @@ int bmg_model_set_d(bmg_model* m, int x_ind, double val);
  * separate datasets).  opts[c] are the options of chain c (seed = its
  * reference seed).  Delayed rejection (src/sampler.cpp:882-1049) runs on
  * the device for min(delay_rejection, max_move_size) <= 20.  Unsupported on
- * the device: multiple effect types, covariates, missing genotypes. */
+ * the device: multiple effect types, missing genotypes. */
 typedef struct {
   int32_t trace_every_iter;    /* keep per-iteration move/jd/size/loglik */
   int32_t proposal;            /* 0 plain, 1 plain + learned swap (online) */
@@ -344,6 +350,12 @@ int bmg_batch_thinned(bmg_batch* b, int chain, uint32_t* sizes, int64_t cap_size
  * chain number. */
 #define BMG_MOVE_LEARNED 7
 int bmg_batch_write_outputs(bmg_batch* b, int chain, const char* basename);
+/* save_beta (src/sampler.cpp:584,710-713): the beta of every thinned sample
+ * of one chain, concatenated (the fixed columns first, then the loci's
+ * columns in model order); *len = the number of doubles kept.  With
+ * bmg_opts.save_beta set, bmg_batch_write_outputs also writes
+ * basename + "_beta.dat" in this layout. */
+int bmg_batch_thinned_beta(bmg_batch* b, int chain, double* out, int64_t cap, int64_t* len);
 /* RB scan of one chain's current state with a given y_hat (the drop-in for
  * RaoBlackwellizer::p_raoblackwell, src/sampler.cpp:32-261) */
 int bmg_batch_rb(bmg_batch* b, int chain, const double* y_hat, double* p_r);

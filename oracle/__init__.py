@@ -175,6 +175,7 @@ def load():
     _sig(lib, "orc_sampler_trace", C.c_int64, _P, _U8, _U8, _U8, _U32, _D)
     _sig(lib, "orc_sampler_thinned", C.c_int64, _P, _U32, _U32, C.c_int64, _D, _D, _D)
     _sig(lib, "orc_sampler_thinned_pve", C.c_int64, _P, _D)
+    _sig(lib, "orc_sampler_thinned_beta", C.c_int64, _P, _D, C.c_int64)
     _sig(lib, "orc_sampler_state", None, _P, _D)
     _sig(lib, "orc_sampler_loci", C.c_int, _P, _U32)
     _sig(lib, "orc_sampler_rb", None, _P, _D, _D)
@@ -452,8 +453,11 @@ class Sampler:
                                         nl, ptr(ll), ptr(s2), ptr(al))
         pve = np.zeros(3 * (n + 1))
         lib().orc_sampler_thinned_pve(self.h, ptr(pve))
+        nb = lib().orc_sampler_thinned_beta(self.h, None, 0)
+        beta = np.zeros(max(nb, 1))
+        lib().orc_sampler_thinned_beta(self.h, ptr(beta), nb)
         return dict(sizes=sizes[:cnt], loci=loci[:nl], loglik=ll[:cnt], sigma2=s2[:cnt],
-                    alpha=al[:cnt], pve=pve[:3 * cnt].reshape(cnt, 3))
+                    alpha=al[:cnt], pve=pve[:3 * cnt].reshape(cnt, 3), beta=beta[:nb])
 
     def state(self):
         o = np.zeros(8)

@@ -165,6 +165,9 @@ int64_t orc_sampler_thinned(const orc_sampler* s, uint32_t* sizes,
                             double* sigma2, double* alpha);
 /* PVE of every thinned sample (3 values each, src/model.hpp:345-392) */
 int64_t orc_sampler_thinned_pve(const orc_sampler* s, double* pve);
+/* save_beta: every thinned sample's beta, concatenated (src/sampler.cpp:
+ * 710-713); returns the number of doubles kept */
+int64_t orc_sampler_thinned_beta(const orc_sampler* s, double* out, int64_t cap);
 void   orc_sampler_state(const orc_sampler* s, double out[8]);
 int    orc_sampler_loci(const orc_sampler* s, uint32_t* loci);
 /* RB scan on the current model with a given y_hat, src/sampler.cpp:32-261 */

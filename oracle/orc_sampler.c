@@ -249,6 +249,7 @@ struct orc_sampler {
   uint8_t *tr_move, *tr_jd, *tr_ms; uint32_t* tr_size; double* tr_ll;
   int64_t th_len, th_cap, th_loci_len, th_loci_cap;
   uint32_t* th_size; uint32_t* th_loci; double *th_ll, *th_s2, *th_alpha, *th_pve;
+  double* th_beta; int64_t th_beta_len, th_beta_cap;   /* save_beta */
 };
 
 /* src/sampler.cpp:455-515 (PMV only) */
@@ -369,6 +370,7 @@ void orc_sampler_destroy(orc_sampler* s)
   free(s->dr_probs); free(s->dr_q_add); free(s->dr_q_rem); free(s->dr_lqt); free(s->dr_b2n);
   free(s->tr_move); free(s->tr_jd); free(s->tr_ms); free(s->tr_size); free(s->tr_ll);
   free(s->th_size); free(s->th_loci); free(s->th_ll); free(s->th_s2); free(s->th_alpha); free(s->th_pve);
+  free(s->th_beta);
   free(s);
 }
 
@@ -1367,6 +1369,16 @@ static void thinned_push(orc_sampler* s)
   s->th_s2[s->th_len] = s->cur->sigma2;
   s->th_alpha[s->th_len] = s->pr.alpha;
   memcpy(s->th_pve + 3 * s->th_len, s->pves, 3 * sizeof(double));
+  if (s->o.save_beta) {
+    /* src/sampler.cpp:710-713 */
+    int kb = s->cur->beta_len;
+    while (s->th_beta_len + kb > s->th_beta_cap) {
+      s->th_beta_cap = s->th_beta_cap ? 2 * s->th_beta_cap : 4096;
+      s->th_beta = (double*)realloc(s->th_beta, s->th_beta_cap * sizeof(double));
+    }
+    if (kb > 0) memcpy(s->th_beta + s->th_beta_len, s->cur->beta, sizeof(double) * kb);
+    s->th_beta_len += kb;
+  }
   s->th_len++;
 }
 
@@ -1526,6 +1538,12 @@ int64_t orc_sampler_thinned(const orc_sampler* s, uint32_t* sizes,
   if (sigma2) memcpy(sigma2, s->th_s2, s->th_len * sizeof(double));
   if (alpha) memcpy(alpha, s->th_alpha, s->th_len * sizeof(double));
   return s->th_len;
+}
+
+int64_t orc_sampler_thinned_beta(const orc_sampler* s, double* out, int64_t cap)
+{
+  if (out && s->th_beta_len) memcpy(out, s->th_beta, sizeof(double) * (size_t)(cap < s->th_beta_len ? cap : s->th_beta_len));
+  return s->th_beta_len;
 }
 
 int64_t orc_sampler_thinned_pve(const orc_sampler* s, double* pve)

@@ -380,3 +380,74 @@ def test_model_size_limit_matches_exact_enumeration(bmg, dr):
     assert np.max(np.abs(freq - exact)) < 0.03, (freq, exact)
     p_rao = b.p_rao(0)
     assert np.max(np.abs(p_rao - exact)) < 0.01, (p_rao, exact)
+
+
+@pytest.mark.parametrize("types,dr", [("A", 0), ("A", 10), ("AH", 4)])
+def test_covariates_trajectory_matches_oracle(bmg, types, dr, tmp_path):
+    """Two covariates beside the constant, the shape of the reference's own
+    example (testdata/testdata.ini: m_e = 2, delay_rejection = 10): the
+    chain's trajectory equals the oracle's, so do its RB PIPs, and the
+    thinned PVE carries the covariates' share (src/model.hpp:345-392)."""
+    n, p, n_iter = 600, 1500, 2000
+    body, y0, _ = dataset(n, p, (0.9, 0.9, 10, 30), 6, 61, types)
+    rng = np.random.default_rng(9)
+    E = np.column_stack([rng.normal(size=n), rng.integers(0, 2, size=n).astype(float)])
+    y = y0 + E @ np.array([0.8, -0.5])
+    cfg = synth.sampler_cfg(n, p, 6, types=types, seed=4, n_rao=100, n_rao_burnin=3, thin=10,
+                            do_n_iter=n_iter, delay_rejection=dr, m_e=2, inv_tau2_e_val=0.001)
+    d = oracle.Data(n, p, body, y, e_cov=E)
+    s = oracle.Sampler(cfg, d)
+    s.init_flat()
+    assert s.setup() == 0
+    s.iterate(n_iter)
+    ctx = bmg.Ctx(n, p, body, y, types=(types,), e_cov=E)
+    b = bmg.Batch([ctx], [0], [cfg], dict(trace_every_iter=1))
+    b.setup()
+    b.run(n_iter)
+    compare(b.trace(0), s.trace(), n_iter)
+    assert np.allclose(b.p_rao(0), s.p_rao(), rtol=1e-8, atol=1e-10)
+    base = str(tmp_path / "cov")
+    b.write_outputs(0, base)
+    pve = np.fromfile(base + "_pve.dat", dtype=np.float64).reshape(-1, 3)
+    po = s.thinned()["pve"]
+    assert pve.shape == po.shape
+    assert np.allclose(pve, po, rtol=1e-8, atol=1e-12)
+    assert (pve[:, 2] > 0).all()
+
+
+def test_covariates_need_matching_options(bmg):
+    """m_e of the options must be the dataset's number of covariates."""
+    n, p = 100, 200
+    body, y, _ = dataset(n, p, None, 2, 3)
+    ctx = bmg.Ctx(n, p, body, y, e_cov=np.random.default_rng(1).normal(size=(n, 2)))
+    cfg = synth.sampler_cfg(n, p, 2, seed=1, m_e=1)
+    with pytest.raises(bmg.BmgError, match="m_e"):
+        bmg.Batch([ctx], [0], [cfg])
+
+
+def test_save_beta_matches_oracle(bmg, tmp_path):
+    """save_beta = 1 (src/sampler.cpp:584,710-713): the beta of every
+    thinned sample, in _beta.dat and through bmg_batch_thinned_beta, equals
+    the oracle's (the constant and covariates first, then the loci)."""
+    n, p, n_iter = 400, 900, 1500
+    body, y0, _ = dataset(n, p, (0.9, 0.9, 10, 30), 5, 71, "AH")
+    E = np.random.default_rng(3).normal(size=(n, 1))
+    y = y0 + 0.4 * E[:, 0]
+    cfg = synth.sampler_cfg(n, p, 5, types="AH", seed=8, n_rao=100, n_rao_burnin=3, thin=10,
+                            do_n_iter=n_iter, save_beta=1, m_e=1, inv_tau2_e_val=0.01)
+    s = oracle.Sampler(cfg, oracle.Data(n, p, body, y, e_cov=E))
+    s.init_flat()
+    assert s.setup() == 0
+    s.iterate(n_iter)
+    ctx = bmg.Ctx(n, p, body, y, types=("AH",), e_cov=E)
+    b = bmg.Batch([ctx], [0], [cfg], dict(trace_every_iter=1))
+    b.setup()
+    b.run(n_iter)
+    compare(b.trace(0), s.trace(), n_iter)
+    want = s.thinned()["beta"]
+    got = b.thinned_beta(0)
+    assert got.shape == want.shape and len(got) > 2 * (n_iter // 10)
+    assert np.allclose(got, want, rtol=1e-9, atol=1e-12)
+    base = str(tmp_path / "sb")
+    b.write_outputs(0, base)
+    assert np.array_equal(np.fromfile(base + "_beta.dat", dtype=np.float64), got)
