@@ -912,6 +912,7 @@ static int batch_create(bmg_ctx** ctxs, int n_ctx, int n_chains, const int32_t* 
     Kc.n_plus_nu = x->n + o.nu_sigma2;
     Kc.minus_n_plus_nu_2 = -0.5 * Kc.n_plus_nu;
     Kc.nus2_plus_yy = o.nu_sigma2 * s2_sigma2 + x->yy;
+    Kc.nus2 = o.nu_sigma2 * s2_sigma2;
     Kc.mu_alpha = o.mu_alpha;
     double ga, gb;
     if (solve_beta(m_g, o.e_qg, o.var_qg, &ga, &gb)) { bmg_batch_destroy(b); set_error("invalid a or b of the model-size prior"); return BMG_E_ARG; }
@@ -1605,6 +1606,7 @@ extern "C" int bmg_model_create(bmg_ctx* ctx, const bmg_model_prior* prior, bmg_
   K.n_plus_nu = prior->n_plus_nu;
   K.minus_n_plus_nu_2 = prior->minus_n_plus_nu_2;
   K.nus2_plus_yy = prior->nus2_plus_yy;
+  K.nus2 = prior->nus2_plus_yy - ctx->yy;
   BMG_HIP(hipMemcpy(b->d_consts, &K, sizeof(ChainConst), hipMemcpyHostToDevice));
   bmg_model* m = new bmg_model();
   m->b = b;

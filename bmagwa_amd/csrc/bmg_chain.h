@@ -44,6 +44,7 @@ struct ChainConst {
   int32_t adapt_p_ms, adaptation, flat, trace_iter;
   double acpt_goal, q_add_min, q_rem_min, k_move_size;
   double n_plus_nu, minus_n_plus_nu_2, nus2_plus_yy, mu_alpha;
+  double nus2;           // nu_sigma2 * s2_sigma2 alone (the direct residual term)
   double g_a, g_b, m_g, tp_type, tp_sum;
   double nu_tau2[4], s2_tau2[4], nus2_tau2[4];
   int32_t allow_terms[4];

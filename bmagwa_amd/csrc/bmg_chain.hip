@@ -729,6 +729,36 @@ __device__ __forceinline__ void set_slot_bit(Ctx<KMAX>& C, int s, int on) {
   __syncthreads();
 }
 
+// The residual term S = nu s2 + y'y - v'v of the log-likelihood, formed the
+// reference's way, cancels when the factor is nearly singular (strongly
+// correlated columns: v'v comes close to y'y and rounding can leave S at
+// zero or below).  Where it has lost its significant digits (below 1e-8 of
+// nu s2 + y'y; a fit that close never occurs otherwise) it is formed
+// directly instead, as nu s2 + |y - X mu|^2 + mu'D mu with mu = U^-1 v,
+// a sum of non-negative terms equal to it in exact arithmetic.
+#define BMG_S_TINY 1e-8
+template <int KMAX>
+__device__ __noinline__ double direct_syx(Ctx<KMAX>& C) {
+  Smem<KMAX>& S = C.S;
+  const int k = C.R.cols;
+  __syncthreads();
+  for (int i = threadIdx.x; i < k; i += BMG_BLOCK) S.work[i] = S.v[i];
+  __syncthreads();
+  if (wave_id() == 0) trsv_un_w0(S, S.work, k);
+  __syncthreads();
+  double o[8];
+  xcoef(C, S.work, k, o, 0);
+  double md = 0.0;
+  for (int i = threadIdx.x; i < k; i += BMG_BLOCK) md += S.d[i] * S.work[i] * S.work[i];
+  md = block_sum(md, S.red);
+  return C.K.nus2 + o[7] + md;
+}
+
+template <int KMAX>
+__device__ __forceinline__ void guard_syx(Ctx<KMAX>& C) {
+  if (!(C.R.syx >= BMG_S_TINY * C.K.nus2_plus_yy)) C.R.syx = direct_syx(C);
+}
+
 // src/model.hpp:432-506 for one column (type A, H, D or R)
 template <int KMAX>
 __device__ __forceinline__ void update_on_add(Ctx<KMAX>& C, uint32_t ind, int model_ind, int coding,
@@ -801,6 +831,7 @@ __device__ __forceinline__ void update_on_add(Ctx<KMAX>& C, uint32_t ind, int mo
   if (threadIdx.x == 0) { S.L[P(col) + col] = lcc; S.v[col] = vc; }
   __syncthreads();
   R.syx = R.syx - vc * vc;
+  guard_syx(C);
   R.ldq += 0.5 * log(dval);
   R.ldl += log(lcc);
   R.ll = finite_ll(R.ldq - R.ldl + C.K.minus_n_plus_nu_2 * log(R.syx));
@@ -898,6 +929,7 @@ __device__ __forceinline__ void update_on_remove_impl(Ctx<KMAX>& C, int x_ind) {
     R.ldl = block_sum(ld, S.red);
     vv = block_sum(vv, S.red);
     R.syx = C.K.nus2_plus_yy - vv;
+    guard_syx(C);
   }
   R.ldq -= 0.5 * log(inv_val);
   R.ll = finite_ll(R.ldq - R.ldl + C.K.minus_n_plus_nu_2 * log(R.syx));
@@ -991,6 +1023,7 @@ __device__ __forceinline__ void compute_log_likelihood(Ctx<KMAX>& C) {
   R.ldl = block_sum(ld, S.red);
   vv = block_sum(vv, S.red);
   R.syx = C.K.nus2_plus_yy - vv;
+  guard_syx(C);
   R.ll = finite_ll(R.ldq - R.ldl + C.K.minus_n_plus_nu_2 * log(R.syx));
   R.mu_computed = 0;
 }
@@ -1142,17 +1175,17 @@ __device__ __forceinline__ double coded_times(uint32_t code, double t, double t2
 }
 
 template <int KMAX, int CODING>
-__device__ __forceinline__ void xbeta_impl(Ctx<KMAX>& C, double* out7, int write_yhat) {
+__device__ __forceinline__ void xbeta_impl(Ctx<KMAX>& C, const double* coef, int L, double* out8,
+                                           int write_yhat) {
   Smem<KMAX>& S = C.S;
   Reg& R = C.R;
-  const double b0 = S.beta[0];
+  const double b0 = coef[0];
   const int me = C.D.me;
   const uint32_t n = C.D.n, W = C.D.W32;
   const uint32_t* __restrict__ xcol = C.A.xcol;
   const double* __restrict__ y = C.D.y;
   double* __restrict__ yhat = C.A.y_hat;
-  const int L = R.beta_len;
-  double sxb = 0, sxb2 = 0, sy = 0, sy2 = 0, syx = 0, se = 0, se2 = 0;
+  double sxb = 0, sxb2 = 0, sy = 0, sy2 = 0, syx = 0, se = 0, se2 = 0, srr = 0;
   for (uint32_t w = threadIdx.x; w * 16 < n; w += BMG_BLOCK) {
     double acc[16];
 #pragma unroll
@@ -1162,13 +1195,13 @@ __device__ __forceinline__ void xbeta_impl(Ctx<KMAX>& C, double* out7, int write
 #pragma unroll
       for (int u = 0; u < 16; ++u) {
         const int c = c0 + u;
-        wd[u] = (c < L && S.beta[c] != 0.0) ? xcol[(size_t)S.slot[c] * W + w] : 0u;
+        wd[u] = (c < L && coef[c] != 0.0) ? xcol[(size_t)S.slot[c] * W + w] : 0u;
       }
 #pragma unroll
       for (int u = 0; u < 16; ++u) {
         const int c = c0 + u;
         if (c < L) {
-          const double t = S.beta[c];
+          const double t = coef[c];
           if (t != 0.0) {
             if (CODING >= 0) {
               const double t2 = t + t;
@@ -1195,7 +1228,7 @@ __device__ __forceinline__ void xbeta_impl(Ctx<KMAX>& C, double* out7, int write
         sxb += a; sxb2 += a * a;
         // E beta_E of this individual in column order (the constant first)
         double e = b0;
-        for (int c = 1; c < me; ++c) e = e + C.D.E[(size_t)(c - 1) * C.D.n_pad + i] * S.beta[c];
+        for (int c = 1; c < me; ++c) e = e + C.D.E[(size_t)(c - 1) * C.D.n_pad + i] * coef[c];
         se += e; se2 += e * e;
         // with covariates and no loci the reference adds E beta_E to the
         // y_hat it holds (src/model.hpp:369-373 leaves the genetic part
@@ -1203,17 +1236,20 @@ __device__ __forceinline__ void xbeta_impl(Ctx<KMAX>& C, double* out7, int write
         const double yh = R.nloci > 0 ? a + e : (me > 1 && write_yhat ? yhat[i] + e : e);
         sy += yh; sy2 += yh * yh;
         syx += (e - y[i]) * a;
+        const double ri = y[i] - (R.nloci > 0 ? a + e : e);
+        srr += ri * ri;
         if (write_yhat) yhat[i] = yh;
       }
     }
   }
-  out7[0] = block_sum(sxb, S.red);
-  out7[1] = block_sum(sxb2, S.red);
-  out7[2] = block_sum(sy, S.red);
-  out7[3] = block_sum(sy2, S.red);
-  out7[4] = block_sum(syx, S.red);
-  out7[5] = block_sum(se, S.red);
-  out7[6] = block_sum(se2, S.red);
+  out8[0] = block_sum(sxb, S.red);
+  out8[1] = block_sum(sxb2, S.red);
+  out8[2] = block_sum(sy, S.red);
+  out8[3] = block_sum(sy2, S.red);
+  out8[4] = block_sum(syx, S.red);
+  out8[5] = block_sum(se, S.red);
+  out8[6] = block_sum(se2, S.red);
+  out8[7] = block_sum(srr, S.red);
 }
 
 // y_hat = X_g beta_g + E beta_E into the chain's y_hat (write_yhat) and
@@ -1224,25 +1260,30 @@ __device__ __forceinline__ void xbeta_impl(Ctx<KMAX>& C, double* out7, int write
 // flight together) and each individual's sum is formed in model-column
 // order, as the per-individual loop does.
 template <int KMAX>
-__device__ __forceinline__ void xbeta(Ctx<KMAX>& C, double* out7, int write_yhat) {
+__device__ __forceinline__ void xcoef(Ctx<KMAX>& C, const double* coef, int L, double* out8, int write_yhat) {
   // the sampler's columns share one coding unless its type is AH (the
   // Model API may mix them)
   if (!C.D.mixed && C.D.type != BMG_AH) {
     switch (C.D.type) {
-      case 0: xbeta_impl<KMAX, 0>(C, out7, write_yhat); return;
-      case 1: xbeta_impl<KMAX, 1>(C, out7, write_yhat); return;
-      case 2: xbeta_impl<KMAX, 2>(C, out7, write_yhat); return;
-      default: xbeta_impl<KMAX, 3>(C, out7, write_yhat); return;
+      case 0: xbeta_impl<KMAX, 0>(C, coef, L, out8, write_yhat); return;
+      case 1: xbeta_impl<KMAX, 1>(C, coef, L, out8, write_yhat); return;
+      case 2: xbeta_impl<KMAX, 2>(C, coef, L, out8, write_yhat); return;
+      default: xbeta_impl<KMAX, 3>(C, coef, L, out8, write_yhat); return;
     }
   }
-  xbeta_impl<KMAX, -1>(C, out7, write_yhat);
+  xbeta_impl<KMAX, -1>(C, coef, L, out8, write_yhat);
+}
+
+template <int KMAX>
+__device__ __forceinline__ void xbeta(Ctx<KMAX>& C, double* out8, int write_yhat) {
+  xcoef(C, C.S.beta, C.R.beta_len, out8, write_yhat);
 }
 
 // src/model.hpp:345-392: shares of var(E beta_E) (covariates beyond the
 // constant), var(X_g beta_g) and var(y_hat) in var(y_hat) + sigma2
 template <int KMAX>
 __device__ __forceinline__ void compute_pve(Ctx<KMAX>& C, double* pves) {
-  double s[7];
+  double s[8];
   xbeta(C, s, 1);
   const double n = (double)C.D.n;
   const bool ce = C.D.me > 1, cg = C.R.nloci > 0;
@@ -1337,7 +1378,7 @@ __device__ __forceinline__ void sample_alpha_and_tau2(Ctx<KMAX>& C) {
   double xbxb = 0, ebxb = 0;
   if (nterms > 0) {
     // x_b = X_g beta_g, e_b = E beta_E - y
-    double s[7];
+    double s[8];
     xbeta(C, s, 0);
     ebxb = s[4];
     xbxb = s[1];

@@ -6,7 +6,9 @@
 // (r < 4).  Wave w owns hidden columns [64 w, 64 w + 64).  Activations live in
 // LDS as bf16, in both row-major and transposed form where a later product
 // reduces over rows; weights come from L2 as bf16 copies whose reduction
-// index is contiguous.  Gradients are accumulated with fp32 atomics.
+// index is contiguous.  Each workgroup writes its fp32 gradient partials,
+// and k_mlp_reduce sums them in workgroup order (the same gradient for the
+// same batch, run after run).
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <string.h>

@@ -40,6 +40,7 @@ void orc_swap_colrow(double* a, int ld, int m, int colrow);
 /* src/prior.hpp:38-291 */
 typedef struct {
   double n_plus_nu, minus_n_plus_nu_2, nus2_plus_yy, mu_alpha;
+  double nus2;   /* nu_sigma2 * s2_sigma2 alone */
   int use_individual_tau2;
   size_t n, m_g, m_e;
   int n_types;

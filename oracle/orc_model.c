@@ -42,6 +42,7 @@ int orc__prior_init(orc_prior* p, const orc_data* d, const orc_datamodel* dm,
   p->n_plus_nu = p->n + nu_sigma2;
   p->minus_n_plus_nu_2 = -0.5 * p->n_plus_nu;
   p->nus2_plus_yy = nu_sigma2 * s2_sigma2 + st[0];
+  p->nus2 = nu_sigma2 * s2_sigma2;
   p->mu_alpha = mu_alpha;
   p->use_individual_tau2 = use_ind;
   p->n_types = orc__dm_n_types(dm);
@@ -247,6 +248,35 @@ static double prepare_l_chol(orc_model* m)
   return 0.5 * ls2;
 }
 
+/* The residual term S = nu s2 + y'y - v'v (src/model.hpp:226,498,545) cancels
+ * when the factor is nearly singular; where it has lost its significant
+ * digits (below 1e-8 of nu s2 + y'y) it is formed directly, as
+ * nu s2 + |y - X mu|^2 + mu'D mu with mu = U^-1 v (equal in exact
+ * arithmetic, a sum of non-negative terms).  The reference keeps the
+ * cancelled value; a fit that close does not occur away from that case. */
+#define ORC_S_TINY 1e-8
+static void guard_syx(orc_model* m)
+{
+  if (m->syx_plus_vs2 >= ORC_S_TINY * m->pr->nus2_plus_yy) return;
+  int k = m->cols, ld = m->kmax;
+  size_t n = m->n;
+  double* mu = (double*)malloc(sizeof(double) * (size_t)k);
+  memcpy(mu, m->v, sizeof(double) * (size_t)k);
+  orc_trsv_un(m->l, ld, k, mu);
+  const double* y = orc__data_y(m->d);
+  double rss = 0.0;
+  for (size_t i = 0; i < n; ++i) {
+    double yh = 0.0;
+    for (int c = 0; c < k; ++c) yh += m->x[(size_t)c * n + i] * mu[c];
+    double r = y[i] - yh;
+    rss += r * r;
+  }
+  double md = 0.0;
+  for (int c = 0; c < k; ++c) md += m->dv[c] * mu[c] * mu[c];
+  free(mu);
+  m->syx_plus_vs2 = m->pr->nus2 + rss + md;
+}
+
 /* src/model.hpp:199-237 */
 void orc_model_compute_log_likelihood(orc_model* m)
 {
@@ -260,6 +290,7 @@ void orc_model_compute_log_likelihood(orc_model* m)
   memcpy(m->v, m->xy, sizeof(double) * k);
   orc_trsv_ut(m->l, ld, k, m->v);
   m->syx_plus_vs2 = m->pr->nus2_plus_yy - dot(m->v, m->v, k);
+  guard_syx(m);
   m->log_det_invQ_plus_xx = 0;
   for (int i = 0; i < k; ++i) m->log_det_invQ_plus_xx += log(E(m->l, ld, i, i));
   m->loglik = orc__finite_ll(m->log_det_invQ - m->log_det_invQ_plus_xx
@@ -295,6 +326,7 @@ static void update_on_add(orc_model* m, uint32_t ind, int model_ind,
     m->v[col] = vm2 / lcol[col];
     vm2 = m->v[col] * m->v[col];
     m->syx_plus_vs2 = m->syx_plus_vs2 - vm2;
+    guard_syx(m);
     m->log_det_invQ += 0.5 * log(dval);
     m->log_det_invQ_plus_xx += log(lcol[col]);
     m->loglik = orc__finite_ll(m->log_det_invQ - m->log_det_invQ_plus_xx
@@ -354,6 +386,7 @@ static void update_on_remove(orc_model* m, int x_ind, int upd)
       for (int i = 0; i < m->cols; ++i)
         m->log_det_invQ_plus_xx += log(E(m->l, ld, i, i));
       m->syx_plus_vs2 = m->pr->nus2_plus_yy - dot(m->v, m->v, m->cols);
+      guard_syx(m);
     }
     m->log_det_invQ -= 0.5 * log(inv_val);
     m->loglik = orc__finite_ll(m->log_det_invQ - m->log_det_invQ_plus_xx

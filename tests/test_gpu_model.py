@@ -261,3 +261,38 @@ def test_cpp_caller_matches_oracle(bmg, tmp_path):
         i += 1
     assert len(got) == len(OPS)
     _compare_ops(got, want)
+
+
+def test_residual_term_in_the_cancelling_regime(bmg):
+    """The device model forms S directly where nu s2 + y'y - v'v has
+    cancelled (two identical columns, an exact fit, prior precision 1e-14):
+    it matches the regularised least-squares residual of the augmented
+    system and the oracle's guarded value, and the log-likelihood is
+    finite."""
+    rng = np.random.default_rng(17)
+    n, p = 50, 20
+    G = rng.integers(0, 3, size=(p, n))
+    G[4] = G[3]
+    body = pack_genotypes(G)
+    snps = (3, 4, 7, 11)
+    X = np.column_stack([np.ones(n)] + [G[j].astype(float) for j in snps])
+    y = X @ np.array([3.0, 7.0, 7.0, -11.0, 5.0])
+    from bmagwa_amd import synth
+    cfg = synth.sampler_cfg(n, p, 2, seed=3, nu_sigma2=0, s2_sigma2=1, R2mode_sigma2=0.0)
+    dv = 1e-14
+    ctx = bmg.Ctx(n, p, body, y)
+    m = bmg.Model(ctx, cfg=cfg)
+    s = oracle.Sampler(cfg, oracle.Data(n, p, body, y))
+    om = s.current()
+    om.compute_log_likelihood()
+    for j in snps:
+        m.add_term(j, 0, [dv, dv])
+        om.add_term(j, 0, [dv, dv])
+    A = np.vstack([X, np.sqrt(np.diag([0.0] + [dv] * len(snps)))])
+    b = np.concatenate([y, np.zeros(len(snps) + 1)])
+    mu, *_ = np.linalg.lstsq(A, b, rcond=None)
+    s_true = float(np.sum((A @ mu - b) ** 2))
+    g = m.get()
+    assert g["syx"] == pytest.approx(s_true, rel=1e-3)
+    assert g["syx"] == pytest.approx(om.get()["syx"], rel=1e-6)
+    assert np.isfinite(m.log_likelihood())

@@ -3,10 +3,10 @@ unchanged: its Metropolis-Hastings ratio carries the forward and reverse
 partner-draw probabilities.  Checked against exact enumeration of all 2^7
 models with tau2/alpha held fixed, with deliberately skewed weights.
 
-The plain swap and state-change moves of the reference are left out of these
-runs (move mix 1/0/0): the swap's walk over free neighbours is not exactly
-reversible, which biases even the plain sampler by ~0.015 here, and would hide
-the learned move's own error.  With the Hastings term of the learned move
+The runs use add/remove moves and swaps half and half, every swap a learned
+swap, and no state changes (move mix 0.5/0.5/0): the plain swap's walk over
+free neighbours is not exactly reversible, which biases even the plain
+sampler by ~0.015 here, and would hide the learned move's own error.  With the Hastings term of the learned move
 dropped, the same runs miss the exact PIPs by 0.08-0.11 (checked once by
 editing the oracle), far outside the 0.02 tolerance."""
 import numpy as np

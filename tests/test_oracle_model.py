@@ -264,3 +264,51 @@ def test_exh_proposals_equal_brute_force_sequential_draws():
                 i = moved[rem_pos[ri]]; ri += 1
                 lpp += math.log(q_rem[i] / zr); zr -= q_rem[i]; mrem -= 1
         assert lp[pattern] == pytest.approx(lpp, abs=1e-12)
+
+
+def test_residual_term_in_the_cancelling_regime():
+    """Two identical SNP columns and a phenotype that they and two others fit
+    exactly, with a tiny prior precision: S = nu s2 + y'y - v'v cancels to
+    noise (the reference's formula, src/model.hpp:226,498), and the model
+    forms S directly instead; it matches the regularised least-squares
+    residual |y - X mu|^2 + mu'D mu of the augmented system [X; D^1/2],
+    solved by SVD in numpy, and the log-likelihood stays finite."""
+    rng = np.random.default_rng(17)
+    n, p = 50, 20
+    G = rng.integers(0, 3, size=(p, n))
+    G[4] = G[3]                                     # perfect LD
+    from fixtures import pack_genotypes
+    body = pack_genotypes(G)
+    snps = (3, 4, 7, 11)
+    X = np.column_stack([np.ones(n)] + [G[j].astype(float) for j in snps])
+    y = X @ np.array([3.0, 7.0, 7.0, -11.0, 5.0])   # no noise: an exact fit
+    cfg = dict(MODEL_KAT_CFG, n=n, m_g=p, types="A", nu_sigma2=0, s2_sigma2=1)
+    s, d = sampler_on(cfg, body, y)
+    m = s.current()
+    m.compute_log_likelihood()
+    dv = 1e-14
+    for j in snps:
+        m.add_term(j, 0, [dv, dv])
+    st = m.get()
+    D = np.diag([0.0] + [dv] * len(snps))
+    A = np.vstack([X, np.sqrt(D)])
+    b = np.concatenate([y, np.zeros(len(snps) + 1)])
+    mu, *_ = np.linalg.lstsq(A, b, rcond=None)
+    s_true = float(np.sum((A @ mu - b) ** 2))
+    assert s_true > 0
+    naive = s.prior()["nus2_plus_yy"] - st["v"] @ st["v"]
+    assert abs(naive - s_true) > 0.1 * s_true       # the cancelled value is noise
+    assert st["syx"] == pytest.approx(s_true, rel=1e-3)
+    assert np.isfinite(m.log_likelihood())
+    # removing an inner column (the dchex path) and the full refactorisation
+    # keep it too
+    m.remove_term(1)
+    X2 = np.delete(X, 2, axis=1)
+    A2 = np.vstack([X2, np.sqrt(np.diag([0.0] + [dv] * 3))])
+    b2 = np.concatenate([y, np.zeros(4)])
+    mu2, *_ = np.linalg.lstsq(A2, b2, rcond=None)
+    s2_true = float(np.sum((A2 @ mu2 - b2) ** 2))
+    assert m.get()["syx"] == pytest.approx(s2_true, rel=1e-3)
+    m.compute_log_likelihood()
+    assert m.get()["syx"] == pytest.approx(s2_true, rel=1e-3)
+    assert np.isfinite(m.log_likelihood())
