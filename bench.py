@@ -115,7 +115,8 @@ def run_batch(bmg, synth, ctx, n_chains, steps, warmup, ips, seed0, batch_cfg=No
     b.sync()
     t_burnin = time.perf_counter() - t0
     first = [len(b.thinned(c)["sizes"]) for c in range(n_chains)]
-    learn = b.learn_state(0) if (batch_cfg or {}).get("proposal") == 1 else None
+    prop = (batch_cfg or {}).get("proposal", 0)
+    learn = b.learn_state(0) if prop else None
     tm0 = b.timing()
     b.sync()
     t0 = time.perf_counter()
@@ -139,7 +140,10 @@ def run_batch(bmg, synth, ctx, n_chains, steps, warmup, ips, seed0, batch_cfg=No
             if n_chains >= 2 else None)
     del p_rao_chains
     acc = [b.state(c)["n_accepted"] / max(b.state(c)["n_iter"], 1) for c in range(n_chains)]
-    if learn is not None:
+    if learn is not None and not prop & 1:
+        learn = dict(rounds=learn["rounds"], frozen=learn["frozen"],
+                     table="per-SNP network (features: marginal score, RB running means, LD to the model)")
+    elif learn is not None:
         tr = b.trace(0)
         mv, jd = tr["move"], tr["jd"]
         k0 = len(mv) - steps * ips      # first timed iteration
@@ -455,6 +459,11 @@ def main():
                                             100, dict(proposal=1, trace_every_iter=1), th)
     runs["batched"] = run_batch(bmg, synth, ctx, args.chains, args.steps, args.warmup, ips, 100,
                                 dict(proposal=0), th)
+    if args.learned:
+        # the per-SNP learned proposal: add/remove weights from the network's
+        # table, set at the end of burn-in (same seeds as the batched run)
+        runs["learned_proposal"] = run_batch(bmg, synth, ctx, args.chains, args.steps, args.warmup, ips, 100,
+                                             dict(proposal=2), th)
     if not args.no_plain:
         runs["plain"] = run_batch(bmg, synth, ctx, 1, args.plain_steps, args.warmup, ips, 100,
                                   dict(proposal=0), th)

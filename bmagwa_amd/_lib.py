@@ -185,6 +185,7 @@ SIGNATURES = {
     "bmg_batch_loci": (C.c_int, _P, C.c_int, _U32, C.c_int, C.POINTER(C.c_int)),
     "bmg_batch_p_rao": (C.c_int, _P, C.c_int, _D),
     "bmg_batch_p_proposal": (C.c_int, _P, C.c_int, _D),
+    "bmg_batch_set_p_proposal": (C.c_int, _P, C.c_int, _D),
     "bmg_batch_trace": (C.c_int, _P, C.c_int, _U8, _U8, _U8, _U32, _D, C.c_int64, _I64),
     "bmg_batch_write_outputs": (C.c_int, _P, C.c_int, C.c_char_p),
     "bmg_batch_thinned_beta": (C.c_int, _P, C.c_int, _D, C.c_int64, _I64),
@@ -671,6 +672,14 @@ class Batch:
         out = np.zeros(self.p[c])
         check(lib().bmg_batch_p_proposal(self.h, c, _p(out)))
         return out
+
+    def set_p_proposal(self, c, p):
+        """Replace chain c's add/remove proposal source (q_add, q_rem and the
+        proposal trees follow)."""
+        a = np.ascontiguousarray(p, dtype=np.float64)
+        if a.shape != (self.p[c],):
+            raise ValueError("proposal table must have one value per SNP")
+        check(lib().bmg_batch_set_p_proposal(self.h, c, _p(a)))
 
     def trace(self, c, cap=10 ** 7):
         st = self.state(c)

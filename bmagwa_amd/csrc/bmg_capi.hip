@@ -609,6 +609,7 @@ static int rb_block_ds(bmg_batch* b, int ds, const std::vector<int>& chains, int
     if (learn_now) {
       rc = learn_labels(b->learn[ds], b->d_data, b->d_consts, b->d_scal, b->d_arrays, b->d_list, nv, BMG_KMAXG,
                         b->xa, b->rsum, b->stream);
+      if (!rc) rc = learn_snp_labels(b->learn[ds], b->d_scal, b->d_arrays, b->d_list, nv, BMG_KMAXG, b->stream);
       if (rc) return rc;
     }
   }
@@ -641,10 +642,19 @@ static int learn_after_block(bmg_batch* b, int ds, const std::vector<int>& all,
   for (int c = 0; c < b->n_chains; ++c) if (b->chain_ds[c] == ds) chains.push_back(c);
   int rc = learn_round(L, b->d_arrays, chains.data(), (int)chains.size(), b->stream);
   if (rc) return rc;
-  if (learn_rounds(L) > 0 && !b->learn_on[ds]) {
+  if ((b->cfg.proposal & 1) && learn_rounds(L) > 0 && !b->learn_on[ds]) {
     rc = set_learn_on(b, ds, 1, b->cfg.p_learned);
     if (rc) return rc;
     b->learn_on[ds] = 1;
+  }
+  if (freeze && (b->cfg.proposal & 2)) {
+    // the per-SNP network's tables replace the chains' proposals, after
+    // this block's adaptation step (the last of burn-in at the default
+    // freeze point)
+    const DevData& D = b->h_data[ds];
+    rc = learn_snp_apply(L, b->d_data, b->d_consts, b->d_scal, b->d_arrays, chains.data(), (int)chains.size(),
+                         D.lev_len, D.top, b->stream);
+    if (rc) return rc;
   }
   if (freeze) learn_freeze(L);
   BMG_HIP(hipStreamSynchronize(b->stream));
@@ -982,10 +992,15 @@ static int batch_create(bmg_ctx** ctxs, int n_ctx, int n_chains, const int32_t* 
     BMG_HIP(hipMemcpyAsync(A.adapt, adapt.data(), sizeof(double) * 6 * 256, hipMemcpyHostToDevice, b->stream));
     BMG_HIP(hipStreamSynchronize(b->stream));
   }
-  // online learned proposal: one learner per dataset that has chains
+  // online learned proposals: one learner per dataset that has chains
   b->learn.assign(n_ctx, nullptr);
   b->learn_on.assign(n_ctx, 0);
-  if (b->cfg.proposal == 1) {
+  if (b->cfg.proposal & ~3) {
+    bmg_batch_destroy(b);
+    set_error("proposal: 0 plain, 1 learned swap, 2 learned per-SNP proposal, 3 both");
+    return BMG_E_ARG;
+  }
+  if (b->cfg.proposal != 0) {
     bmg_batch_cfg& bc = b->cfg;
     if (bc.learn_window <= 0) bc.learn_window = 20;
     if (bc.learn_steps <= 0) bc.learn_steps = 100;
@@ -1001,14 +1016,12 @@ static int batch_create(bmg_ctx** ctxs, int n_ctx, int n_chains, const int32_t* 
       for (int c = 0; c < n_chains; ++c) nc += chain_ctx[c] == ds;
       if (!nc) continue;
       LearnCfg lc{bc.learn_window, bc.learn_steps, bc.learn_batch, (float)bc.learn_rate,
-                  opts[0].seed * 7919u + (uint32_t)ds};
+                  opts[0].seed * 7919u + (uint32_t)ds, bc.proposal & 1, (bc.proposal & 2) ? 1 : 0};
       int rc = learn_create(&b->learn[ds], ctxs[ds], b->h_data[ds], lc, nc, b->stream);
       if (rc) { bmg_batch_destroy(b); return rc; }
       b->h_data[ds].lw = learn_table(b->learn[ds]);
       b->h_data[ds].lH = bc.learn_window;
     }
-  } else if (b->cfg.proposal != 0) {
-    bmg_batch_destroy(b); set_error("proposal must be 0 (plain) or 1 (learned)"); return BMG_E_ARG;
   }
   BMG_HIP(hipMalloc(&b->d_data, sizeof(DevData) * n_ctx));
   BMG_HIP(hipMalloc(&b->d_consts, sizeof(ChainConst) * n_chains));
@@ -1270,7 +1283,7 @@ extern "C" int bmg_batch_set_learned(bmg_batch* b, int ds, const float* w, int H
     set_error("learned swap: 1 <= H <= 32 and 0 <= p_learned <= 1"); return BMG_E_ARG;
   }
   if (ds < (int)b->learn.size() && b->learn[ds]) {
-    set_error("this dataset's table is learned online (proposal = 1)"); return BMG_E_STATE;
+    set_error("this dataset's proposals are learned online (proposal != 0)"); return BMG_E_STATE;
   }
   BMG_HIP(hipSetDevice(b->device));
   BMG_HIP(hipStreamSynchronize(b->stream));
@@ -1290,7 +1303,7 @@ extern "C" int bmg_batch_set_learned(bmg_batch* b, int ds, const float* w, int H
 
 static LearnDS* learner(bmg_batch* b, int ds) {
   if (!b || ds < 0 || ds >= (int)b->learn.size() || !b->learn[ds]) {
-    set_error("no learned proposal for this dataset (proposal = 1 at create)");
+    set_error("no learned proposal for this dataset (proposal != 0 at create)");
     return nullptr;
   }
   hipSetDevice(b->device);
@@ -1416,6 +1429,34 @@ extern "C" int bmg_batch_p_rao(bmg_batch* b, int chain, double* out) {
 extern "C" int bmg_batch_p_proposal(bmg_batch* b, int chain, double* out) {
   if (!b || chain < 0 || chain >= b->n_chains) { set_error("bad chain"); return BMG_E_ARG; }
   return copy_p(b, chain, b->h_arrays[chain].p_proposal, out);
+}
+
+extern "C" int bmg_batch_set_p_proposal(bmg_batch* b, int chain, const double* p) {
+  if (!b || chain < 0 || chain >= b->n_chains || !p) { set_error("bad argument"); return BMG_E_ARG; }
+  if (!b->setup_done) { set_error("call bmg_batch_setup first"); return BMG_E_STATE; }
+  const int ds = b->chain_ds[chain];
+  const uint64_t np = b->h_data[ds].p;
+  for (uint64_t j = 0; j < np; ++j)
+    if (!(p[j] >= 0.0 && p[j] <= 1.0)) { set_error("proposal table: values must lie in [0, 1]"); return BMG_E_ARG; }
+  hipSetDevice(b->device);
+  BMG_HIP(hipStreamSynchronize(b->stream));
+  double* d_p = nullptr;
+  BMG_HIP(hipMalloc(&d_p, sizeof(double) * np));
+  int rc = BMG_OK;
+  if (hipMemcpy(d_p, p, sizeof(double) * np, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(b->d_list, &chain, sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
+    set_error("proposal table: copy failed"); rc = BMG_E_HIP;
+  }
+  if (!rc) {
+    if (launch_set_proposal(b->d_data, b->d_consts, b->d_arrays, chain, d_p, np, b->stream) ||
+        launch_tree_build(b->d_data, b->d_consts, b->d_scal, b->d_arrays, b->d_list, 1, b->h_data[ds].lev_len,
+                          b->h_data[ds].top, 0, 0, b->stream) ||
+        hipStreamSynchronize(b->stream) != hipSuccess) {
+      set_error("proposal table: device update failed"); rc = BMG_E_HIP;
+    }
+  }
+  hipFree(d_p);
+  return rc;
 }
 
 extern "C" int bmg_batch_trace(bmg_batch* b, int chain, uint8_t* move, uint8_t* jd, uint8_t* ms,

@@ -580,10 +580,12 @@ __device__ __forceinline__ void chol_downdate(Smem<KMAX>& S, int m, int cr) {
 // Full factorisation of S.L (m x m, in place), right-looking; returns 0 if
 // not positive definite.  Step k: every thread takes the pivot, the thread
 // of column i > k scales its row-k entry (kept in S.work for the others),
-// then walks its own column down from row k + 1 subtracting work[j] * u_ki.
-// Per element the subtractions come in k order, each a multiply and a
-// subtract, as in the sequential loop; two barriers a step, no index
-// arithmetic beyond the packed offsets.
+// then walks its own column down from row k + 1 subtracting work[j] * u_ki,
+// eight rows at a time: the chunk's sixteen LDS loads are issued together
+// before its eight stores (a row-at-a-time loop waits out the LDS latency
+// on every row, since the stores may alias the next loads).  Per element
+// the subtractions come in k order, each a multiply and a subtract, as in
+// the sequential loop; two barriers a step.
 template <int KMAX>
 __device__ __forceinline__ int chol_full(Smem<KMAX>& S, int m) {
   __syncthreads();
@@ -602,7 +604,18 @@ __device__ __forceinline__ int chol_full(Smem<KMAX>& S, int m) {
     if (threadIdx.x == 0) S.L[P(k) + k] = dk;
     if (i < m) {
       double* col = S.L + P(i);
-      for (int j = k + 1; j <= i; ++j) col[j] = col[j] - S.work[j] * u;
+      for (int j0 = k + 1; j0 <= i; j0 += 8) {
+        double c[8], w[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const int j = j0 + q <= i ? j0 + q : i;   // past i: row i again, not stored
+          c[q] = col[j];
+          w[q] = S.work[j];
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+          if (j0 + q <= i) col[j0 + q] = c[q] - w[q] * u;
+      }
     }
     __syncthreads();
   }

@@ -19,6 +19,9 @@ struct LearnCfg {
   int batch;          // rows per Adam step
   float lr;           // Adam learning rate
   uint32_t seed;      // minibatch sampling and initial weights
+  int swap;           // train and use the learned swap's partner table
+  int snp;            // train the per-SNP network and, at the freeze, set
+                      // each chain's add/remove proposal table from it
 };
 
 // band X^T X (window H + 1), marginal scores and buffers for dataset D
@@ -45,6 +48,21 @@ int learn_labels(LearnDS* L, const DevData* d_data, const ChainConst* consts, co
 // w[j][c] = exp(clamp(net(features(j, j + off(c))), -30, 30) / 2) for every
 // pair (0 where the partner is outside the genome).
 int learn_round(LearnDS* L, const ChainArrays* d_arrays, const int* chains_host, int n, hipStream_t s);
+
+// Per-SNP proposal samples from one RB group's scan (after the RB
+// probabilities, before the p_proposal update): for every chain in
+// list[0..nv), the SNPs within the window of each model SNP and LEARN_SRAND
+// uniformly drawn SNPs, each with its features in the chain's state and the
+// scan's RB probability (clipped logit) as the label.
+int learn_snp_labels(LearnDS* L, const ChainScal* scal, const ChainArrays* arrays, const int* d_list, int nv,
+                     int kmax, hipStream_t s);
+// At the freeze: each chain's table p_j = logistic(net(features of j in the
+// chain's state)) becomes its p_proposal (q_add, q_rem and trees follow);
+// with adaptation over it stays fixed for the rest of the run.
+int learn_snp_apply(LearnDS* L, const DevData* d_data, const ChainConst* d_consts, ChainScal* d_scal,
+                    const ChainArrays* d_arrays, const int* chains_host, int n, const uint64_t* lev_len, int top,
+                    hipStream_t s);
+int learn_snp_applied(const LearnDS* L);
 
 // device-side features of pairs (j, offset) for checking: out rows x 16
 int learn_features(LearnDS* L, const uint32_t* j_host, const int32_t* off_host, int64_t rows, float* out,

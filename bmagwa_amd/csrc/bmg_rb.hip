@@ -2,6 +2,8 @@
 // (src/sampler.cpp:32-261) and the proposal updates that follow it
 // (src/sampler.cpp:739-810), plus the sum-tree (re)build.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 #include <math.h>
 
 #include "bmg_chain.h"
@@ -129,6 +131,27 @@ __global__ void k_rb_update(const DevData* __restrict__ data, const ChainConst* 
   }
 }
 
+// a proposal table from outside for chain c: p_proposal <- p, q_add and
+// q_rem from it (src/sampler.cpp:517-524, 797-806) and the leaves of the
+// sum trees (a zeroed leaf stays 0: every weight is at least q_add_min or
+// q_rem_min, both positive); the upper levels are rebuilt after it
+__global__ void k_set_proposal(const DevData* __restrict__ data, const ChainConst* __restrict__ consts,
+                               const ChainArrays* __restrict__ arrays, int c, const double* __restrict__ p) {
+  const ChainConst& K = consts[c];
+  const DevData& D = data[K.ds];
+  const ChainArrays& A = arrays[c];
+  double* la = A.tree_add + D.lev_off[0];
+  double* lr = A.tree_rem + D.lev_off[0];
+  for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < D.p;
+       j += (uint64_t)gridDim.x * blockDim.x) {
+    const double pp = p[j];
+    const double qa = fmax(pp, K.q_add_min), qr = fmax(1 - pp, K.q_rem_min);
+    A.p_proposal[j] = pp; A.q_add[j] = qa; A.q_rem[j] = qr;
+    if (la[j] != 0.0) la[j] = qa;
+    if (lr[j] != 0.0) lr[j] = qr;
+  }
+}
+
 // one level of the sum trees: entry g of level h = group sum of group g of
 // level h - 1 (inclusive wave scan, last lane); one wave per group
 __global__ void k_tree_level(const DevData* __restrict__ data, const ChainConst* __restrict__ consts,
@@ -207,6 +230,13 @@ int launch_rb_update(const DevData* data, const ChainConst* consts, const ChainS
                      const ChainArrays* arrays, const int* list, int nv, uint64_t p, hipStream_t s) {
   hipLaunchKernelGGL(k_rb_update, dim3((unsigned)((p + 255) / 256), nv), dim3(256), 0, s, data, consts,
                      scal, arrays, list);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_set_proposal(const DevData* data, const ChainConst* consts, const ChainArrays* arrays, int chain,
+                        const double* p, uint64_t np, hipStream_t s) {
+  hipLaunchKernelGGL(k_set_proposal, dim3((unsigned)std::min<uint64_t>((np + 255) / 256, 4096)), dim3(256), 0, s,
+                     data, consts, arrays, chain, p);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

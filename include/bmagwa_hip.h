@@ -328,6 +328,14 @@ int bmg_batch_loci(bmg_batch* b, int chain, uint32_t* loci, int cap, int* k);
 /* p_rao (RB PIP estimate, src/sampler.cpp:739-757) and p_proposal */
 int bmg_batch_p_rao(bmg_batch* b, int chain, double* out);
 int bmg_batch_p_proposal(bmg_batch* b, int chain, double* out);
+/* a new source for a chain's add/remove proposal (after bmg_batch_setup):
+ * p_proposal <- p (p values in [0, 1]), q_add = max(p, q_add_min) and
+ * q_rem = max(1 - p, q_rem_min) with the proposal trees rebuilt, as an
+ * adaptation step leaves them (src/sampler.cpp:517-524, 797-806).  Set at
+ * the end of burn-in (adaptation off) the table is the proposal for the
+ * rest of the run, and the moves keep the posterior for any table; the
+ * learned per-SNP proposal sets its network's table through it. */
+int bmg_batch_set_p_proposal(bmg_batch* b, int chain, const double* p);
 /* per-iteration trace (needs trace_every_iter): move, jd, movesize, size,
  * loglik; returns the number of iterations in *len */
 int bmg_batch_trace(bmg_batch* b, int chain, uint8_t* move, uint8_t* jd, uint8_t* ms,

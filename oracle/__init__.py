@@ -172,6 +172,7 @@ def load():
     _sig(lib, "orc_sampler_iterate", C.c_int, _P, C.c_uint64)
     _sig(lib, "orc_sampler_p_rao", _D, _P)
     _sig(lib, "orc_sampler_p_proposal", _D, _P)
+    _sig(lib, "orc_sampler_set_p_proposal", C.c_int, _P, _D)
     _sig(lib, "orc_sampler_trace", C.c_int64, _P, _U8, _U8, _U8, _U32, _D)
     _sig(lib, "orc_sampler_thinned", C.c_int64, _P, _U32, _U32, C.c_int64, _D, _D, _D)
     _sig(lib, "orc_sampler_thinned_pve", C.c_int64, _P, _D)
@@ -476,6 +477,13 @@ class Sampler:
     def p_proposal(self):
         return np.ctypeslib.as_array(lib().orc_sampler_p_proposal(self.h),
                                      shape=(self.data.p,)).copy()
+
+    def set_p_proposal(self, p):
+        """Replace the add/remove proposal's source (q_add / q_rem and the
+        trees follow); 0 on success."""
+        self._pp = np.ascontiguousarray(p, dtype=np.float64)
+        assert self._pp.shape == (self.data.p,)
+        return lib().orc_sampler_set_p_proposal(self.h, ptr(self._pp, C.c_double))
 
     def refresh_draws(self, count):
         """count independent tau2/alpha refreshes of the current model from

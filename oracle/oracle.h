@@ -155,6 +155,9 @@ int    orc_sampler_set_move_mix(orc_sampler* s, const double p[3]);
 int    orc_sampler_iterate(orc_sampler* s, uint64_t n_iter);
 const double* orc_sampler_p_rao(const orc_sampler* s);
 const double* orc_sampler_p_proposal(const orc_sampler* s);
+/* replace p_proposal (each in [0, 1]) and rebuild q_add / q_rem and the
+ * proposal trees from it; -1 on a value outside [0, 1] */
+int    orc_sampler_set_p_proposal(orc_sampler* s, const double* p);
 /* per-iteration trace (length = iterations run): move, jumpdistance,
  * movesize, model size after the move, log-likelihood after the move */
 int64_t orc_sampler_trace(const orc_sampler* s, uint8_t* move, uint8_t* jd,

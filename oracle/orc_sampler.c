@@ -1509,6 +1509,23 @@ int orc_sampler_sample(orc_sampler* s, uint64_t n_iter)
 
 const double* orc_sampler_p_rao(const orc_sampler* s) { return s->p_rao; }
 const double* orc_sampler_p_proposal(const orc_sampler* s) { return s->p_proposal; }
+
+/* A new source of the add/remove proposal: p_proposal is replaced and the
+ * proposal weights follow as after an adaptation step (src/sampler.cpp:
+ * 797-806, 517-524): q_add = max(p, q_add_min), q_rem = max(1 - p,
+ * q_rem_min), sum trees rebuilt with their zeroed leaves kept.  With
+ * adaptation finished the proposal then stays fixed, so the moves keep the
+ * posterior for any such table. */
+int orc_sampler_set_p_proposal(orc_sampler* s, const double* p)
+{
+  for (size_t i = 0; i < s->m_g; ++i)
+    if (!(p[i] >= 0.0 && p[i] <= 1.0)) return -1;
+  memcpy(s->p_proposal, p, sizeof(double) * s->m_g);
+  compute_proposal_dist(s);
+  orc_sumtree_set_weights(s->dd_add, s->q_add);
+  orc_sumtree_set_weights(s->dd_rem, s->q_rem);
+  return 0;
+}
 orc_model* orc_sampler_current_model(orc_sampler* s) { return s->cur; }
 orc_model* orc_sampler_new_model(orc_sampler* s) { return s->nw; }
 uint64_t orc_sampler_rng_counter(const orc_sampler* s)

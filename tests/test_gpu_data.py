@@ -77,7 +77,8 @@ def test_recode_matches_oracle(bmg):
 
 @pytest.mark.parametrize("W,ca,cb,p", [(8, 0, 0, 300), (40, 0, 0, 300), (17, 0, 1, 300), (100, 3, 2, 300),
                                         (1, 0, 0, 300), (21, 0, 0, 389), (64, 1, 1, 300), (200, 2, 0, 700),
-                                        (500, 0, 0, 700), (560, 3, 3, 900), (560, 1, 2, 700)])
+                                        (500, 0, 0, 700), (560, 3, 3, 900), (560, 1, 2, 700),
+                                        (100, 0, 0, 60), (33, 2, 1, 129)])
 def test_band_xtx_exact(bmg, W, ca, cb, p):
     """Every column-tile bucket (W up to 560), same and different codings,
     row blocks cut by p, missing genotypes."""
