@@ -577,44 +577,66 @@ __device__ __forceinline__ void chol_downdate(Smem<KMAX>& S, int m, int cr) {
   }
 }
 
-// Full factorisation of S.L (m x m, in place), right-looking; returns 0 if
-// not positive definite.  Step k: every thread takes the pivot, the thread
-// of column i > k scales its row-k entry (kept in S.work for the others),
-// then walks its own column down from row k + 1 subtracting work[j] * u_ki,
-// eight rows at a time: the chunk's sixteen LDS loads are issued together
-// before its eight stores (a row-at-a-time loop waits out the LDS latency
-// on every row, since the stores may alias the next loads).  Per element
-// the subtractions come in k order, each a multiply and a subtract, as in
-// the sequential loop; two barriers a step.
+// Full factorisation of S.L (m x m, in place), right-looking in panels of
+// eight columns; returns 0 if not positive definite.  Within a panel, step
+// k takes the pivot, the thread of column i > k scales its row-k entry
+// (kept in S.work for the others) and subtracts work[j] * u_ki from its rows
+// j in (k, panel end).  After the panel, the thread of each later column i
+// walks its rows j at or past the panel end once, subtracting the panel's
+// eight products u_kj * u_ki in k order, with u_kj read from column j (one
+// address for all lanes) and its own u_ki in registers.  Every element thus
+// receives the sequential loop's subtractions in k order, each a multiply
+// and a subtract, but the rows below a panel are read and written once per
+// panel instead of once per step: those reads and writes (each lane in its
+// own column, so they meet bank conflicts) bound the unblocked loop.
 template <int KMAX>
 __device__ __forceinline__ int chol_full(Smem<KMAX>& S, int m) {
+  constexpr int PB = 8;
   __syncthreads();
-  for (int k = 0; k < m; ++k) {
-    const double akk = S.L[P(k) + k];
-    if (!(akk > 0.0)) { __syncthreads(); return 0; }
-    const double dk = sqrt(akk);
-    const int i = k + 1 + (int)threadIdx.x;   // this thread's column (m <= 256)
-    double u = 0.0;
-    if (i < m) {
-      u = S.L[P(i) + k] / dk;
-      S.L[P(i) + k] = u;
-      S.work[i] = u;
+  for (int k0 = 0; k0 < m; k0 += PB) {
+    const int k1 = k0 + PB < m ? k0 + PB : m;
+    for (int k = k0; k < k1; ++k) {
+      const double akk = S.L[P(k) + k];
+      if (!(akk > 0.0)) { __syncthreads(); return 0; }
+      const double dk = sqrt(akk);
+      const int i = k + 1 + (int)threadIdx.x;   // this thread's column (m <= 256)
+      double u = 0.0;
+      if (i < m) {
+        u = S.L[P(i) + k] / dk;
+        S.L[P(i) + k] = u;
+        S.work[i] = u;
+      }
+      __syncthreads();
+      if (threadIdx.x == 0) S.L[P(k) + k] = dk;
+      if (i < m) {
+        double* col = S.L + P(i);
+        const int je = i < k1 - 1 ? i : k1 - 1;   // panel rows only
+        for (int j = k + 1; j <= je; ++j) col[j] = col[j] - S.work[j] * u;
+      }
+      __syncthreads();
     }
-    __syncthreads();
-    if (threadIdx.x == 0) S.L[P(k) + k] = dk;
+    // rows at or past k1 of the columns past k1: the panel's products
+    const int i = k1 + (int)threadIdx.x;
     if (i < m) {
       double* col = S.L + P(i);
-      for (int j0 = k + 1; j0 <= i; j0 += 8) {
-        double c[8], w[8];
+      const int nk = k1 - k0;
+      double ui[PB];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-          const int j = j0 + q <= i ? j0 + q : i;   // past i: row i again, not stored
-          c[q] = col[j];
-          w[q] = S.work[j];
+      for (int q = 0; q < PB; ++q) ui[q] = q < nk ? col[k0 + q] : 0.0;
+      for (int j0 = k1; j0 <= i; j0 += 4) {
+        double a[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) a[r] = col[j0 + r <= i ? j0 + r : i];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const double* cj = S.L + P(j0 + r <= i ? j0 + r : i) + k0;   // u_kj of column j
+#pragma unroll
+          for (int q = 0; q < PB; ++q)
+            if (q < nk) a[r] = a[r] - cj[q] * ui[q];
         }
 #pragma unroll
-        for (int q = 0; q < 8; ++q)
-          if (j0 + q <= i) col[j0 + q] = c[q] - w[q] * u;
+        for (int r = 0; r < 4; ++r)
+          if (j0 + r <= i) col[j0 + r] = a[r];
       }
     }
     __syncthreads();

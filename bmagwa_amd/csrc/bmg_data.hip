@@ -361,25 +361,35 @@ __global__ void k_scan_combine(const int32_t* __restrict__ acc, uint64_t p, int 
 
 // ------------------------------------------------------ band X^T X ------
 // out[j * W + d] = sum_i x_j(i) x_{j+d}(i).  A workgroup of 8 waves owns
-// 128 row SNPs [r0, r0 + 128) and the column SNPs [r0, r0 + 128 + W); wave w
-// owns row tile w and every column tile within the band of it.  Per chunk
-// of 4 words (64 individuals) the workgroup expands those SNPs to int8 in
-// LDS once (two buffers: the chunk after is loaded from HBM into registers
-// while this chunk's MFMAs run, so one barrier per chunk suffices); the
-// wave then reads its row tile once and runs one 16x16x64 int8 MFMA per
-// column tile.  Both operands use the split byte order of word_split_i8x16,
-// so they agree on which individual each k is.  SNP slots are 80 bytes (64
-// used), so the 16 rows a lane group reads fall in distinct banks.
+// 128 row SNPs [r0, r0 + 128) (eight 16-row tiles) and the column SNPs
+// [r0, r0 + 128 + W).  Wave w takes the four row tiles 4 (w & 1) ..
+// 4 (w & 1) + 3 and a quarter of the column offsets, d in [g NDQ, (g + 1)
+// NDQ) with g = w >> 1 (a 16-row tile needs offsets 0 .. ND - 1, ND =
+// (W + 31) / 16 <= 4 NDQ).  Column tile c is offset c - rt of row tile rt,
+// so each column tile the wave reads serves up to four MFMAs.
+//
+// LDS holds the packed 2-bit words (a chunk of 16 words, 256 individuals,
+// per SNP; slots of 20 words so that the 64 lanes of a read hit distinct
+// banks), not expanded bytes: each lane reads the one word its MFMA operand
+// needs and expands it to sixteen int8 values in registers (the split byte
+// order of word_split_i8x16, the same for both operands, so they agree on
+// which individual each k is), which cuts the LDS traffic per MFMA by four
+// and lets the row and column codings differ without a second copy.  Each
+// 16x16x64 int8 MFMA takes one word per lane of each operand.
+//
+// Chunks are double-buffered in LDS: chunk c + 1's loads are issued after
+// the barrier that ends chunk c - 1 and written to LDS after chunk c's
+// MFMAs, so a chunk of MFMAs covers their latency; one barrier per chunk.
 #define BAND_ROWS 128
 #define BAND_WAVES 8
 #define BAND_MAXP 36   // column tiles per row tile: (W + 31) / 16 <= 36, W <= 560
-#define BAND_SLOT 20   // words per SNP slot in LDS
-template <int CA, int CB, int MAXP>
+#define BAND_CW 16     // packed words per SNP per chunk
+#define BAND_SW 20     // LDS words per SNP slot
+template <int CA, int CB, int NDQ>
 __global__ __launch_bounds__(64 * BAND_WAVES) void k_band(const uint32_t* __restrict__ g, uint64_t p,
                                                           uint32_t W32, uint32_t W,
                                                           int32_t* __restrict__ out, uint64_t n_row_blocks) {
-  constexpr bool SAME = CA == CB;
-  extern __shared__ uint32_t sx[];  // 2 buffers x [ncol (+ 128 if !SAME) slots][BAND_SLOT words]
+  extern __shared__ uint32_t sx[];  // 2 buffers x ncol slots x BAND_SW words
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   // XCD-aware ordering: workgroups are dealt round-robin over the 8 XCDs, so
   // row block rb goes to XCD rb / per; neighbouring row blocks (whose column
@@ -391,72 +401,98 @@ __global__ __launch_bounds__(64 * BAND_WAVES) void k_band(const uint32_t* __rest
   if (rb >= nb) return;
   const uint64_t r0 = rb * BAND_ROWS;
   const uint32_t ncol = ((BAND_ROWS + W + 15) / 16) * 16;
-  const uint32_t nslot = ncol + (SAME ? 0 : BAND_ROWS);
-  const int ND = (int)((W + 31) / 16);  // column tiles ct - rt in [0, ND)
-  // the raw words of this thread: SNP slot sn, 16 bytes per chunk
-  constexpr int LPT = (BAND_ROWS + 560 + 15) / 16 * 16 / (64 * BAND_WAVES) + 1;  // loads per thread
+  const int ND = (int)((W + 31) / 16);
+  const uint32_t nch = W32 / BAND_CW;
+  // HBM -> registers -> LDS: uint4 e of a chunk is word 4 (e % 4) of slot e / 4
+  constexpr int UPS = BAND_CW / 4;
+  constexpr int LPT = ((BAND_ROWS + 560 + 15) / 16 * 16 * UPS + 64 * BAND_WAVES - 1) / (64 * BAND_WAVES);
   uint4 raw[LPT];
-  auto load = [&](uint32_t wb) {
+  auto load = [&](uint32_t ch) {
+    const uint32_t wc = (ch < nch ? ch : nch - 1) * BAND_CW;
 #pragma unroll
     for (int i = 0; i < LPT; ++i) {
-      const uint32_t sn = threadIdx.x + i * 64 * BAND_WAVES;
+      const uint32_t e = threadIdx.x + i * 64 * BAND_WAVES;
+      const uint32_t sn = min(e / UPS, ncol - 1);
       const uint64_t snp = min(r0 + sn, p - 1);
-      raw[i] = (sn < ncol && r0 + sn < p) ? *(const uint4*)(g + snp * W32 + wb) : make_uint4(0, 0, 0, 0);
+      raw[i] = *(const uint4*)(g + snp * W32 + wc + (e % UPS) * 4);
     }
   };
-  v4i c[MAXP];
-#pragma unroll
-  for (int i = 0; i < MAXP; ++i) c[i] = v4i{0, 0, 0, 0};
-  const int row = lane & 15, grp = lane >> 4;
-  const uint32_t nch = W32 / 4;
-  const uint32_t last_tile = ncol / 16 - 1;
-  load(0);
-  for (uint32_t ch = 0; ch < nch; ++ch) {
-    uint32_t* buf = sx + (size_t)(ch & 1) * nslot * BAND_SLOT;
+  auto store = [&](uint32_t* buf) {
 #pragma unroll
     for (int i = 0; i < LPT; ++i) {
-      const uint32_t sn = threadIdx.x + i * 64 * BAND_WAVES;
+      const uint32_t e = threadIdx.x + i * 64 * BAND_WAVES;
+      const uint32_t sn = e / UPS;
       if (sn < ncol) {
-        uint32_t o[4];
-        const uint32_t w4[4] = {raw[i].x, raw[i].y, raw[i].z, raw[i].w};
+        const bool inb = r0 + sn < p;
+        *(uint4*)(buf + sn * BAND_SW + (e % UPS) * 4) =
+            inb ? raw[i] : make_uint4(0, 0, 0, 0);
+      }
+    }
+  };
+  auto expand = [](uint32_t w, int cod) -> v4i {
+    uint32_t o[4];
+    if (cod == 0) word_split_i8x16<0>(w, o);
+    else if (cod == 1) word_split_i8x16<1>(w, o);
+    else if (cod == 2) word_split_i8x16<2>(w, o);
+    else word_split_i8x16<3>(w, o);
+    return v4i{(int)o[0], (int)o[1], (int)o[2], (int)o[3]};
+  };
+  v4i c[4][NDQ];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          word_split_i8x16<CB>(w4[q], o);
-          *(uint4*)(buf + sn * BAND_SLOT + q * 4) = make_uint4(o[0], o[1], o[2], o[3]);
-        }
-        if (!SAME && sn < BAND_ROWS) {
+  for (int i = 0; i < 4; ++i)
 #pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            word_split_i8x16<CA>(w4[q], o);
-            *(uint4*)(buf + (ncol + sn) * BAND_SLOT + q * 4) = make_uint4(o[0], o[1], o[2], o[3]);
-          }
+    for (int e = 0; e < NDQ; ++e) c[i][e] = v4i{0, 0, 0, 0};
+  const int row = lane & 15, grp = lane >> 4;
+  const int rt0 = 4 * (wave & 1);          // row tiles rt0 .. rt0 + 3
+  const int dlo = (wave >> 1) * NDQ;       // offsets [dlo, dlo + NDQ)
+  const uint32_t last_tile = ncol / 16 - 1;
+  constexpr int NB = NDQ + 3;              // column tiles rt0 + dlo + t, t < NB
+  load(0);
+  store(sx);
+  __syncthreads();
+  load(1);
+  for (uint32_t ch = 0; ch < nch; ++ch) {
+    const uint32_t* buf = sx + (size_t)(ch & 1) * ncol * BAND_SW;
+#pragma unroll 1
+    for (int st = 0; st < BAND_CW / 4; ++st) {
+      const int wo = st * 4 + grp;         // this lane's word of the 64 individuals
+      uint32_t aw[4], bw[NB];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) aw[i] = buf[((rt0 + i) * 16 + row) * BAND_SW + wo];
+#pragma unroll
+      for (int t = 0; t < NB; ++t)
+        bw[t] = buf[(min((uint32_t)(rt0 + dlo + t), last_tile) * 16 + row) * BAND_SW + wo];
+      v4i a[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) a[i] = expand(aw[i], CA);
+#pragma unroll
+      for (int t = 0; t < NB; ++t) {
+        const v4i b = expand(bw[t], CB);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int e = t - i;
+          if (e >= 0 && e < NDQ) c[i][e] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[i], b, c[i][e], 0, 0, 0);
         }
       }
     }
+    if (ch + 1 < nch) store(sx + (size_t)((ch + 1) & 1) * ncol * BAND_SW);
     __syncthreads();
-    if (ch + 1 < nch) load((ch + 1) * 4);
-    const uint32_t* xa = buf + (SAME ? 0 : ncol * BAND_SLOT);
-    const uint4 av = *(const uint4*)(xa + (wave * 16 + row) * BAND_SLOT + grp * 4);
-    const v4i a = {(int)av.x, (int)av.y, (int)av.z, (int)av.w};
-    // MAXP >= ND column tiles, unconditionally (so the LDS reads batch); the
-    // tiles past ND read a clamped slot and are never stored
-#pragma unroll
-    for (int d = 0; d < MAXP; ++d) {
-      const uint32_t sb = min((uint32_t)(wave + d), last_tile) * 16 + row;
-      const uint4 bv = *(const uint4*)(buf + sb * BAND_SLOT + grp * 4);
-      const v4i b = {(int)bv.x, (int)bv.y, (int)bv.z, (int)bv.w};
-      c[d] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, c[d], 0, 0, 0);
-    }
+    if (ch + 2 < nch) load(ch + 2);
   }
   const int col = lane & 15;
 #pragma unroll
-  for (int d = 0; d < MAXP; ++d) {
-    if (d < ND) {
+  for (int i = 0; i < 4; ++i) {
+    const int rt = rt0 + i;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const uint64_t j = r0 + wave * 16 + 4 * grp + r;   // row SNP
-        const uint64_t jj = r0 + (wave + d) * 16 + col;    // column SNP
-        if (j < p && jj >= j && jj - j < W && jj < p) out[j * W + (jj - j)] = c[d][r];
+    for (int e = 0; e < NDQ; ++e) {
+      const int d = dlo + e;
+      if (d < ND) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const uint64_t j = r0 + rt * 16 + 4 * grp + r;    // row SNP
+          const uint64_t jj = r0 + (rt + d) * 16 + col;     // column SNP
+          if (j < p && jj >= j && jj - j < W && jj < p) out[j * W + (jj - j)] = c[i][e][r];
+        }
       }
     }
   }
@@ -951,15 +987,17 @@ int launch_read_timed(const void* src, uint64_t bytes, void* sink, int reps, dou
 template <int CA, int CB>
 static void band_p(bmg_ctx* c, uint32_t W, int32_t* out_dev, uint64_t nrb, unsigned grid, hipStream_t s) {
   const uint32_t ncol = ((BAND_ROWS + W + 15) / 16) * 16;
-  const size_t lds = (size_t)2 * (ncol + (CA == CB ? 0 : BAND_ROWS)) * BAND_SLOT * sizeof(uint32_t);
+  const size_t lds = (size_t)2 * ncol * BAND_SW * sizeof(uint32_t);
   const dim3 blk(64 * BAND_WAVES);
   const int nd = (int)((W + 31) / 16);
+  // offsets per quarter: the smallest bucket with 4 NDQ >= nd
   if (nd <= 4)
-    hipLaunchKernelGGL((k_band<CA, CB, 4>), dim3(grid), blk, lds, s, c->g, c->p, c->W32, W, out_dev, nrb);
+    hipLaunchKernelGGL((k_band<CA, CB, 1>), dim3(grid), blk, lds, s, c->g, c->p, c->W32, W, out_dev, nrb);
   else if (nd <= 12)
-    hipLaunchKernelGGL((k_band<CA, CB, 12>), dim3(grid), blk, lds, s, c->g, c->p, c->W32, W, out_dev, nrb);
+    hipLaunchKernelGGL((k_band<CA, CB, 3>), dim3(grid), blk, lds, s, c->g, c->p, c->W32, W, out_dev, nrb);
   else
-    hipLaunchKernelGGL((k_band<CA, CB, BAND_MAXP>), dim3(grid), blk, lds, s, c->g, c->p, c->W32, W, out_dev, nrb);
+    hipLaunchKernelGGL((k_band<CA, CB, BAND_MAXP / 4>), dim3(grid), blk, lds, s, c->g, c->p, c->W32, W, out_dev,
+                       nrb);
 }
 
 template <int CA>
