@@ -12,13 +12,17 @@ back on the same dataset, each after `--warmup` untimed steps (which cover
 the burn-in of 4 RB blocks, so adaptation and the learned table are frozen
 before the timed steps):
 
-* learned_batched: `--chains` chains (16) in one kernel, plain PMV moves
-  plus the learned swap (network trained online during burn-in);
+* learned_batched: `--chains` chains (16) in one kernel with both learned
+  proposals: the learned swap and the learned per-SNP add/remove table
+  (both networks trained online during burn-in, frozen at its end);
 * batched:         `--chains` chains with the plain moves only;
+* learned_swap:    `--chains` chains, plain moves plus the learned swap only;
+* learned_proposal: `--chains` chains, the learned per-SNP table only;
 * plain:           1 chain, plain moves (the reference's sampler on the GPU),
                    `--plain-steps` timed steps (60: long enough for its own ESS);
-* batched_full:    `--full-chains` chains (256: one per CU) with the learned
-  swap, to show the whole GPU (reported beside the headline).
+* batched_full:    `--full-chains` chains (256: one per CU) with both learned
+  proposals (`--full-proposal`), to show the whole GPU (reported beside the
+  headline) and to carry the PIP check.
 
 PIP ESS (median over the monitored SNPs: those whose PIP in the long
 plain-method run lies in [0.05, 0.95]) is the multi-chain ESS
@@ -153,6 +157,8 @@ def run_batch(bmg, synth, ctx, n_chains, steps, warmup, ips, seed0, batch_cfg=No
                      learned_swap_share=round(float(learned.mean()), 3) if len(learned) else None,
                      learned_swap_accept=round(float((jd[k0:][learned] > 0).mean()), 3)
                      if learned.any() else None)
+        if prop & 2:
+            learn["table"] = "per-SNP network (features: marginal score, RB running means, LD to the model)"
     b.close()
     return dict(wall=wall, steps=steps, setup=t_setup, burnin=t_burnin, half_split=half, ths=ths, first=first, sizes=sizes, acc=acc, learn=learn,
                 graph_replays=gs["replays"], graph_ms=gs["graph_ms"],
@@ -391,7 +397,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the band and scan timings")
     ap.add_argument("--no-learned", dest="learned", action="store_false",
-                    help="skip the learned-swap batch (the headline is then the plain batched run)")
+                    help="skip the learned-proposal batches (the headline is then the plain batched run)")
     ap.add_argument("--full-chains", type=int, default=256,
                     help="also run the learned batch with this many chains (e.g. 256: one per CU)")
     ap.add_argument("--kernels-only", action="store_true",
@@ -407,11 +413,14 @@ def main():
                          "reference example uses 10)")
     ap.add_argument("--plain-steps", type=int, default=60,
                     help="timed steps of the 1-chain plain run (long enough for its own ESS)")
+    ap.add_argument("--full-proposal", type=int, default=3, choices=(0, 1, 2, 3),
+                    help="proposal of the --full-chains run (1: learned swap, 2: learned per-SNP "
+                         "proposal, 3: both; 0: plain)")
     ap.add_argument("--full-steps", type=int, default=80,
                     help="timed steps of the --full-chains run")
     ap.add_argument("--pip-ref-chains", type=int, default=256,
                     help="chains of the long plain-moves reference run for the PIP check (0: skip)")
-    ap.add_argument("--pip-ref-steps", type=int, default=80,
+    ap.add_argument("--pip-ref-steps", type=int, default=160,
                     help="timed steps of the PIP reference run")
     args = ap.parse_args()
     global DELAY_REJECTION, MAX_MODEL_SIZE, N, P, N_CAUSAL, H2, MAF, SEED
@@ -456,12 +465,15 @@ def main():
     th = args.thin
     if args.learned:
         runs["learned_batched"] = run_batch(bmg, synth, ctx, args.chains, args.steps, args.warmup, ips,
-                                            100, dict(proposal=1, trace_every_iter=1), th)
+                                            100, dict(proposal=3, trace_every_iter=1), th)
     runs["batched"] = run_batch(bmg, synth, ctx, args.chains, args.steps, args.warmup, ips, 100,
                                 dict(proposal=0), th)
     if args.learned:
-        # the per-SNP learned proposal: add/remove weights from the network's
-        # table, set at the end of burn-in (same seeds as the batched run)
+        # each learned proposal alone (same seeds as the batched run): the
+        # learned swap, and the per-SNP add/remove table set at the end of
+        # burn-in
+        runs["learned_swap"] = run_batch(bmg, synth, ctx, args.chains, args.steps, args.warmup, ips, 100,
+                                         dict(proposal=1, trace_every_iter=1), th)
         runs["learned_proposal"] = run_batch(bmg, synth, ctx, args.chains, args.steps, args.warmup, ips, 100,
                                              dict(proposal=2), th)
     if not args.no_plain:
@@ -470,7 +482,8 @@ def main():
     if args.full_chains > 0:
         runs["batched_full"] = run_batch(bmg, synth, ctx, args.full_chains, args.full_steps,
                                          args.warmup, ips,
-                                         100, dict(proposal=1 if args.learned else 0, trace_every_iter=1),
+                                         100, dict(proposal=args.full_proposal if args.learned else 0,
+                                                   trace_every_iter=1),
                                          th)
     # the long plain-method run the PIPs are checked against (its own seeds;
     # RB PIPs only, so its samples are thinned)
@@ -599,8 +612,9 @@ def main():
                    "chains": args.chains, "iters_per_step": ips,
                    "max_model_size": MAX_MODEL_SIZE or None,
                    "delay_rejection": DELAY_REJECTION,
-                   "proposal": ("plain moves + learned swap (MLP 16-256-256-1, bf16)" if args.learned
-                                else "plain PMV moves (learned swap not run at this size)")},
+                   "proposal": ("plain moves + learned swap + learned per-SNP add/remove table "
+                                "(two MLPs 16-256-256-1, bf16)" if args.learned
+                                else "plain PMV moves (learned proposals not run at this size)")},
         "iters_per_s": lb["iters_per_s"],
         "runs": summary,
         "headline_run": head,
