@@ -75,7 +75,7 @@ HBM_PEAK_GBS = 8000.0
 # (profiles/kernels_scan_band_copy_n10000_p500000.md): fetch 2 x 626,386.4
 # KiB, write 31,264 KiB
 SCAN_TRAFFIC = dict(n=10000, p=500000, nvec=3, digits=5,
-                    fetch_bytes=int(2 * 626386.4 * 1024), write_bytes=31264 * 1024,
+                    fetch_bytes=int(2 * 626737.6 * 1024), write_bytes=int(31269.7 * 1024),
                     source="profiles/kernels_scan_band_copy_n10000_p500000.md")
 INT8_PEAK_TOPS = 5000.0
 
