@@ -181,6 +181,7 @@ SIGNATURES = {
     "bmg_batch_learn_params": (C.c_int, _P, C.c_int, C.POINTER(C.c_float)),
     "bmg_batch_learn_static": (C.c_int, _P, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float), _I32),
     "bmg_batch_learn_features": (C.c_int, _P, C.c_int, _U32, _I32, C.c_int64, C.POINTER(C.c_float)),
+    "bmg_batch_learn_snp_features": (C.c_int, _P, C.c_int, C.c_int, _U32, C.c_int64, C.POINTER(C.c_float)),
     "bmg_batch_state": (C.c_int, _P, C.c_int, _D),
     "bmg_batch_loci": (C.c_int, _P, C.c_int, _U32, C.c_int, C.POINTER(C.c_int)),
     "bmg_batch_p_rao": (C.c_int, _P, C.c_int, _D),
@@ -639,6 +640,14 @@ class Batch:
         out = np.zeros((len(jj), 16), np.float32)
         check(lib().bmg_batch_learn_features(self.h, ds, _p(jj, C.c_uint32), _p(oo, C.c_int32),
                                              len(jj), _p(out, C.c_float)))
+        return out
+
+    def learn_snp_features(self, ds, chain, j):
+        """Per-SNP proposal features of SNPs j in chain's present state."""
+        jj = np.ascontiguousarray(j, dtype=np.uint32)
+        out = np.zeros((len(jj), 16), np.float32)
+        check(lib().bmg_batch_learn_snp_features(self.h, ds, chain, _p(jj, C.c_uint32), len(jj),
+                                                 _p(out, C.c_float)))
         return out
 
     def set_learned(self, ds, w, H, p_learned):

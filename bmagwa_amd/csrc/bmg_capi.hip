@@ -1347,6 +1347,20 @@ extern "C" int bmg_batch_learn_static(bmg_batch* b, int ds, float* z, float* pba
   return learn_static(L, z, pbar, band, b->stream);
 }
 
+extern "C" int bmg_batch_learn_snp_features(bmg_batch* b, int ds, int chain, const uint32_t* j, int64_t rows,
+                                            float* out) {
+  LearnDS* L = learner(b, ds);
+  if (!L) return BMG_E_ARG;
+  if (chain < 0 || chain >= b->n_chains || b->chain_ds[chain] != ds || !j || !out || rows < 1) {
+    set_error("bad argument"); return BMG_E_ARG;
+  }
+  const uint64_t p = b->h_data[ds].p;
+  for (int64_t i = 0; i < rows; ++i)
+    if (j[i] >= p) { set_error("SNP index out of range"); return BMG_E_ARG; }
+  BMG_HIP(hipStreamSynchronize(b->stream));
+  return learn_snp_features(L, b->d_arrays, chain, j, rows, out, b->stream);
+}
+
 extern "C" int bmg_batch_learn_features(bmg_batch* b, int ds, const uint32_t* j, const int32_t* off,
                                         int64_t rows, float* out) {
   LearnDS* L = learner(b, ds);

@@ -63,6 +63,10 @@ int learn_snp_apply(LearnDS* L, const DevData* d_data, const ChainConst* d_const
                     const ChainArrays* d_arrays, const int* chains_host, int n, const uint64_t* lev_len, int top,
                     hipStream_t s);
 int learn_snp_applied(const LearnDS* L);
+// device-side per-SNP features of SNPs j in chain `chain`'s state, for
+// checking: out rows x 16
+int learn_snp_features(LearnDS* L, const ChainArrays* d_arrays, int chain, const uint32_t* j_host, int64_t rows,
+                       float* out, hipStream_t s);
 
 // device-side features of pairs (j, offset) for checking: out rows x 16
 int learn_features(LearnDS* L, const uint32_t* j_host, const int32_t* off_host, int64_t rows, float* out,

@@ -451,6 +451,16 @@ __global__ void k_snp_table_feat(LearnView L, const ChainArrays* __restrict__ ar
   store_row(X, i, f);
 }
 
+// features of SNPs js[0..rows) in chain c's state, for checking
+__global__ void k_snp_features_rows(LearnView L, const ChainArrays* __restrict__ arrays, int c,
+                                    const uint32_t* __restrict__ js, int64_t rows, float* __restrict__ out) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= rows) return;
+  float f[16];
+  snp_features(L, arrays[c], js[i], f);
+  for (int q = 0; q < 16; ++q) out[i * 16 + q] = f[q];
+}
+
 // the table: the network's logit through the logistic function
 __global__ void k_snp_table_store(uint64_t r0, int64_t rows, const float* __restrict__ out,
                                   double* __restrict__ ptab) {
@@ -689,6 +699,27 @@ int learn_snp_apply(LearnDS* L, const DevData* d_data, const ChainConst* d_const
 }
 
 int learn_snp_applied(const LearnDS* L) { return L->applied; }
+
+int learn_snp_features(LearnDS* L, const ChainArrays* d_arrays, int chain, const uint32_t* j_host, int64_t rows,
+                       float* out, hipStream_t s) {
+  uint32_t* dj = nullptr;
+  float* dout = nullptr;
+  int rc = BMG_OK;
+  if (hipMalloc(&dj, sizeof(uint32_t) * rows) != hipSuccess || hipMalloc(&dout, sizeof(float) * 16 * rows) != hipSuccess) {
+    set_error("allocation failed"); rc = BMG_E_HIP;
+  }
+  if (!rc) {
+    hipMemcpyAsync(dj, j_host, sizeof(uint32_t) * rows, hipMemcpyHostToDevice, s);
+    hipLaunchKernelGGL(k_snp_features_rows, dim3(blocks(rows)), dim3(256), 0, s, L->v, d_arrays, chain, dj, rows, dout);
+    if (hipGetLastError() != hipSuccess) { set_error("features kernel"); rc = BMG_E_HIP; }
+  }
+  if (!rc) {
+    hipMemcpyAsync(out, dout, sizeof(float) * 16 * rows, hipMemcpyDeviceToHost, s);
+    if (hipStreamSynchronize(s) != hipSuccess) { set_error("features failed"); rc = BMG_E_HIP; }
+  }
+  hipFree(dj); hipFree(dout);
+  return rc;
+}
 
 int learn_features(LearnDS* L, const uint32_t* j_host, const int32_t* off_host, int64_t rows, float* out,
                    hipStream_t s) {

@@ -317,6 +317,11 @@ int bmg_batch_learn_samples(bmg_batch* b, int ds, uint32_t* j, int32_t* off, flo
                             int64_t cap, int64_t* count);
 int bmg_batch_learn_params(bmg_batch* b, int ds, float* params);
 int bmg_batch_learn_static(bmg_batch* b, int ds, float* z, float* pbar, int32_t* band);
+/* features of SNPs j[0..rows) for the per-SNP proposal network in chain
+ * `chain`'s present state (marginal score, running RB means, LD to the
+ * model; out rows x 16), for checking against the oracle */
+int bmg_batch_learn_snp_features(bmg_batch* b, int ds, int chain, const uint32_t* j, int64_t rows,
+                                 float* out);
 int bmg_batch_learn_features(bmg_batch* b, int ds, const uint32_t* j, const int32_t* off,
                              int64_t rows, float* out);
 /* per-chain state: loglik, size, sigma2, alpha, p_move_size, n_accepted,

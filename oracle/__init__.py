@@ -166,6 +166,9 @@ def load():
     _sig(lib, "orc_mlp_adam", None, _F, _F, _F, _F, C.c_float, C.c_int)
     _sig(lib, "orc_learn_pair_features", None, _D, C.c_int, C.c_int, C.c_uint32,
          C.POINTER(C.c_int32), C.c_int, _F, _F, C.c_uint64, C.c_int, _F)
+    _sig(lib, "orc_learn_snp_features", None, _D, C.c_int, C.c_int, C.c_uint32, C.c_uint64,
+         C.POINTER(C.c_int32), C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_int32), _D,
+         C.c_uint64, C.POINTER(C.c_float))
     _sig(lib, "orc_sampler_marginal_z", None, _P, _F)
     _sig(lib, "orc_sampler_swap_labels", C.c_int64, _P, C.c_int, C.POINTER(C.c_uint32),
          C.POINTER(C.c_int32), _F, C.c_int64)
@@ -599,6 +602,20 @@ def learn_pair_features(pre, ot, n, band, H, z, pbar, j, o):
     lib().orc_learn_pair_features(ptr(pre), pre.shape[1], ot, n, ptr(band, C.c_int32), H,
                                   ptr(z, C.c_float), ptr(pbar, C.c_float), int(j), int(o),
                                   ptr(f, C.c_float))
+    return f
+
+
+def learn_snp_features(pre, ot, n, band, H, z, pbar, model_inds, p_proposal, j):
+    """Features of SNP j for the per-SNP proposal network in a chain's state."""
+    pre = np.ascontiguousarray(pre, dtype=np.float64)
+    band = np.ascontiguousarray(band, dtype=np.int32)
+    z = _f32(z); pbar = _f32(pbar)
+    mi = np.ascontiguousarray(model_inds, dtype=np.int32)
+    pp = np.ascontiguousarray(p_proposal, dtype=np.float64)
+    f = np.zeros(16, np.float32)
+    lib().orc_learn_snp_features(ptr(pre), pre.shape[1], ot, n, len(z), ptr(band, C.c_int32), H,
+                                 ptr(z, C.c_float), ptr(pbar, C.c_float), ptr(mi, C.c_int32), ptr(pp), int(j),
+                                 ptr(f, C.c_float))
     return f
 
 

@@ -271,6 +271,12 @@ float  orc_bf16_round(float f);
 void   orc_learn_pair_features(const double* pre, int off, int ot, uint32_t n,
                                const int32_t* band, int H, const float* z, const float* pbar,
                                uint64_t j, int o, float* f);
+/* features of SNP j in a chain's state (model_inds, p_proposal) for the
+ * per-SNP proposal network */
+void   orc_learn_snp_features(const double* pre, int off, int ot, uint32_t n, uint64_t p,
+                              const int32_t* band, int H, const float* z, const float* pbar,
+                              const int32_t* model_inds, const double* p_proposal, uint64_t j,
+                              float* f);
 void   orc_sampler_marginal_z(orc_sampler* s, float* z);
 int64_t orc_sampler_swap_labels(orc_sampler* s, int H, uint32_t* oj, int32_t* oo, float* olab,
                                 int64_t cap);
