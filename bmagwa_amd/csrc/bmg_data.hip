@@ -389,7 +389,7 @@ template <int CA, int CB, int NDQ>
 __global__ __launch_bounds__(64 * BAND_WAVES) void k_band(const uint32_t* __restrict__ g, uint64_t p,
                                                           uint32_t W32, uint32_t W,
                                                           int32_t* __restrict__ out, uint64_t n_row_blocks) {
-  extern __shared__ uint32_t sx[];  // 2 buffers x ncol slots x BAND_SW words
+  extern __shared__ uint32_t sx[];  // 2 buffers x nslot slots x BAND_SW words
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   // XCD-aware ordering: workgroups are dealt round-robin over the 8 XCDs, so
   // row block rb goes to XCD rb / per; neighbouring row blocks (whose column
@@ -401,6 +401,7 @@ __global__ __launch_bounds__(64 * BAND_WAVES) void k_band(const uint32_t* __rest
   if (rb >= nb) return;
   const uint64_t r0 = rb * BAND_ROWS;
   const uint32_t ncol = ((BAND_ROWS + W + 15) / 16) * 16;
+  const uint32_t nslot = ncol + (CA == CB ? 0 : BAND_ROWS);   // per buffer
   const int ND = (int)((W + 31) / 16);
   const uint32_t nch = W32 / BAND_CW;
   // HBM -> registers -> LDS: uint4 e of a chunk is word 4 (e % 4) of slot e / 4
@@ -417,25 +418,32 @@ __global__ __launch_bounds__(64 * BAND_WAVES) void k_band(const uint32_t* __rest
       raw[i] = *(const uint4*)(g + snp * W32 + wc + (e % UPS) * 4);
     }
   };
+  // each word is stored as its value word: a 2-bit field per individual
+  // holding the coded genotype (0..2), so that an operand's expansion is a
+  // shift and a mask per output register; the row codings' copy (when they
+  // differ from the column coding) follows the ncol column slots
+  auto value_word = [](uint32_t w, int cod) -> uint32_t {
+    const uint32_t lo = w & BMG_EVEN, hi = (w >> 1) & BMG_EVEN;
+    return cod == 0 ? hi + (hi & lo) : (cod == 1 ? hi & ~lo : (cod == 2 ? hi : hi & lo));
+  };
   auto store = [&](uint32_t* buf) {
 #pragma unroll
     for (int i = 0; i < LPT; ++i) {
       const uint32_t e = threadIdx.x + i * 64 * BAND_WAVES;
       const uint32_t sn = e / UPS;
       if (sn < ncol) {
-        const bool inb = r0 + sn < p;
+        const uint4 w = r0 + sn < p ? raw[i] : make_uint4(0, 0, 0, 0);
         *(uint4*)(buf + sn * BAND_SW + (e % UPS) * 4) =
-            inb ? raw[i] : make_uint4(0, 0, 0, 0);
+            make_uint4(value_word(w.x, CB), value_word(w.y, CB), value_word(w.z, CB), value_word(w.w, CB));
+        if (CA != CB && sn < BAND_ROWS)
+          *(uint4*)(buf + (ncol + sn) * BAND_SW + (e % UPS) * 4) =
+              make_uint4(value_word(w.x, CA), value_word(w.y, CA), value_word(w.z, CA), value_word(w.w, CA));
       }
     }
   };
-  auto expand = [](uint32_t w, int cod) -> v4i {
-    uint32_t o[4];
-    if (cod == 0) word_split_i8x16<0>(w, o);
-    else if (cod == 1) word_split_i8x16<1>(w, o);
-    else if (cod == 2) word_split_i8x16<2>(w, o);
-    else word_split_i8x16<3>(w, o);
-    return v4i{(int)o[0], (int)o[1], (int)o[2], (int)o[3]};
+  auto expand = [](uint32_t v) -> v4i {
+    return v4i{(int)(v & 0x03030303u), (int)((v >> 2) & 0x03030303u), (int)((v >> 4) & 0x03030303u),
+               (int)((v >> 6) & 0x03030303u)};
   };
   v4i c[4][NDQ];
 #pragma unroll
@@ -452,30 +460,47 @@ __global__ __launch_bounds__(64 * BAND_WAVES) void k_band(const uint32_t* __rest
   __syncthreads();
   load(1);
   for (uint32_t ch = 0; ch < nch; ++ch) {
-    const uint32_t* buf = sx + (size_t)(ch & 1) * ncol * BAND_SW;
+    const uint32_t* buf = sx + (size_t)(ch & 1) * nslot * BAND_SW;
 #pragma unroll 1
     for (int st = 0; st < BAND_CW / 4; ++st) {
       const int wo = st * 4 + grp;         // this lane's word of the 64 individuals
       uint32_t aw[4], bw[NB];
+      const uint32_t* abuf = buf + (CA == CB ? 0 : (size_t)ncol * BAND_SW);
 #pragma unroll
-      for (int i = 0; i < 4; ++i) aw[i] = buf[((rt0 + i) * 16 + row) * BAND_SW + wo];
+      for (int i = 0; i < 4; ++i) aw[i] = abuf[((rt0 + i) * 16 + row) * BAND_SW + wo];
 #pragma unroll
       for (int t = 0; t < NB; ++t)
         bw[t] = buf[(min((uint32_t)(rt0 + dlo + t), last_tile) * 16 + row) * BAND_SW + wo];
       v4i a[4];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) a[i] = expand(aw[i], CA);
+      for (int i = 0; i < 4; ++i) a[i] = expand(aw[i]);
 #pragma unroll
       for (int t = 0; t < NB; ++t) {
-        const v4i b = expand(bw[t], CB);
+        const v4i b = expand(bw[t]);
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           const int e = t - i;
           if (e >= 0 && e < NDQ) c[i][e] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[i], b, c[i][e], 0, 0, 0);
         }
       }
+      // issue order: the LDS reads, the row operands' expansion, then per
+      // column tile its expansion (7 VALU) and its MFMAs, so that only one
+      // expanded column operand is live at a time
+      __builtin_amdgcn_sched_group_barrier(0x0100, 4 + NB, 1);
+      __builtin_amdgcn_sched_group_barrier(0x0002, 28, 1);
+#pragma unroll
+      for (int t = 0; t < NB; ++t) {
+        const int lo = t - 3 > 0 ? t - 3 : 0, hi = t < NDQ - 1 ? t : NDQ - 1;
+        __builtin_amdgcn_sched_group_barrier(0x0002, 7, 1);
+        switch (hi - lo + 1) {
+          case 1: __builtin_amdgcn_sched_group_barrier(0x0008, 1, 1); break;
+          case 2: __builtin_amdgcn_sched_group_barrier(0x0008, 2, 1); break;
+          case 3: __builtin_amdgcn_sched_group_barrier(0x0008, 3, 1); break;
+          default: __builtin_amdgcn_sched_group_barrier(0x0008, 4, 1); break;
+        }
+      }
     }
-    if (ch + 1 < nch) store(sx + (size_t)((ch + 1) & 1) * ncol * BAND_SW);
+    if (ch + 1 < nch) store(sx + (size_t)((ch + 1) & 1) * nslot * BAND_SW);
     __syncthreads();
     if (ch + 2 < nch) load(ch + 2);
   }
@@ -987,7 +1012,7 @@ int launch_read_timed(const void* src, uint64_t bytes, void* sink, int reps, dou
 template <int CA, int CB>
 static void band_p(bmg_ctx* c, uint32_t W, int32_t* out_dev, uint64_t nrb, unsigned grid, hipStream_t s) {
   const uint32_t ncol = ((BAND_ROWS + W + 15) / 16) * 16;
-  const size_t lds = (size_t)2 * ncol * BAND_SW * sizeof(uint32_t);
+  const size_t lds = (size_t)2 * (ncol + (CA == CB ? 0 : BAND_ROWS)) * BAND_SW * sizeof(uint32_t);
   const dim3 blk(64 * BAND_WAVES);
   const int nd = (int)((W + 31) / 16);
   // offsets per quarter: the smallest bucket with 4 NDQ >= nd
