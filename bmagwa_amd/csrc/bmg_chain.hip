@@ -219,7 +219,10 @@ __device__ __forceinline__ void planes_c(uint32_t w, int coding, uint32_t& p1, u
 }
 
 // S.work[c] = x_c^T x_new for c in [0, cols], column `cols` being the new
-// one itself; column 0 is the constant.  Exact integer popcount dots.
+// one itself; column 0 is the constant.  Exact integer dots: a word's value
+// planes give a value word (the coded genotype, 0..2, in each 2-bit
+// field), whose even and odd fields, masked into nibbles, meet the new
+// column's in two 8-way unsigned 4-bit dot products (v_dot8_u32_u4).
 //
 // A lane owns one 16-word chunk of the individuals and a group of columns:
 // it decodes the new column's chunk once, then streams its columns' chunks
@@ -240,13 +243,19 @@ __device__ __forceinline__ void column_dots(Ctx<KMAX>& C, uint64_t snp_new, int 
     const int grp = (int)(threadIdx.x / span);
     const uint32_t w0 = (ch0 + threadIdx.x % span) * 16;
     if (grp < groups) {
-      uint32_t n1[16], n2[16];
+      uint32_t ne[16], no[16];   // the new column's even / odd fields as nibbles
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const uint4 v = *(const uint4*)(wn + w0 + 4 * q);
         const uint32_t xs[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
-        for (int u = 0; u < 4; ++u) planes_c<CODING>(xs[u], coding_new, n1[4 * q + u], n2[4 * q + u]);
+        for (int u = 0; u < 4; ++u) {
+          uint32_t p1, p2;
+          planes_c<CODING>(xs[u], coding_new, p1, p2);
+          const uint32_t vw = p1 + p2;
+          ne[4 * q + u] = vw & 0x33333333u;
+          no[4 * q + u] = (vw >> 2) & 0x33333333u;
+        }
       }
       const int me = C.D.me;
       auto col_ptr = [&](int c) -> const uint32_t* {
@@ -268,7 +277,7 @@ __device__ __forceinline__ void column_dots(Ctx<KMAX>& C, uint64_t snp_new, int 
           const int cu = c + u * groups;
           if (cu > cols) break;
           const int cc = (cu >= cols || cu == 0) ? coding_new : (int)S.xtype[cu];
-          int sum = 0;
+          uint32_t sum = 0;
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
             const uint32_t xs[4] = {cur[u][q].x, cur[u][q].y, cur[u][q].z, cur[u][q].w};
@@ -276,13 +285,14 @@ __device__ __forceinline__ void column_dots(Ctx<KMAX>& C, uint64_t snp_new, int 
             for (int k4 = 0; k4 < 4; ++k4) {
               uint32_t a1, a2;
               planes_c<CODING>(xs[k4], cc, a1, a2);
-              // column 0 is the constant: its planes are all ones
-              if (cu == 0) { a1 = BMG_EVEN; a2 = 0u; }
+              // column 0 is the constant: a 1 in every field
+              const uint32_t vw = cu == 0 ? BMG_EVEN : a1 + a2;
               const int k = 4 * q + k4;
-              sum += (int)(__popc(a1 & n1[k]) + __popc(a1 & n2[k]) + __popc(a2 & n1[k]) + __popc(a2 & n2[k]));
+              sum = __builtin_amdgcn_udot8(vw & 0x33333333u, ne[k], sum, false);
+              sum = __builtin_amdgcn_udot8((vw >> 2) & 0x33333333u, no[k], sum, false);
             }
           }
-          if (cu == 0 || cu >= me) atomicAdd(&S.iacc[cu], sum);   // covariates: cov_dots
+          if (cu == 0 || cu >= me) atomicAdd(&S.iacc[cu], (int)sum);   // covariates: cov_dots
         }
       }
     }
