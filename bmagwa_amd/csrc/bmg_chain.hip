@@ -556,10 +556,26 @@ __device__ __forceinline__ void chol_downdate(Smem<KMAX>& S, int m, int cr) {
   // sign fix over the kept rows i in [cr, m-2]: flags first, then negation
   for (int i = cr + threadIdx.x; i <= m - 2; i += BMG_BLOCK) S.work[i] = S.L[P(i + 1) + i] < 0 ? -1.0 : 1.0;
   __syncthreads();
+  // eight rows at a time with unconditional stores: a row past jl is
+  // clamped to jl, so its store rewrites col[jl] with the value the last
+  // real row gives it (a conditional store per row waits out an LDS round
+  // trip each)
   for (int jl = cr + threadIdx.x; jl <= m - 2; jl += BMG_BLOCK) {
     double* col = S.L + P(jl + 1);
-    for (int i = cr; i <= jl; ++i)
-      if (S.work[i] < 0) col[i] = -col[i];
+    for (int i0 = cr; i0 <= jl; i0 += 8) {
+      double c[8], f[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int i = i0 + q <= jl ? i0 + q : jl;
+        c[q] = col[i];
+        f[q] = S.work[i];
+      }
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int i = i0 + q <= jl ? i0 + q : jl;
+        col[i] = f[q] < 0 ? -c[q] : c[q];
+      }
+    }
     if (S.work[jl] < 0) S.v[jl] = -S.v[jl];
   }
   __syncthreads();
@@ -618,10 +634,21 @@ __device__ __forceinline__ int chol_full(Smem<KMAX>& S, int m) {
       }
       __syncthreads();
       if (threadIdx.x == 0) S.L[P(k) + k] = dk;
-      if (i < m) {
+      if (i < m && k + 1 < k1) {
+        // the panel rows (k, min(i, k1 - 1)], at most seven: loads first,
+        // then unconditional stores (rows past the last one are clamped to
+        // it and rewrite it with the same value)
         double* col = S.L + P(i);
-        const int je = i < k1 - 1 ? i : k1 - 1;   // panel rows only
-        for (int j = k + 1; j <= je; ++j) col[j] = col[j] - S.work[j] * u;
+        const int je = i < k1 - 1 ? i : k1 - 1;
+        double c[PB - 1], w[PB - 1];
+#pragma unroll
+        for (int q = 0; q < PB - 1; ++q) {
+          const int j = k + 1 + q <= je ? k + 1 + q : je;
+          c[q] = col[j];
+          w[q] = S.work[j];
+        }
+#pragma unroll
+        for (int q = 0; q < PB - 1; ++q) col[k + 1 + q <= je ? k + 1 + q : je] = c[q] - w[q] * u;
       }
       __syncthreads();
     }
@@ -644,9 +671,10 @@ __device__ __forceinline__ int chol_full(Smem<KMAX>& S, int m) {
           for (int q = 0; q < PB; ++q)
             if (q < nk) a[r] = a[r] - cj[q] * ui[q];
         }
+        // unconditional stores: a row past i is clamped to i and rewrites
+        // col[i] with the value the real row i gives it
 #pragma unroll
-        for (int r = 0; r < 4; ++r)
-          if (j0 + r <= i) col[j0 + r] = a[r];
+        for (int r = 0; r < 4; ++r) col[j0 + r <= i ? j0 + r : i] = a[r];
       }
     }
     __syncthreads();
