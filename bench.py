@@ -233,7 +233,8 @@ def phase_profile(bmg, synth, ctx, args):
     cfgs = [synth.sampler_cfg(N, P, N_CAUSAL, types="A", seed=100 + c, chain=c, do_n_iter=total,
                               n_rao_burnin=4, thin=args.thin, delay_rejection=DELAY_REJECTION)
             for c in range(n)]
-    b = bmg.Batch([ctx], [0] * n, cfgs, dict(proposal=0, use_graph=0, max_model_size=MAX_MODEL_SIZE,
+    b = bmg.Batch([ctx], [0] * n, cfgs, dict(proposal=args.phase_proposal, use_graph=0,
+                                              max_model_size=MAX_MODEL_SIZE,
                                               trace_capacity=(total // args.thin + 16) * 200))
     b.setup()
     b.run(args.warmup * ips)
@@ -403,8 +404,10 @@ def main():
     ap.add_argument("--kernels-only", action="store_true",
                     help="time only the scan, band and copy kernels (for profiler runs)")
     ap.add_argument("--phase-profile", action="store_true",
-                    help="print each chain's per-phase device clocks over the timed steps of a plain batch "
+                    help="print each chain's per-phase device clocks over the timed steps of a batch "
                          "(eager, so no graph hides the segments) and exit")
+    ap.add_argument("--phase-proposal", type=int, default=0, choices=(0, 1, 2, 3),
+                    help="proposal of the --phase-profile batch (0: plain; 3: both learned proposals)")
     ap.add_argument("--thin", type=int, default=10,
                     help="thinning (part of the sampler's schedule: beta and sigma2 are drawn at "
                          "every thinned iteration, as in the reference)")

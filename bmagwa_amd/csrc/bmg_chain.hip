@@ -98,6 +98,13 @@ struct Smem {
   int32_t ex_x1[BMG_DR_MAX], ex_x2[BMG_DR_MAX];
   int8_t ex_t[BMG_DR_MAX];
   uint8_t ex_b2n[BMG_DR_MAX];
+  // undo journal of an add/remove move's sum-tree writes: the global
+  // entries it overwrote (leaf and level 1, in order) and a copy of the LDS
+  // levels taken when the move began
+  double tj_old[2 * BMG_MAXMS][2];
+  uint64_t tj_ind[2 * BMG_MAXMS];        // leaf index, tree in bit 63
+  double tj_tlev[2][TLEV_LDS];
+  double bq[BMG_MAXMS];                  // proposal weights of the reverse move
   double red[8];
   long long redi[8];
   double bcd[4];
@@ -118,6 +125,8 @@ struct Reg {
   double tot_add, tot_rem;
   int err;
   int learn_on;
+  int tj_on, tj_n;                       // journal active, entries
+  double tj_tot[2];                      // tree totals when the move began
 };
 
 template <int KMAX>
@@ -747,6 +756,13 @@ __device__ __forceinline__ void tree_set(Ctx<KMAX>& C, int tr, uint64_t i, int z
     uint64_t g1 = i >> 6;
     double c0 = g0[C.D.lev_off[0] + 64 * g1 + lane];
     double c1 = C.D.top >= 1 ? g0[C.D.lev_off[1] + 64 * (g1 >> 6) + lane] : 0.0;
+    if (C.R.tj_on) {
+      const double o0 = __shfl(c0, (int)(i & 63), 64), o1 = __shfl(c1, (int)(g1 & 63), 64);
+      if (lane == 0) {
+        C.S.tj_old[C.R.tj_n][0] = o0; C.S.tj_old[C.R.tj_n][1] = o1;
+        C.S.tj_ind[C.R.tj_n] = i | ((uint64_t)tr << 63);
+      }
+    }
     if (lane == (int)(i & 63)) c0 = nv;
     double s = wave_scan(c0);
     double gs = __shfl(s, 63, 64);
@@ -777,6 +793,125 @@ __device__ __forceinline__ void tree_set(Ctx<KMAX>& C, int tr, uint64_t i, int z
   total = C.S.bcd[1];
   __syncthreads();
   if (tr == 0) C.R.tot_add = total; else C.R.tot_rem = total;
+  if (C.R.tj_on) C.R.tj_n++;
+}
+
+// Pick a leaf with probability w_i / total and zero it (a tree_pick then a
+// tree_set(zero)), reusing the groups the descent read: the path's sums are
+// recomputed from them in tree_set's order, so the tree ends up exactly as
+// after the two calls.  *w_out: the picked leaf's weight.
+template <int KMAX>
+__device__ __forceinline__ uint64_t tree_pick_zero(Ctx<KMAX>& C, int tr, double u, double* w_out) {
+  double* g0 = tr == 0 ? C.A.tree_add : C.A.tree_rem;
+  if (wave_id() == 0) {
+    const int lane = lane_id();
+    double total = tr == 0 ? C.R.tot_add : C.R.tot_rem;
+    double r = u * total, carry = 0.0;
+    uint64_t g = 0;
+    long long res = 0;
+    double cl[BMG_TREE_LEV];             // this lane's value in the group read at each level
+#pragma unroll
+    for (int h = 0; h < BMG_TREE_LEV; ++h) cl[h] = 0.0;
+    for (int h = C.D.top; h >= 0; --h) {
+      double c = tree_val(C, tr, h, 64 * g + lane);
+#pragma unroll
+      for (int q = 0; q < BMG_TREE_LEV; ++q) if (q == h) cl[q] = c;
+      double s = wave_scan(c);
+      unsigned long long m1 = __ballot(c > 0.0 && r < carry + s);
+      unsigned long long nz = __ballot(c > 0.0);
+      int pick = m1 ? (__ffsll((long long)m1) - 1) : (nz ? 63 - __clzll((long long)nz) : -1);
+      if (pick < 0) { res = -1; break; }
+      double sp = __shfl(s, pick > 0 ? pick - 1 : 0, 64);
+      if (pick > 0) carry = carry + sp;
+      g = 64 * g + (uint64_t)pick;
+      res = (long long)g;
+    }
+    if (res >= 0) {
+      const uint64_t i = (uint64_t)res;
+      const uint64_t g1 = i >> 6;
+      const double wv = __shfl(cl[0], (int)(i & 63), 64);
+      if (C.R.tj_on) {
+        const double o1 = C.D.top >= 1 ? __shfl(cl[1], (int)(g1 & 63), 64) : 0.0;
+        if (lane == 0) {
+          C.S.tj_old[C.R.tj_n][0] = wv; C.S.tj_old[C.R.tj_n][1] = o1;
+          C.S.tj_ind[C.R.tj_n] = i | ((uint64_t)tr << 63);
+        }
+      }
+      // tree_set(zero) on the groups already read
+      double c0 = cl[0];
+      if (lane == (int)(i & 63)) c0 = 0.0;
+      double s = wave_scan(c0);
+      double gs = __shfl(s, 63, 64);
+      if (lane == 0) g0[C.D.lev_off[0] + i] = 0.0;
+      if (C.D.top >= 1) {
+        if (lane == 0) g0[C.D.lev_off[1] + g1] = gs;
+        double c1 = cl[1];
+        if (lane == (int)(g1 & 63)) c1 = gs;
+        s = wave_scan(c1);
+        gs = __shfl(s, 63, 64);
+        uint64_t gi = g1 >> 6;
+        for (int h = 2; h <= C.D.top; ++h) {
+          double* lv = C.S.tlev[tr] + (C.D.lev_off[h] - C.D.lev_off[2]);
+          if (lane == 0) lv[gi] = gs;
+          double c = 0.0;
+#pragma unroll
+          for (int q = 2; q < BMG_TREE_LEV; ++q) if (q == h) c = cl[q];
+          if (lane == (int)(gi & 63)) c = gs;
+          s = wave_scan(c);
+          gs = __shfl(s, 63, 64);
+          gi = gi >> 6;
+        }
+      }
+      if (lane == 0) { C.S.bcd[1] = gs; C.S.bcd[2] = wv; }
+    }
+    if (lane == 0) C.S.bci[1] = res;
+  }
+  __syncthreads();
+  long long r = C.S.bci[1];
+  const double total = C.S.bcd[1], wv = C.S.bcd[2];
+  __syncthreads();
+  if (r < 0) { C.R.err |= ERR_TREE; *w_out = 1.0; return 0; }
+  if (tr == 0) C.R.tot_add = total; else C.R.tot_rem = total;
+  if (C.R.tj_on) C.R.tj_n++;
+  *w_out = wv;
+  return (uint64_t)r;
+}
+
+// start journalling the sum-tree writes of a move (all threads)
+template <int KMAX>
+__device__ __forceinline__ void journal_begin(Ctx<KMAX>& C) {
+  const int nl = C.D.top >= 2 ? (int)(C.D.lev_off[C.D.top] + C.D.lev_len[C.D.top] - C.D.lev_off[2]) : 0;
+  for (int e = threadIdx.x; e < nl; e += BMG_BLOCK) {
+    C.S.tj_tlev[0][e] = C.S.tlev[0][e];
+    C.S.tj_tlev[1][e] = C.S.tlev[1][e];
+  }
+  __syncthreads();
+  C.R.tj_on = 1; C.R.tj_n = 0;
+  C.R.tj_tot[0] = C.R.tot_add; C.R.tj_tot[1] = C.R.tot_rem;
+}
+
+// put the trees back as they were when the journal began (all threads):
+// the LDS levels from their copy, the global entries from the journal in
+// reverse order, the totals
+template <int KMAX>
+__device__ __forceinline__ void journal_restore(Ctx<KMAX>& C) {
+  const int nl = C.D.top >= 2 ? (int)(C.D.lev_off[C.D.top] + C.D.lev_len[C.D.top] - C.D.lev_off[2]) : 0;
+  for (int e = threadIdx.x; e < nl; e += BMG_BLOCK) {
+    C.S.tlev[0][e] = C.S.tj_tlev[0][e];
+    C.S.tlev[1][e] = C.S.tj_tlev[1][e];
+  }
+  if (threadIdx.x == 0) {
+    for (int k = C.R.tj_n - 1; k >= 0; --k) {
+      const uint64_t v = C.S.tj_ind[k];
+      const uint64_t i = v & ~(1ull << 63);
+      double* g0 = (v >> 63) ? C.A.tree_rem : C.A.tree_add;
+      g0[C.D.lev_off[0] + i] = C.S.tj_old[k][0];
+      if (C.D.top >= 1) g0[C.D.lev_off[1] + (i >> 6)] = C.S.tj_old[k][1];
+    }
+  }
+  __syncthreads();
+  C.R.tot_add = C.R.tj_tot[0]; C.R.tot_rem = C.R.tj_tot[1];
+  C.R.tj_on = 0; C.R.tj_n = 0;
 }
 
 // -------------------------------------------------------------- model -----
@@ -1070,9 +1205,14 @@ __device__ __forceinline__ void compute_log_likelihood(Ctx<KMAX>& C) {
       double v[8];
 #pragma unroll
       for (int u = 0; u < 8; ++u) v[u] = row[S.slot[i0 + u <= j ? i0 + u : j]];   // past j: a valid slot
+      const double dj = S.d[j];
+      // unconditional stores: a row past j is clamped to j and rewrites the
+      // diagonal with the value row j gives it
 #pragma unroll
-      for (int u = 0; u < 8; ++u)
-        if (i0 + u <= j) col[i0 + u] = i0 + u == j ? v[u] + S.d[j] : v[u];
+      for (int u = 0; u < 8; ++u) {
+        const int i = i0 + u <= j ? i0 + u : j;
+        col[i] = i == j ? v[u] + dj : v[u];
+      }
     }
   }
   double ls = 0.0;
@@ -2031,6 +2171,9 @@ __device__ __forceinline__ int move_addrem(Ctx<KMAX>& C, double* adapt) {
   int ms = sample_naive(adapt, max_ms, rng_u01(R.rng)) + 1;
   R.movesize = ms;
   double lqf = 0, lqb = 0, lmpc = 0;
+  // the move's tree writes are journalled, so a rejection puts the trees
+  // back without recomputing their sums
+  journal_begin(C);
   // prepare_addrem (src/sampler.hpp:1126-1165)
   uint64_t max_adds = C.D.p - R.c_nloci, max_rems = R.c_nloci;
   for (int i = 0; i < ms; ++i) {
@@ -2040,16 +2183,17 @@ __device__ __forceinline__ int move_addrem(Ctx<KMAX>& C, double* adapt) {
     else { lqf += LOG_HALF; add = rng_u01(R.rng) < 0.5; }
     double u = rng_u01(R.rng);
     uint64_t ind;
+    double wv;
     if (add) {
       --max_adds;
-      ind = tree_pick(C, 0, u);
-      lqf += log(C.A.q_add[ind]) - log(R.tot_add);
-      tree_set(C, 0, ind, 1);
+      const double tot = R.tot_add;
+      ind = tree_pick_zero(C, 0, u, &wv);   // wv = q_add[ind]
+      lqf += log(wv) - log(tot);
     } else {
       --max_rems;
-      ind = tree_pick(C, 1, u);
-      lqf += log(C.A.q_rem[ind]) - log(R.tot_rem);
-      tree_set(C, 1, ind, 1);
+      const double tot = R.tot_rem;
+      ind = tree_pick_zero(C, 1, u, &wv);   // wv = q_rem[ind]
+      lqf += log(wv) - log(tot);
     }
     if (threadIdx.x == 0) {
       S.mi_isadd[i] = (uint8_t)add; S.mi[i] = ind; S.mi_map[i] = add ? (int64_t)ind : -1;
@@ -2079,28 +2223,41 @@ __device__ __forceinline__ int move_addrem(Ctx<KMAX>& C, double* adapt) {
     prepare_add_new_term(C, d2);
     add_term(C, (uint32_t)ind, 0, d2);
   }
-  // backward_prepare_addrem (src/sampler.hpp:1168-1202)
+  // backward_prepare_addrem (src/sampler.hpp:1168-1202); the reverse
+  // move's weights are loaded together first (thread i loads step i's)
   {
-    uint64_t madd = C.D.p - R.nloci, mrem = R.nloci;
-    double w_add = R.tot_add, w_rem = R.tot_rem;
-    int lr = ms - 1;
-    for (int i = 0; i < ms; ++i) {
-      if (madd > 0 && mrem > 0) lqb += LOG_HALF;
+    if ((int)threadIdx.x < ms) {
+      const int i = (int)threadIdx.x;
       uint64_t ind;
       if (S.mi_isadd[i]) {
-        --mrem;
-        while (!S.mi_isadd[lr]) --lr;
+        int k = 0;   // step i is the k-th addition; it pairs with the k-th from last
+        for (int q = 0; q < i; ++q) k += S.mi_isadd[q];
+        int lr = ms - 1, seen = 0;
+        for (;; --lr) if (S.mi_isadd[lr] && seen++ == k) break;
         ind = (uint64_t)S.mi_map[lr];
-        --lr;
-        lqb += log(C.A.q_rem[ind]) - log(w_rem);
-        w_rem -= C.A.q_rem[ind];
+        S.bq[i] = C.A.q_rem[ind];
       } else {
-        --madd;
         ind = (uint64_t)S.mi_map[i];
-        lqb += log(C.A.q_add[ind]) - log(w_add);
-        w_add -= C.A.q_add[ind];
+        S.bq[i] = C.A.q_add[ind];
       }
     }
+    __syncthreads();
+    uint64_t madd = C.D.p - R.nloci, mrem = R.nloci;
+    double w_add = R.tot_add, w_rem = R.tot_rem;
+    for (int i = 0; i < ms; ++i) {
+      if (madd > 0 && mrem > 0) lqb += LOG_HALF;
+      const double q = S.bq[i];
+      if (S.mi_isadd[i]) {
+        --mrem;
+        lqb += log(q) - log(w_rem);
+        w_rem -= q;
+      } else {
+        --madd;
+        lqb += log(q) - log(w_add);
+        w_add -= q;
+      }
+    }
+    __syncthreads();
   }
   double log_r = lqb - lqf;
   log_r += lmpc + R.ll - R.c_ll;
@@ -2111,19 +2268,21 @@ __device__ __forceinline__ int move_addrem(Ctx<KMAX>& C, double* adapt) {
     r_ms_n[ms - 1] += 1.0;
   }
   if (log_r >= 0 || log(rng_u01(R.rng)) <= log_r) {
+    R.tj_on = 0;
     commit(C);
     if (K.dr != 0 && threadIdx.x == 0) r_ms_sum[ms - 1] += 1.0;
     return ms;
   }
   // a proposal past the compiled capacity has no second stage
   if (ms <= K.dr && ms > 1 && !(R.err & ERR_CAPACITY)) {
+    R.tj_on = 0;
     const uint64_t t0 = wall_clock64();
     const int moved = dr_addrem(C, ms, adapt);
     if (threadIdx.x == 0 && C.prof) C.prof[18] += wall_clock64() - t0;
     return moved;
   }
   restore(C);
-  undo_dd(C, ms);
+  journal_restore(C);
   return 0;
 }
 
@@ -2463,6 +2622,7 @@ __device__ __forceinline__ void load_chain(Ctx<KMAX>& C, const ChainScal& CS) {
   R.tot_add = CS.tot_add; R.tot_rem = CS.tot_rem;
   R.err = CS.err;
   R.learn_on = CS.learn_on && C.D.lw != nullptr && C.D.lH >= 1 && C.D.lH <= BMG_LH_MAX;
+  R.tj_on = 0; R.tj_n = 0; R.tj_tot[0] = R.tj_tot[1] = 0.0;
   const double* vec = C.A.vec;
   const int32_t* idx = C.A.idx;
   for (int e = threadIdx.x; e < P(R.cols); e += BMG_BLOCK) S.L[e] = C.A.Lcur[e];
