@@ -228,23 +228,28 @@ __device__ __forceinline__ void planes_c(uint32_t w, int coding, uint32_t& p1, u
 }
 
 // S.work[c] = x_c^T x_new for c in [0, cols], column `cols` being the new
-// one itself; column 0 is the constant.  Exact integer dots: a word's value
-// planes give a value word (the coded genotype, 0..2, in each 2-bit
-// field), whose even and odd fields, masked into nibbles, meet the new
-// column's in two 8-way unsigned 4-bit dot products (v_dot8_u32_u4).
+// one itself; column 0 is the constant.  Exact integer dots: the slot
+// copies of the model's columns hold value words (the coded genotype, 0..2,
+// in each 2-bit field), whose even fields (masked in place) and odd fields
+// (the rest, four times their value) meet the new column's in two 8-way
+// unsigned 4-bit dot products (v_dot8_u32_u4); the odd sum is divided by
+// four at the end.  The constant column and the new column's own dot come
+// from the new column's nibbles alone.
 //
 // A lane owns one 16-word chunk of the individuals and a group of columns:
-// it decodes the new column's chunk once, then streams its columns' chunks
-// (four columns' words are loaded together, then summed);
-// the chunk sums meet in LDS integer accumulators (exact, so the order is
-// free).  W / 16 lanes cover a column, so 256 / (W / 16) column groups run
-// side by side.  CODING >= 0: every column has that coding (one effect type).
+// it decodes the new column's chunk once, then streams its columns' chunks,
+// four columns' words in flight at a time, each column's loads issued as
+// soon as the previous column in its register set is summed; the chunk
+// sums meet in LDS integer accumulators (exact, so the order is free).
+// W / 16 lanes cover a column, so 256 / (W / 16) column groups run side by
+// side.  CODING: the new column's coding.
 template <int KMAX, int CODING>
-__device__ __forceinline__ void column_dots(Ctx<KMAX>& C, uint64_t snp_new, int coding_new, int cols) {
+__device__ __forceinline__ void column_dots(Ctx<KMAX>& C, uint64_t snp_new, int cols) {
   const uint32_t W = C.D.W32;                 // a multiple of 32 words
   const uint32_t* __restrict__ wn = C.D.g + snp_new * W;
   const uint32_t* __restrict__ xcol = C.A.xcol;
   Smem<KMAX>& S = C.S;
+  const int me = C.D.me;
   const uint32_t cpc = W / 16;                // chunks per column
   for (uint32_t ch0 = 0; ch0 < cpc; ch0 += BMG_BLOCK) {
     const uint32_t span = min(cpc - ch0, (uint32_t)BMG_BLOCK);
@@ -253,6 +258,7 @@ __device__ __forceinline__ void column_dots(Ctx<KMAX>& C, uint64_t snp_new, int 
     const uint32_t w0 = (ch0 + threadIdx.x % span) * 16;
     if (grp < groups) {
       uint32_t ne[16], no[16];   // the new column's even / odd fields as nibbles
+      uint32_t s_one = 0, s_self = 0;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const uint4 v = *(const uint4*)(wn + w0 + 4 * q);
@@ -260,48 +266,54 @@ __device__ __forceinline__ void column_dots(Ctx<KMAX>& C, uint64_t snp_new, int 
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
           uint32_t p1, p2;
-          planes_c<CODING>(xs[u], coding_new, p1, p2);
+          planes_c<CODING>(xs[u], CODING, p1, p2);
           const uint32_t vw = p1 + p2;
-          ne[4 * q + u] = vw & 0x33333333u;
-          no[4 * q + u] = (vw >> 2) & 0x33333333u;
+          const int k = 4 * q + u;
+          ne[k] = vw & 0x33333333u;
+          no[k] = (vw >> 2) & 0x33333333u;
+          s_one = __builtin_amdgcn_udot8(0x11111111u, ne[k], s_one, false);
+          s_one = __builtin_amdgcn_udot8(0x11111111u, no[k], s_one, false);
+          s_self = __builtin_amdgcn_udot8(ne[k], ne[k], s_self, false);
+          s_self = __builtin_amdgcn_udot8(no[k], no[k], s_self, false);
         }
       }
-      const int me = C.D.me;
-      auto col_ptr = [&](int c) -> const uint32_t* {
-        return (c >= cols || c < me) ? wn : xcol + (size_t)S.slot[c] * W;
+      // column of round r, register set u: grp + (4 r + u) groups; every
+      // group runs the rounds of group 0, the one with the most columns
+      const int rounds = (cols / groups + 1 + 3) / 4;
+      auto src = [&](int cu) -> const uint32_t* {
+        const int cc = (cu >= me && cu < cols) ? cu : 0;
+        return (cu >= me && cu < cols) ? xcol + (size_t)S.slot[cc] * W : wn;
       };
-      // four columns of the group per round: their 16 loads are in flight
-      // together, so a lane waits on memory once per four columns
-      for (int c = grp; c <= cols; c += 4 * groups) {
-        uint4 cur[4][4];
+      uint4 cur[4][4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t* pc = src(grp + u * groups);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) cur[u][q] = *(const uint4*)(pc + w0 + 4 * q);
+      }
+      for (int r = 0; r < rounds; ++r) {
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-          const int cu = c + u * groups;
-          const uint32_t* pc = cu <= cols ? col_ptr(cu) : wn;
-#pragma unroll
-          for (int q = 0; q < 4; ++q) cur[u][q] = *(const uint4*)(pc + w0 + 4 * q);
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int cu = c + u * groups;
-          if (cu > cols) break;
-          const int cc = (cu >= cols || cu == 0) ? coding_new : (int)S.xtype[cu];
-          uint32_t sum = 0;
+          const int cu = grp + (4 * r + u) * groups;
+          uint32_t se = 0, so = 0;
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
             const uint32_t xs[4] = {cur[u][q].x, cur[u][q].y, cur[u][q].z, cur[u][q].w};
 #pragma unroll
             for (int k4 = 0; k4 < 4; ++k4) {
-              uint32_t a1, a2;
-              planes_c<CODING>(xs[k4], cc, a1, a2);
-              // column 0 is the constant: a 1 in every field
-              const uint32_t vw = cu == 0 ? BMG_EVEN : a1 + a2;
               const int k = 4 * q + k4;
-              sum = __builtin_amdgcn_udot8(vw & 0x33333333u, ne[k], sum, false);
-              sum = __builtin_amdgcn_udot8((vw >> 2) & 0x33333333u, no[k], sum, false);
+              const uint32_t e = xs[k4] & 0x33333333u;
+              se = __builtin_amdgcn_udot8(e, ne[k], se, false);
+              so = __builtin_amdgcn_udot8(xs[k4] ^ e, no[k], so, false);
             }
           }
-          if (cu == 0 || cu >= me) atomicAdd(&S.iacc[cu], (int)sum);   // covariates: cov_dots
+          // next round's column of this register set
+          const uint32_t* pn = src(cu + 4 * groups);
+#pragma unroll
+          for (int q = 0; q < 4; ++q) cur[u][q] = *(const uint4*)(pn + w0 + 4 * q);
+          const uint32_t sum = cu == 0 ? s_one : (cu == cols ? s_self : se + (so >> 2));
+          // covariate columns (0 < c < me): cov_dots
+          if (cu <= cols && (cu == 0 || cu >= me)) atomicAdd(&S.iacc[cu], (int)sum);
         }
       }
     }
@@ -332,17 +344,11 @@ __device__ __forceinline__ void new_column_dots(Ctx<KMAX>& C, uint64_t snp_new, 
   Smem<KMAX>& S = C.S;
   for (int c = threadIdx.x; c <= cols; c += BMG_BLOCK) S.iacc[c] = 0;
   __syncthreads();
-  // the sampler's columns share one coding unless its type is AH (the
-  // Model API may mix them)
-  if (!C.D.mixed && C.D.type != BMG_AH && coding_new == C.D.type) {
-    switch (coding_new) {
-      case 0: column_dots<KMAX, 0>(C, snp_new, coding_new, cols); break;
-      case 1: column_dots<KMAX, 1>(C, snp_new, coding_new, cols); break;
-      case 2: column_dots<KMAX, 2>(C, snp_new, coding_new, cols); break;
-      default: column_dots<KMAX, 3>(C, snp_new, coding_new, cols); break;
-    }
-  } else {
-    column_dots<KMAX, -1>(C, snp_new, coding_new, cols);
+  switch (coding_new) {
+    case 0: column_dots<KMAX, 0>(C, snp_new, cols); break;
+    case 1: column_dots<KMAX, 1>(C, snp_new, cols); break;
+    case 2: column_dots<KMAX, 2>(C, snp_new, cols); break;
+    default: column_dots<KMAX, 3>(C, snp_new, cols); break;
   }
   __syncthreads();
   for (int c = threadIdx.x; c <= cols; c += BMG_BLOCK) S.work[c] = (double)S.iacc[c];
@@ -361,109 +367,116 @@ __device__ __forceinline__ double readlane_d(double v, int src) {
 
 // x := U^-T x for x[0..m) in LDS (dtrsv Upper/Trans order).  Wave 0 only.
 // Lane l holds x[l + 64 t] in b[t] and the reciprocal of that row's
-// diagonal entry in rd[t] (all computed at once before the sweep); the
+// diagonal entry in dg[t] (all computed at once before the sweep); the
 // outer loop over t is unrolled so the register arrays are only ever
-// indexed by constants.  Row i: every lane scales its own entry and the
-// owner's product is read across lanes, so the serial chain per row is one
-// multiply, one cross-lane read and one fused multiply-subtract, with no
-// division and no LDS wait; the next row's coefficients are loaded while
-// this row's updates run.  Per element the subtractions come in row order,
-// as in the sequential loop; the reciprocal and the fused update round
+// indexed by constants.  Row i: the owner's scaled entry is read across
+// lanes and every lane subtracts its coefficient times it, so the serial
+// chain per row is one multiply, one cross-lane read and one fused
+// multiply-subtract, with no division.  The sweep has no branches and no
+// masks: every lane applies every row with the entry at its own column's
+// row i (a lane at or past its row reads another column's entry into a
+// value it no longer needs; its result was kept in xo when its row came),
+// and the coefficients are read two rows ahead from clamped addresses, so
+// no row waits on LDS.  Per element the subtractions come in row order, as
+// in the sequential loop; the reciprocal and the fused update round
 // differently from its divisions (by an ulp), which the log-likelihoods'
 // 1e-10 agreement with the oracle absorbs.
 template <int KMAX>
 __device__ __forceinline__ void trsv_ut_w0(Smem<KMAX>& S, double* x, int m) {
   constexpr int NT = (KMAX + 63) / 64;
+  constexpr int LIM = KMAX * (KMAX + 1) / 2 - 1;
   const int lane = lane_id();
-  double b[NT], dg[NT];
+  double b[NT], dg[NT], xo[NT];
+  int base[NT];
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
-    int j = lane + 64 * t;
+    const int j = lane + 64 * t;
     b[t] = j < m ? x[j] : 0.0;
     dg[t] = j < m ? 1.0 / S.L[P(j) + j] : 1.0;
+    base[t] = j < m ? P(j) : 0;
+    xo[t] = 0.0;
   }
+  auto ld = [&](int t, int i) -> double {
+    const int a = base[t] + i;
+    return S.L[a < LIM ? a : LIM];
+  };
 #pragma unroll
   for (int ti = 0; ti < NT; ++ti) {
     const int i0 = 64 * ti;
     const int iend = m < 64 * (ti + 1) ? m : 64 * (ti + 1);
     if (i0 >= iend) break;
-    double cf[NT];
+    double ca[NT], cb[NT];
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
-      int j = lane + 64 * t;
-      cf[t] = (t >= ti && j > i0 && j < m) ? S.L[P(j) + i0] : 0.0;
-    }
-    for (int i = i0; i < iend; ++i) {
+    for (int t = ti; t < NT; ++t) { ca[t] = ld(t, i0); cb[t] = ld(t, i0 + 1); }
+    auto row = [&](int i, double (&c)[NT]) {
       const int owner = i & 63;
       const double xi = readlane_d(b[ti] * dg[ti], owner);
-      double cn[NT];
+      xo[ti] = lane == owner ? xi : xo[ti];
 #pragma unroll
-      for (int t = 0; t < NT; ++t) {
-        int j = lane + 64 * t;
-        cn[t] = (t >= ti && j > i + 1 && j < m) ? S.L[P(j) + i + 1] : 0.0;
-      }
-      if (lane == owner) b[ti] = xi;
+      for (int t = ti; t < NT; ++t)
+        if (64 * t < m) b[t] = fma(-c[t], xi, b[t]);
 #pragma unroll
-      for (int t = ti; t < NT; ++t) {
-        int j = lane + 64 * t;
-        if (j > i && j < m) b[t] = fma(-cf[t], xi, b[t]);
-      }
-#pragma unroll
-      for (int t = 0; t < NT; ++t) cf[t] = cn[t];
-    }
+      for (int t = ti; t < NT; ++t) c[t] = ld(t, i + 2);
+    };
+    int i = i0;
+    for (; i + 1 < iend; i += 2) { row(i, ca); row(i + 1, cb); }
+    if (i < iend) row(i, ca);
   }
 #pragma unroll
-  for (int t = 0; t < NT; ++t) { int j = lane + 64 * t; if (j < m) x[j] = b[t]; }
+  for (int t = 0; t < NT; ++t) { const int j = lane + 64 * t; if (j < m) x[j] = xo[t]; }
 }
 
 // x := U^-1 x (dtrsv Upper/NoTrans, back substitution).  Wave 0 only.  Lane
-// l holds x[l + 64 t] and the reciprocal of that row's diagonal; column j's
-// entries for the next row are read from LDS while row j's updates run, so
-// the serial chain per row is a multiply, a cross-lane read and a fused
-// update (rounding within an ulp of the divide-and-subtract loop).
+// l holds x[l + 64 t] and the reciprocal of that row's diagonal; step j
+// reads the owner's scaled entry across lanes and every lane subtracts its
+// entry of column j (one column: consecutive addresses) times it, so the
+// serial chain per step is a multiply, a cross-lane read and a fused update
+// (rounding within an ulp of the divide-and-subtract loop).  As in the
+// forward sweep there are no branches or masks (a lane at or past its row
+// updates a value it no longer needs; its result is in xo) and the columns
+// are read two steps ahead.  A zero x_j leaves every entry as it was, as
+// the reference's skip of zero entries does.
 template <int KMAX>
 __device__ __forceinline__ void trsv_un_w0(Smem<KMAX>& S, double* x, int m) {
   constexpr int NT = (KMAX + 63) / 64;
+  constexpr int LIM = KMAX * (KMAX + 1) / 2 - 1;
   const int lane = lane_id();
-  double b[NT], rd[NT], cf[NT];
+  if (m <= 0) return;
+  double b[NT], rd[NT], xo[NT];
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
-    int j = lane + 64 * t;
+    const int j = lane + 64 * t;
     b[t] = j < m ? x[j] : 0.0;
     rd[t] = j < m ? 1.0 / S.L[P(j) + j] : 1.0;
+    xo[t] = 0.0;
   }
-  if (m <= 0) return;
-#pragma unroll
-  for (int q = 0; q < NT; ++q) {
-    int i = lane + 64 * q;
-    cf[q] = i < m - 1 ? S.L[P(m - 1) + i] : 0.0;
-  }
+  // entry (lane + 64 q) of column j, clamped into the factor
+  auto ld = [&](int q, int j) -> double {
+    const int a = (j > 0 ? P(j) : 0) + lane + 64 * q;
+    return S.L[a < LIM ? a : LIM];
+  };
 #pragma unroll
   for (int tj = NT - 1; tj >= 0; --tj) {
     const int jend = m < 64 * (tj + 1) ? m : 64 * (tj + 1);
-    for (int j = jend - 1; j >= 64 * tj; --j) {
+    if (64 * tj >= jend) continue;
+    double ca[NT], cb[NT];
+#pragma unroll
+    for (int q = 0; q <= tj; ++q) { ca[q] = ld(q, jend - 1); cb[q] = ld(q, jend - 2); }
+    auto step = [&](int j, double (&c)[NT]) {
       const int owner = j & 63;
-      double cn[NT];
-#pragma unroll
-      for (int q = 0; q < NT; ++q) {
-        int i = lane + 64 * q;
-        cn[q] = (j >= 1 && i < j - 1) ? S.L[P(j - 1) + i] : 0.0;
-      }
       const double xj = readlane_d(b[tj] * rd[tj], owner);
-      if (xj != 0.0) {
-        if (lane == owner) b[tj] = xj;
+      xo[tj] = lane == owner ? xj : xo[tj];
 #pragma unroll
-        for (int q = 0; q <= tj; ++q) {
-          int i = lane + 64 * q;
-          if (i < j) b[q] = fma(-xj, cf[q], b[q]);
-        }
-      }
+      for (int q = 0; q <= tj; ++q) b[q] = fma(-xj, c[q], b[q]);
 #pragma unroll
-      for (int q = 0; q < NT; ++q) cf[q] = cn[q];
-    }
+      for (int q = 0; q <= tj; ++q) c[q] = ld(q, j - 2);
+    };
+    int j = jend - 1;
+    for (; j - 1 >= 64 * tj; j -= 2) { step(j, ca); step(j - 1, cb); }
+    if (j >= 64 * tj) step(j, ca);
   }
 #pragma unroll
-  for (int t = 0; t < NT; ++t) { int j = lane + 64 * t; if (j < m) x[j] = b[t]; }
+  for (int t = 0; t < NT; ++t) { const int j = lane + 64 * t; if (j < m) x[j] = xo[t]; }
 }
 
 // netlib drotg (as the oracle restates it)
@@ -487,21 +500,131 @@ __device__ __forceinline__ void drotg(double& a, double& b, double& c, double& s
 
 // The rotation drotg generates (same sign convention: r takes the sign of
 // the larger input), without its overflow scaling, which a Cholesky factor's
-// entries never need: one square root and one division instead of four
-// divisions, so the downdate's serial chain per step is shorter.  c, s and r
-// agree with drotg's to rounding.
+// entries never need.  1 / r comes from the hardware reciprocal square root
+// of a^2 + b^2 refined by two Newton steps (each roughly doubles the correct
+// bits, ending within an ulp or two of the correctly rounded value), so the
+// downdate's serial chain per step is a handful of multiplies instead of a
+// square root and a division.  c, s and r agree with drotg's to rounding.
 __device__ __forceinline__ void rot_gen(double& a, double b, double& c, double& s) {
   const double roe = fabs(a) > fabs(b) ? a : b;
-  if (a == 0.0 && b == 0.0) { c = 1.0; s = 0.0; a = 0.0; return; }
-  double r = sqrt(fma(a, a, b * b));
-  r = roe >= 0.0 ? r : -r;
-  const double ri = 1.0 / r;
-  c = a * ri; s = b * ri;
-  a = r;
+  const double q = fma(a, a, b * b);
+  double y = __builtin_amdgcn_rsq(q);
+  const double h = 0.5 * q;
+  y = y * fma(-h * y, y, 1.5);
+  y = y * fma(-h * y, y, 1.5);
+  const double ri = q > 0.0 ? (roe >= 0.0 ? y : -y) : 0.0;
+  c = q > 0.0 ? a * ri : 1.0;
+  s = b * ri;
+  a = q * ri;
 }
 
 // Cholesky downdate removing column cr of the m x m factor (LINPACK dchex
 // job 2 + drop + sign fix, src/symmmatrix.cpp:166-195).  Logical column jl >=
+// cr lives in physical column jl + 1 until it is written back; rotation j is
+// generated from logical column j's rows (j, j + 1) and applied to every
+// later column at once (per element the same sequence as the sequential
+// loop).  The two rows a rotation touches stay in registers: lane jl % 64
+// keeps rows (j, j + 1) of its columns, reads rows two steps ahead from LDS
+// (rows below j + 1 are not rotated yet) and stores only the finished row,
+// and the owner of column j hands the rotation's inputs to all lanes with a
+// cross-lane read, so no step waits on an LDS round trip.
+//
+// Finished rows go straight to their place in the compacted factor
+// (physical column jl, whose entries at and past row cr were read two or
+// more steps earlier), with the sign fix folded in: a row whose diagonal
+// comes out negative is stored negated, which is what the separate pass
+// over the finished rows does.  Meanwhile wave 1 moves the rows above cr of
+// the later columns down by one column, in ascending order of destination
+// (every source lies above every earlier destination, so one wave's
+// batches never overwrite what a later batch reads); the two touch disjoint
+// entries.
+//
+// v (U^T v = X^T y) goes through the same rotations and sign fix as the
+// factor's rows, so after the removal it solves the new factor's system
+// without a second triangular solve (the reference re-solves,
+// src/model.hpp:530-545; the two agree to rounding).
+template <int KMAX>
+__device__ __forceinline__ void chol_downdate_lds(Smem<KMAX>& S, int m, int cr) {
+  constexpr int NT = (KMAX + 63) / 64;
+  constexpr int LIM = KMAX * (KMAX + 1) / 2 - 1;
+  const int lane = lane_id();
+  if (wave_id() == 0) {
+    double vt = S.v[cr], vb = S.v[cr + 1];   // rows (j, j + 1) of v, all lanes
+    double top[NT], nA[NT], nB[NT];
+    int pb[NT], lb[NT];   // physical and logical column offsets
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const int jl = lane + 64 * t;
+      pb[t] = P(jl + 1);
+      lb[t] = P(jl);
+      top[t] = S.L[min(pb[t] + cr, LIM)];
+      nA[t] = S.L[min(pb[t] + cr + 1, LIM)];
+      nB[t] = S.L[min(pb[t] + cr + 2, LIM)];
+    }
+#pragma unroll
+    for (int ti = 0; ti < NT; ++ti) {
+      const int jbeg = cr > 64 * ti ? cr : 64 * ti;
+      const int jend = (m - 1) < 64 * (ti + 1) ? (m - 1) : 64 * (ti + 1);
+      // step j: n1 holds row j + 1 of each column and takes row j + 3
+      auto step = [&](int j, double (&n1)[NT]) {
+        const int owner = j & 63;
+        // every lane generates the rotation from the same two values
+        double a = readlane_d(top[ti], owner), b = readlane_d(n1[ti], owner);
+        const double vbn = S.v[j + 2 < m ? j + 2 : m - 1];   // next row of v, read ahead
+        double cc, ss;
+        rot_gen(a, b, cc, ss);
+        const double sg = a < 0.0 ? -1.0 : 1.0;   // sign fix of row j
+        if (lane == owner) S.L[lb[ti] + j] = a * sg;
+        {
+          const double tv = cc * vt + ss * vb;
+          vt = cc * vb - ss * vt;
+          if (lane == 0) S.v[j] = tv * sg;
+          vb = vbn;
+        }
+#pragma unroll
+        for (int t = ti; t < NT; ++t) {
+          if (64 * t <= m - 2) {
+            const int jl = lane + 64 * t;
+            const double x1 = top[t], x2 = n1[t];
+            const double tt = cc * x1 + ss * x2;
+            const bool on = jl > j && jl <= m - 2;
+            top[t] = on ? cc * x2 - ss * x1 : top[t];
+            if (on) S.L[lb[t] + j] = tt * sg;
+            n1[t] = S.L[min(pb[t] + j + 3, LIM)];
+          }
+        }
+      };
+      int j = jbeg;
+      for (; j + 1 < jend; j += 2) { step(j, nA); step(j + 1, nB); }
+      if (j < jend) {
+        step(j, nA);
+#pragma unroll
+        for (int t = 0; t < NT; ++t) { const double tmp = nA[t]; nA[t] = nB[t]; nB[t] = tmp; }
+      }
+    }
+  } else if (wave_id() == 1) {
+    // rows [0, cr) of logical columns [cr, m - 2]: physical column jl + 1 ->
+    // jl, element k = (jl - cr) * cr + i in ascending order
+    const int total = (m - 1 - cr) * cr;
+    const float rcr = cr > 0 ? 1.0f / (float)cr : 0.0f;
+    for (int k0 = 0; k0 < total; k0 += 64) {
+      const int k = k0 + lane;
+      if (k < total) {
+        int q = (int)((float)k * rcr);
+        if (q * cr > k) --q;
+        if ((q + 1) * cr <= k) ++q;
+        const int jl = cr + q, i = k - q * cr;
+        S.L[P(jl) + i] = S.L[P(jl + 1) + i];
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// The same downdate for a factor in global memory (capacity above the LDS
+// factor's): rows rotated in place, then the sign fix and the compaction as
+// separate passes (LINPACK dchex job 2 + drop + sign fix,
+// src/symmmatrix.cpp:166-195).  Logical column jl >=
 // cr lives in physical column jl + 1 until the final compaction; rotation j
 // is generated from logical column j's rows (j, j + 1) and applied to every
 // later column at once (per element the same sequence as the sequential
@@ -516,7 +639,7 @@ __device__ __forceinline__ void rot_gen(double& a, double b, double& c, double& 
 // without a second triangular solve (the reference re-solves,
 // src/model.hpp:530-545; the two agree to rounding).
 template <int KMAX>
-__device__ __forceinline__ void chol_downdate(Smem<KMAX>& S, int m, int cr) {
+__device__ __forceinline__ void chol_downdate_glb(Smem<KMAX>& S, int m, int cr) {
   constexpr int NT = (KMAX + 63) / 64;
   if (wave_id() == 0) {
     const int lane = lane_id();
@@ -610,6 +733,12 @@ __device__ __forceinline__ void chol_downdate(Smem<KMAX>& S, int m, int cr) {
     }
     __syncthreads();
   }
+}
+
+template <int KMAX>
+__device__ __forceinline__ void chol_downdate(Smem<KMAX>& S, int m, int cr) {
+  if constexpr (KMAX <= KMAX_LDS) chol_downdate_lds(S, m, cr);
+  else chol_downdate_glb(S, m, cr);
 }
 
 // Full factorisation of S.L (m x m, in place), right-looking in panels of
@@ -996,9 +1125,15 @@ __device__ __forceinline__ void update_on_add(Ctx<KMAX>& C, uint32_t ind, int mo
   R.cols = col + 1;
   if (R.lmin > col) R.lmin = col;
   {
+    // the slot keeps the column's value words (coded genotype in each 2-bit
+    // field), which the dot products and X beta read without the coding
     const uint32_t* src = C.D.g + (uint64_t)ind * C.D.W32;
     uint32_t* dst = C.A.xcol + (size_t)slot * C.D.W32;
-    for (uint32_t k = threadIdx.x; k < C.D.W32; k += BMG_BLOCK) dst[k] = src[k];
+    for (uint32_t k = threadIdx.x; k < C.D.W32; k += BMG_BLOCK) {
+      uint32_t p1, p2;
+      planes(src[k], coding, p1, p2);
+      dst[k] = p1 + p2;
+    }
   }
   const uint64_t ta = wall_clock64();
   double xy = snp_dot_y(C, ind, coding);
@@ -1245,13 +1380,33 @@ __device__ __forceinline__ void compute_log_likelihood(Ctx<KMAX>& C) {
 // Accept (*current = *new) or reject (*new = *current) for the single
 // working copy: only factor columns >= lmin differ from the stored current
 // factor.
+
+// dst[e] = src[e] for e in [e0, e1) (all threads): eight elements a thread
+// are read before any is written, so the reads of one pass are in flight
+// together instead of one round trip per element
+__device__ __forceinline__ void copy_factor(double* __restrict__ dst, const double* __restrict__ src, int e0,
+                                            int e1) {
+  for (int base = e0; base < e1; base += BMG_BLOCK * 8) {
+    double t[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int e = base + q * BMG_BLOCK + (int)threadIdx.x;
+      t[q] = e < e1 ? src[e] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int e = base + q * BMG_BLOCK + (int)threadIdx.x;
+      if (e < e1) dst[e] = t[q];
+    }
+  }
+}
+
 template <int KMAX>
 __device__ __forceinline__ void commit(Ctx<KMAX>& C) {
   Smem<KMAX>& S = C.S;
   Reg& R = C.R;
   __syncthreads();
-  for (int e = P(R.lmin < R.cols ? R.lmin : R.cols) + threadIdx.x; e < P(R.cols); e += BMG_BLOCK)
-    C.A.Lcur[e] = S.L[e];
+  copy_factor(C.A.Lcur, S.L, P(R.lmin < R.cols ? R.lmin : R.cols), P(R.cols));
   for (int i = threadIdx.x; i < R.cols; i += BMG_BLOCK) {
     S.c_xy[i] = S.xy[i]; S.c_v[i] = S.v[i]; S.c_d[i] = S.d[i];
     S.c_slot[i] = S.slot[i]; S.c_snp[i] = S.snp[i]; S.c_xtype[i] = S.xtype[i];
@@ -1278,7 +1433,7 @@ __device__ __forceinline__ void restore(Ctx<KMAX>& C) {
   Reg& R = C.R;
   __syncthreads();
   int lm = R.lmin < R.c_cols ? R.lmin : R.c_cols;
-  for (int e = P(lm) + threadIdx.x; e < P(R.c_cols); e += BMG_BLOCK) S.L[e] = C.A.Lcur[e];
+  copy_factor(S.L, C.A.Lcur, P(lm), P(R.c_cols));
   for (int i = threadIdx.x; i < R.c_cols; i += BMG_BLOCK) {
     S.xy[i] = S.c_xy[i]; S.v[i] = S.c_v[i]; S.d[i] = S.c_d[i];
     S.slot[i] = S.c_slot[i]; S.snp[i] = S.c_snp[i]; S.xtype[i] = S.c_xtype[i];
@@ -1375,19 +1530,7 @@ __device__ __forceinline__ void sample_beta_sigma2(Ctx<KMAX>& C) {
   __syncthreads();
 }
 
-// value of a 2-bit genotype code times t under a coding known at compile
-// time: 0, t or 2t, selected (t * (double)value is exactly one of these, and
-// adding +0 leaves a sum that is never -0 unchanged, so the sums are the
-// multiply-add loop's bit for bit)
-template <int CODING>
-__device__ __forceinline__ double coded_times(uint32_t code, double t, double t2) {
-  if (CODING == 0) return code == 3u ? t2 : (code == 2u ? t : 0.0);
-  if (CODING == 1) return code == 2u ? t : 0.0;
-  if (CODING == 2) return code >= 2u ? t : 0.0;
-  return code == 3u ? t : 0.0;
-}
-
-template <int KMAX, int CODING>
+template <int KMAX>
 __device__ __forceinline__ void xbeta_impl(Ctx<KMAX>& C, const double* coef, int L, double* out8,
                                            int write_yhat) {
   Smem<KMAX>& S = C.S;
@@ -1416,19 +1559,10 @@ __device__ __forceinline__ void xbeta_impl(Ctx<KMAX>& C, const double* coef, int
         if (c < L) {
           const double t = coef[c];
           if (t != 0.0) {
-            if (CODING >= 0) {
-              const double t2 = t + t;
+            // t * value is exact (value 0, 1 or 2), so the fused form
+            // rounds the sum as the multiply-then-add loop does
 #pragma unroll
-              for (int q = 0; q < 16; ++q)
-                acc[q] = acc[q] + coded_times<CODING>((wd[u] >> (2 * q)) & 3u, t, t2);
-            } else {
-              const int xt = S.xtype[c];
-#pragma unroll
-              for (int q = 0; q < 16; ++q) {
-                int iv = code_ival((wd[u] >> (2 * q)) & 3u, xt);
-                acc[q] = acc[q] + t * (double)iv;
-              }
-            }
+            for (int q = 0; q < 16; ++q) acc[q] = fma(t, (double)((wd[u] >> (2 * q)) & 3u), acc[q]);
           }
         }
       }
@@ -1474,17 +1608,7 @@ __device__ __forceinline__ void xbeta_impl(Ctx<KMAX>& C, const double* coef, int
 // order, as the per-individual loop does.
 template <int KMAX>
 __device__ __forceinline__ void xcoef(Ctx<KMAX>& C, const double* coef, int L, double* out8, int write_yhat) {
-  // the sampler's columns share one coding unless its type is AH (the
-  // Model API may mix them)
-  if (!C.D.mixed && C.D.type != BMG_AH) {
-    switch (C.D.type) {
-      case 0: xbeta_impl<KMAX, 0>(C, coef, L, out8, write_yhat); return;
-      case 1: xbeta_impl<KMAX, 1>(C, coef, L, out8, write_yhat); return;
-      case 2: xbeta_impl<KMAX, 2>(C, coef, L, out8, write_yhat); return;
-      default: xbeta_impl<KMAX, 3>(C, coef, L, out8, write_yhat); return;
-    }
-  }
-  xbeta_impl<KMAX, -1>(C, coef, L, out8, write_yhat);
+  xbeta_impl<KMAX>(C, coef, L, out8, write_yhat);
 }
 
 template <int KMAX>
