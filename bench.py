@@ -259,7 +259,9 @@ def phase_profile(bmg, synth, ctx, args):
                               beta_sigma2=us("beta_sigma2", it), pve_thin=us("pve_thin", it),
                               addrem_moves=us("addrem", it), statechange=us("statechange", it),
                               swap=us("swap", it), second_stage=us("dr", it),
-                              second_stage_scoring=us("dr_score", it))), flush=True)
+                              second_stage_scoring=us("dr_score", it), addrem_picks=us("ar_picks", it),
+                              addrem_apply=us("ar_apply", it), addrem_reverse=us("ar_reverse", it),
+                              addrem_decide=us("ar_decide", it))), flush=True)
     # a launch lasts as long as its slowest chain: the share of the chains'
     # workgroup-time spent waiting for it is the launch's idle fraction
     print(json.dumps(dict(kernel_us_per_iter=round((t1["chain_ms"] - t0["chain_ms"]) / it * 1e3, 1),

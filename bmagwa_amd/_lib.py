@@ -727,11 +727,12 @@ class Batch:
         return out
 
     def profile(self, c):
-        o = np.zeros(20)
+        o = np.zeros(24)
         check(lib().bmg_batch_profile(self.h, c, _p(o)))
         keys = ["tau2", "addrem", "swap", "statechange", "learned", "beta_sigma2", "pve_thin", "total",
                 "add_xy", "add_xtx", "add_trsv", "add_rest", "n_add", "n_rem", "rem", "refresh_draws",
-                "rem_downdate", "rem_solve", "dr", "dr_score"]
+                "rem_downdate", "rem_solve", "dr", "dr_score", "ar_picks", "ar_apply", "ar_reverse",
+                "ar_decide"]
         return dict(zip(keys, o))
 
     def timing(self):

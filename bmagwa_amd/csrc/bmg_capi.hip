@@ -1376,7 +1376,7 @@ extern "C" int bmg_batch_learn_features(bmg_batch* b, int ds, const uint32_t* j,
   return learn_features(L, j, off, rows, out, b->stream);
 }
 
-extern "C" int bmg_batch_profile(bmg_batch* b, int chain, double out[20]) {
+extern "C" int bmg_batch_profile(bmg_batch* b, int chain, double out[24]) {
   if (!b || chain < 0 || chain >= b->n_chains || !out) { set_error("bad argument"); return BMG_E_ARG; }
   BMG_HIP(hipSetDevice(b->device));
   BMG_HIP(hipStreamSynchronize(b->stream));
@@ -1384,7 +1384,7 @@ extern "C" int bmg_batch_profile(bmg_batch* b, int chain, double out[20]) {
   BMG_HIP(hipMemcpy(&cs, b->d_scal + chain, sizeof(cs), hipMemcpyDeviceToHost));
   int khz = 0;
   BMG_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, b->device));
-  for (int q = 0; q < 20; ++q)
+  for (int q = 0; q < 24; ++q)
     out[q] = (q == 12 || q == 13) ? (double)cs.prof[q] : (khz > 0 ? (double)cs.prof[q] / (double)khz : 0.0);
   return BMG_OK;
 }

@@ -74,8 +74,11 @@ struct ChainScal {
   // update, and the count of additions and removals, time in removals,
   // tau2 draws of the refresh, the downdate and the solve of removals, whole
   // delayed-rejection second stages and their scoring (walk, proposal
-  // probabilities, draw)
-  uint64_t prof[20];
+  // probabilities, draw); then inside add/remove moves: the proposal draws
+  // (sum-tree picks), the removals and additions with their tree updates,
+  // the reverse move's weights, and the accept / reject handling (journal,
+  // commit or restore)
+  uint64_t prof[24];
   uint32_t rb_calls;
   int32_t movesize;
   double pves[3];

@@ -741,78 +741,98 @@ __device__ __forceinline__ void chol_downdate(Smem<KMAX>& S, int m, int cr) {
   else chol_downdate_glb(S, m, cr);
 }
 
-// Full factorisation of S.L (m x m, in place), right-looking in panels of
-// eight columns; returns 0 if not positive definite.  Within a panel, step
-// k takes the pivot, the thread of column i > k scales its row-k entry
-// (kept in S.work for the others) and subtracts work[j] * u_ki from its rows
-// j in (k, panel end).  After the panel, the thread of each later column i
-// walks its rows j at or past the panel end once, subtracting the panel's
-// eight products u_kj * u_ki in k order, with u_kj read from column j (one
-// address for all lanes) and its own u_ki in registers.  Every element thus
-// receives the sequential loop's subtractions in k order, each a multiply
-// and a subtract, but the rows below a panel are read and written once per
-// panel instead of once per step: those reads and writes (each lane in its
-// own column, so they meet bank conflicts) bound the unblocked loop.
+// Full factorisation of S.L (m x m, in place) in panels of CF_PB rows;
+// returns 0 if not positive definite.  Per panel [k0, k1): wave 0 factors
+// the diagonal block in registers (lane l holds column k0 + l; the pivot
+// and each row-k entry reach the other lanes by cross-lane reads, so the
+// block needs no barrier); then a thread per later column computes that
+// column's panel rows from the block (a forward substitution of CF_PB
+// steps); then the trailing update gives a lane to each row i >= k1 and
+// the four waves every fourth column j, so a wave's 64 lanes read and
+// write 64 consecutive entries of one column and all read the same panel
+// entries of column j.  Every element still receives the sequential loop's
+// subtractions in k order, each a multiply then a subtract, and its
+// division or square root after them, so the factor is that loop's bit for
+// bit; the panels cost three barriers each instead of two per column.
+constexpr int CF_PB = 16;
 template <int KMAX>
 __device__ __forceinline__ int chol_full(Smem<KMAX>& S, int m) {
-  constexpr int PB = 8;
+  constexpr int PB = CF_PB;
+  constexpr int NT = (KMAX + 63) / 64;
   __syncthreads();
+  const int lane = lane_id(), wv = wave_id();
   for (int k0 = 0; k0 < m; k0 += PB) {
     const int k1 = k0 + PB < m ? k0 + PB : m;
-    for (int k = k0; k < k1; ++k) {
-      const double akk = S.L[P(k) + k];
-      if (!(akk > 0.0)) { __syncthreads(); return 0; }
-      const double dk = sqrt(akk);
-      const int i = k + 1 + (int)threadIdx.x;   // this thread's column (m <= 256)
-      double u = 0.0;
-      if (i < m) {
-        u = S.L[P(i) + k] / dk;
-        S.L[P(i) + k] = u;
-        S.work[i] = u;
-      }
-      __syncthreads();
-      if (threadIdx.x == 0) S.L[P(k) + k] = dk;
-      if (i < m && k + 1 < k1) {
-        // the panel rows (k, min(i, k1 - 1)], at most seven: loads first,
-        // then unconditional stores (rows past the last one are clamped to
-        // it and rewrite it with the same value)
-        double* col = S.L + P(i);
-        const int je = i < k1 - 1 ? i : k1 - 1;
-        double c[PB - 1], w[PB - 1];
+    const int nb = k1 - k0;
+    if (wv == 0) {
+      const int j = k0 + lane;
+      const bool own = lane < nb;
+      double blk[PB];
 #pragma unroll
-        for (int q = 0; q < PB - 1; ++q) {
-          const int j = k + 1 + q <= je ? k + 1 + q : je;
-          c[q] = col[j];
-          w[q] = S.work[j];
+      for (int r = 0; r < PB; ++r) blk[r] = (own && r <= lane) ? S.L[P(j) + k0 + r] : 0.0;
+      int ok = 1;
+#pragma unroll
+      for (int kk = 0; kk < PB; ++kk) {
+        if (kk < nb) {
+          const double akk = readlane_d(blk[kk], kk);
+          if (!(akk > 0.0)) { ok = 0; break; }
+          const double dk = sqrt(akk);
+          const double u = blk[kk] / dk;
+          blk[kk] = lane == kk ? dk : (lane > kk ? u : blk[kk]);
+#pragma unroll
+          for (int r = kk + 1; r < PB; ++r) {
+            if (r < nb) {
+              const double ur = readlane_d(u, r);
+              if (lane >= r) blk[r] = blk[r] - ur * u;
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < PB; ++r) if (own && r <= lane) S.L[P(j) + k0 + r] = blk[r];
+      if (lane == 0) S.bci[0] = ok;
+    }
+    __syncthreads();
+    if (!S.bci[0]) { __syncthreads(); return 0; }
+    // the panel rows of the columns past the block
+    {
+      const int j = k1 + (int)threadIdx.x;
+      if (j < m) {
+        double c[PB];
+        double* col = S.L + P(j) + k0;
+#pragma unroll
+        for (int r = 0; r < PB; ++r) c[r] = r < nb ? col[r] : 0.0;
+#pragma unroll
+        for (int i = 0; i < PB; ++i) {
+          if (i < nb) {
+            const double* bc = S.L + P(k0 + i) + k0;   // block column k0 + i
+#pragma unroll
+            for (int k = 0; k < i; ++k) c[i] = c[i] - bc[k] * c[k];
+            c[i] = c[i] / bc[i];
+          }
         }
 #pragma unroll
-        for (int q = 0; q < PB - 1; ++q) col[k + 1 + q <= je ? k + 1 + q : je] = c[q] - w[q] * u;
+        for (int r = 0; r < PB; ++r) if (r < nb) col[r] = c[r];
       }
-      __syncthreads();
     }
-    // rows at or past k1 of the columns past k1: the panel's products
-    const int i = k1 + (int)threadIdx.x;
-    if (i < m) {
-      double* col = S.L + P(i);
-      const int nk = k1 - k0;
+    __syncthreads();
+    // trailing update: a_ij -= sum_k u_ki u_kj over the panel, i in [k1, j]
+    for (int q = 0; q < NT; ++q) {
+      const int i = k1 + lane + 64 * q;
+      if (64 * q >= m - k1) break;
       double ui[PB];
 #pragma unroll
-      for (int q = 0; q < PB; ++q) ui[q] = q < nk ? col[k0 + q] : 0.0;
-      for (int j0 = k1; j0 <= i; j0 += 4) {
-        double a[4];
+      for (int k = 0; k < PB; ++k) ui[k] = (i < m && k < nb) ? S.L[P(i) + k0 + k] : 0.0;
+      for (int j = k1 + 64 * q + wv; j < m; j += 4) {
+        const double* cj = S.L + P(j);
+        double uj[PB];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) a[r] = col[j0 + r <= i ? j0 + r : i];
+        for (int k = 0; k < PB; ++k) uj[k] = k < nb ? cj[k0 + k] : 0.0;
+        const int ic = i <= j ? i : j;
+        double a = cj[ic];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const double* cj = S.L + P(j0 + r <= i ? j0 + r : i) + k0;   // u_kj of column j
-#pragma unroll
-          for (int q = 0; q < PB; ++q)
-            if (q < nk) a[r] = a[r] - cj[q] * ui[q];
-        }
-        // unconditional stores: a row past i is clamped to i and rewrites
-        // col[i] with the value the real row i gives it
-#pragma unroll
-        for (int r = 0; r < 4; ++r) col[j0 + r <= i ? j0 + r : i] = a[r];
+        for (int k = 0; k < PB; ++k) if (k < nb) a = a - ui[k] * uj[k];
+        if (i <= j) S.L[P(j) + i] = a;
       }
     }
     __syncthreads();
@@ -2297,7 +2317,9 @@ __device__ __forceinline__ int move_addrem(Ctx<KMAX>& C, double* adapt) {
   double lqf = 0, lqb = 0, lmpc = 0;
   // the move's tree writes are journalled, so a rejection puts the trees
   // back without recomputing their sums
+  const uint64_t tq0 = wall_clock64();
   journal_begin(C);
+  const uint64_t tq1 = wall_clock64();
   // prepare_addrem (src/sampler.hpp:1126-1165)
   uint64_t max_adds = C.D.p - R.c_nloci, max_rems = R.c_nloci;
   for (int i = 0; i < ms; ++i) {
@@ -2324,6 +2346,7 @@ __device__ __forceinline__ int move_addrem(Ctx<KMAX>& C, double* adapt) {
     }
     __syncthreads();
   }
+  const uint64_t tq2 = wall_clock64();
   // do_addrem (src/sampler.hpp:1204-1268): removals first
   int last = ms - 1;
   for (int i = 0; i < ms; ++i) {
@@ -2347,6 +2370,7 @@ __device__ __forceinline__ int move_addrem(Ctx<KMAX>& C, double* adapt) {
     prepare_add_new_term(C, d2);
     add_term(C, (uint32_t)ind, 0, d2);
   }
+  const uint64_t tq3 = wall_clock64();
   // backward_prepare_addrem (src/sampler.hpp:1168-1202); the reverse
   // move's weights are loaded together first (thread i loads step i's)
   {
@@ -2385,6 +2409,10 @@ __device__ __forceinline__ int move_addrem(Ctx<KMAX>& C, double* adapt) {
   }
   double log_r = lqb - lqf;
   log_r += lmpc + R.ll - R.c_ll;
+  const uint64_t tq4 = wall_clock64();
+  if (threadIdx.x == 0 && C.prof) {
+    C.prof[20] += tq2 - tq1; C.prof[21] += tq3 - tq2; C.prof[22] += tq4 - tq3; C.prof[23] += tq1 - tq0;
+  }
   double* r_ms_sum = adapt + 3 * 256;
   double* r_ms_n = adapt + 4 * 256;
   if (threadIdx.x == 0) {
@@ -2395,6 +2423,7 @@ __device__ __forceinline__ int move_addrem(Ctx<KMAX>& C, double* adapt) {
     R.tj_on = 0;
     commit(C);
     if (K.dr != 0 && threadIdx.x == 0) r_ms_sum[ms - 1] += 1.0;
+    if (threadIdx.x == 0 && C.prof) C.prof[23] += wall_clock64() - tq4;
     return ms;
   }
   // a proposal past the compiled capacity has no second stage
@@ -2407,6 +2436,7 @@ __device__ __forceinline__ int move_addrem(Ctx<KMAX>& C, double* adapt) {
   }
   restore(C);
   journal_restore(C);
+  if (threadIdx.x == 0 && C.prof) C.prof[23] += wall_clock64() - tq4;
   return 0;
 }
 

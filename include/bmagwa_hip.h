@@ -389,8 +389,10 @@ int bmg_batch_graph_stats(bmg_batch* b, double out[3]);
  * tau2 draws of the refreshes; then within removals of an inner column: the
  * Cholesky downdate, the triangular solve; then delayed-rejection second
  * stages, and within them the scoring of the sub-models (walk, proposal
- * probabilities, draw) (device clock) */
-int bmg_batch_profile(bmg_batch* b, int chain, double out[20]);
+ * probabilities, draw); then within add/remove moves: the proposal draws,
+ * the removals and additions with their sum-tree updates, the reverse
+ * move's weights, the accept / reject handling (device clock) */
+int bmg_batch_profile(bmg_batch* b, int chain, double out[24]);
 
 /* --------------------------------------- learned-proposal network ----- */
 /* No counterpart in the reference.  The network of the learned proposal:
