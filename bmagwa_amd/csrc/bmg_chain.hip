@@ -852,14 +852,80 @@ __device__ __forceinline__ double tree_val(Ctx<KMAX>& C, int tr, int h, uint64_t
   return C.S.tlev[tr][C.D.lev_off[h] - C.D.lev_off[2] + i];
 }
 
+// a double moved across lanes by a DPP control (both halves; lanes whose
+// source is out of range get 0)
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+  int2 h = *reinterpret_cast<int2*>(&v);
+  h.x = __builtin_amdgcn_update_dpp(0, h.x, CTRL, 0xf, 0xf, false);
+  h.y = __builtin_amdgcn_update_dpp(0, h.y, CTRL, 0xf, 0xf, false);
+  return *reinterpret_cast<double*>(&h);
+}
+
+// the value of the lane 16 below (lanes 16..63): rows 1 and 3 from a row
+// swap, row 2 from a half swap followed by a row swap
+__device__ __forceinline__ double from_row_below(double v) {
+  const int row = (threadIdx.x & 63) >> 4;
+  int2 h = *reinterpret_cast<int2*>(&v), o;
+  {
+    const auto a = __builtin_amdgcn_permlane32_swap((unsigned)h.x, (unsigned)h.x, false, false);
+    const auto b = __builtin_amdgcn_permlane16_swap((unsigned)a[0], (unsigned)h.x, false, false);
+    o.x = (int)(row == 2 ? b[1] : b[0]);
+  }
+  {
+    const auto a = __builtin_amdgcn_permlane32_swap((unsigned)h.y, (unsigned)h.y, false, false);
+    const auto b = __builtin_amdgcn_permlane16_swap((unsigned)a[0], (unsigned)h.y, false, false);
+    o.y = (int)(row == 2 ? b[1] : b[0]);
+  }
+  return *reinterpret_cast<double*>(&o);
+}
+
+// the value of the lane 32 below (lanes 32..63)
+__device__ __forceinline__ double from_half_below(double v) {
+  int2 h = *reinterpret_cast<int2*>(&v), o;
+  o.x = (int)__builtin_amdgcn_permlane32_swap((unsigned)h.x, (unsigned)h.x, false, false)[0];
+  o.y = (int)__builtin_amdgcn_permlane32_swap((unsigned)h.y, (unsigned)h.y, false, false)[0];
+  return *reinterpret_cast<double*>(&o);
+}
+
+// inclusive Hillis-Steele scan over the wave (offsets 1, 2, ..., 32, each
+// lane adding the value `offset` lanes below).  Offset 1 is a wave shift
+// (DPP), 16 and 32 are lane swaps, 2-8 cross-lane reads; every lane adds
+// the same operands in the same order at every offset whichever move
+// brings them, so the sums are the plain shuffle scan's bit for bit.
 __device__ __forceinline__ double wave_scan(double v) {
   const int lane = threadIdx.x & 63;
+  { const double t = dpp_d<0x138>(v); if (lane >= 1) v = v + t; }   // wave_shr:1
 #pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    double t = __shfl_up(v, d, 64);
+  for (int d = 2; d < 16; d <<= 1) {
+    const double t = __shfl_up(v, d, 64);
     if (lane >= d) v = v + t;
   }
+  { const double t = from_row_below(v); if (lane >= 16) v = v + t; }
+  { const double t = from_half_below(v); if (lane >= 32) v = v + t; }
   return v;
+}
+
+// the last element of wave_scan(v), in every lane: lane 63's scan value is
+// a balanced pairwise sum ((v62 + v63) + (v60 + v61)) + ..., which a
+// butterfly over pairs, quads, eights, rows and halves forms with the same
+// operands (partners by DPP quad permutations and mirrors, then lane
+// swaps), so it is bit-identical without the scan's serial reads
+__device__ __forceinline__ double group_total(double v) {
+  v = v + dpp_d<0xB1>(v);    // quad_perm [1,0,3,2]
+  v = v + dpp_d<0x4E>(v);    // quad_perm [2,3,0,1]
+  v = v + dpp_d<0x141>(v);   // row_half_mirror
+  v = v + dpp_d<0x140>(v);   // row_mirror
+  {
+    const bool odd = ((threadIdx.x & 63) >> 4) & 1;
+    int2 h = *reinterpret_cast<int2*>(&v), o;
+    const auto a = __builtin_amdgcn_permlane16_swap((unsigned)h.x, (unsigned)h.x, false, false);
+    const auto b = __builtin_amdgcn_permlane16_swap((unsigned)h.y, (unsigned)h.y, false, false);
+    o.x = (int)(odd ? a[0] : a[1]);
+    o.y = (int)(odd ? b[0] : b[1]);
+    v = v + *reinterpret_cast<double*>(&o);
+  }
+  return readlane_d(v, 0) + readlane_d(v, 32);
 }
 
 // draw an index with probability w_i / total (wave 0 computes, all return)
@@ -913,16 +979,14 @@ __device__ __forceinline__ void tree_set(Ctx<KMAX>& C, int tr, uint64_t i, int z
       }
     }
     if (lane == (int)(i & 63)) c0 = nv;
-    double s = wave_scan(c0);
-    double gs = __shfl(s, 63, 64);
+    double gs = group_total(c0);
     if (lane == 0) g0[C.D.lev_off[0] + i] = nv;
     if (C.D.top == 0) {
       total = gs;
     } else {
       if (lane == 0) g0[C.D.lev_off[1] + g1] = gs;
       if (lane == (int)(g1 & 63)) c1 = gs;
-      s = wave_scan(c1);
-      gs = __shfl(s, 63, 64);
+      gs = group_total(c1);
       uint64_t gi = g1 >> 6;   // entry index at level 2
       for (int h = 2; h <= C.D.top; ++h) {
         double* lv = C.S.tlev[tr] + (C.D.lev_off[h] - C.D.lev_off[2]);
@@ -930,8 +994,7 @@ __device__ __forceinline__ void tree_set(Ctx<KMAX>& C, int tr, uint64_t i, int z
         uint64_t grp = gi >> 6;
         double c = lv[64 * grp + lane];
         if (lane == (int)(gi & 63)) c = gs;
-        s = wave_scan(c);
-        gs = __shfl(s, 63, 64);
+        gs = group_total(c);
         gi = grp;
       }
       total = gs;
@@ -989,15 +1052,13 @@ __device__ __forceinline__ uint64_t tree_pick_zero(Ctx<KMAX>& C, int tr, double 
       // tree_set(zero) on the groups already read
       double c0 = cl[0];
       if (lane == (int)(i & 63)) c0 = 0.0;
-      double s = wave_scan(c0);
-      double gs = __shfl(s, 63, 64);
+      double gs = group_total(c0);
       if (lane == 0) g0[C.D.lev_off[0] + i] = 0.0;
       if (C.D.top >= 1) {
         if (lane == 0) g0[C.D.lev_off[1] + g1] = gs;
         double c1 = cl[1];
         if (lane == (int)(g1 & 63)) c1 = gs;
-        s = wave_scan(c1);
-        gs = __shfl(s, 63, 64);
+        gs = group_total(c1);
         uint64_t gi = g1 >> 6;
         for (int h = 2; h <= C.D.top; ++h) {
           double* lv = C.S.tlev[tr] + (C.D.lev_off[h] - C.D.lev_off[2]);
@@ -1006,8 +1067,7 @@ __device__ __forceinline__ uint64_t tree_pick_zero(Ctx<KMAX>& C, int tr, double 
 #pragma unroll
           for (int q = 2; q < BMG_TREE_LEV; ++q) if (q == h) c = cl[q];
           if (lane == (int)(gi & 63)) c = gs;
-          s = wave_scan(c);
-          gs = __shfl(s, 63, 64);
+          gs = group_total(c);
           gi = gi >> 6;
         }
       }
