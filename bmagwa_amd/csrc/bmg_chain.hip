@@ -242,9 +242,10 @@ __device__ __forceinline__ void planes_c(uint32_t w, int coding, uint32_t& p1, u
 // soon as the previous column in its register set is summed; the chunk
 // sums meet in LDS integer accumulators (exact, so the order is free).
 // W / 16 lanes cover a column, so 256 / (W / 16) column groups run side by
-// side.  CODING: the new column's coding.
+// side; the first group also writes the new column's value words to its
+// slot copy (xnew).  CODING: the new column's coding.
 template <int KMAX, int CODING>
-__device__ __forceinline__ void column_dots(Ctx<KMAX>& C, uint64_t snp_new, int cols) {
+__device__ __forceinline__ void column_dots(Ctx<KMAX>& C, uint64_t snp_new, int cols, uint32_t* __restrict__ xnew) {
   const uint32_t W = C.D.W32;                 // a multiple of 32 words
   const uint32_t* __restrict__ wn = C.D.g + snp_new * W;
   const uint32_t* __restrict__ xcol = C.A.xcol;
@@ -269,6 +270,7 @@ __device__ __forceinline__ void column_dots(Ctx<KMAX>& C, uint64_t snp_new, int 
           planes_c<CODING>(xs[u], CODING, p1, p2);
           const uint32_t vw = p1 + p2;
           const int k = 4 * q + u;
+          if (grp == 0) xnew[w0 + k] = vw;
           ne[k] = vw & 0x33333333u;
           no[k] = (vw >> 2) & 0x33333333u;
           s_one = __builtin_amdgcn_udot8(0x11111111u, ne[k], s_one, false);
@@ -339,16 +341,19 @@ __device__ __forceinline__ void cov_dots(Ctx<KMAX>& C, uint64_t snp_new, int cod
   __syncthreads();
 }
 
+// S.work[c] = x_c^T x_new for c in [0, cols] (covariates included), and
+// the new column's value words into its slot copy xnew
 template <int KMAX>
-__device__ __forceinline__ void new_column_dots(Ctx<KMAX>& C, uint64_t snp_new, int coding_new, int cols) {
+__device__ __forceinline__ void new_column_dots(Ctx<KMAX>& C, uint64_t snp_new, int coding_new, int cols,
+                                                uint32_t* xnew) {
   Smem<KMAX>& S = C.S;
   for (int c = threadIdx.x; c <= cols; c += BMG_BLOCK) S.iacc[c] = 0;
   __syncthreads();
   switch (coding_new) {
-    case 0: column_dots<KMAX, 0>(C, snp_new, cols); break;
-    case 1: column_dots<KMAX, 1>(C, snp_new, cols); break;
-    case 2: column_dots<KMAX, 2>(C, snp_new, cols); break;
-    default: column_dots<KMAX, 3>(C, snp_new, cols); break;
+    case 0: column_dots<KMAX, 0>(C, snp_new, cols, xnew); break;
+    case 1: column_dots<KMAX, 1>(C, snp_new, cols, xnew); break;
+    case 2: column_dots<KMAX, 2>(C, snp_new, cols, xnew); break;
+    default: column_dots<KMAX, 3>(C, snp_new, cols, xnew); break;
   }
   __syncthreads();
   for (int c = threadIdx.x; c <= cols; c += BMG_BLOCK) S.work[c] = (double)S.iacc[c];
@@ -1204,21 +1209,13 @@ __device__ __forceinline__ void update_on_add(Ctx<KMAX>& C, uint32_t ind, int mo
   set_slot_bit(C, slot, 1);
   R.cols = col + 1;
   if (R.lmin > col) R.lmin = col;
-  {
-    // the slot keeps the column's value words (coded genotype in each 2-bit
-    // field), which the dot products and X beta read without the coding
-    const uint32_t* src = C.D.g + (uint64_t)ind * C.D.W32;
-    uint32_t* dst = C.A.xcol + (size_t)slot * C.D.W32;
-    for (uint32_t k = threadIdx.x; k < C.D.W32; k += BMG_BLOCK) {
-      uint32_t p1, p2;
-      planes(src[k], coding, p1, p2);
-      dst[k] = p1 + p2;
-    }
-  }
   const uint64_t ta = wall_clock64();
   double xy = snp_dot_y(C, ind, coding);
   const uint64_t tb = wall_clock64();
-  new_column_dots(C, ind, coding, col);
+  // the slot keeps the column's value words (coded genotype in each 2-bit
+  // field), which the dot products and X beta read without the coding;
+  // the dot products write them as they decode the column
+  new_column_dots(C, ind, coding, col, C.A.xcol + (size_t)slot * C.D.W32);
   const uint64_t tc = wall_clock64();
   // record the new X^T X row/column in the slot matrix
   for (int c = threadIdx.x; c <= col; c += BMG_BLOCK) {
