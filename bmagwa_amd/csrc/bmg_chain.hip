@@ -88,6 +88,7 @@ struct Smem {
   int32_t win[512];                      // window of the current model_inds
   uint64_t slots_cur[(KMAX + 63) / 64], slots_prop[(KMAX + 63) / 64];
   int32_t iacc[KMAX + 1];                // integer column dots of new_column_dots
+  int32_t iacc2[KMAX + 1];               // the next addition's dots, formed ahead
   // delayed rejection: factor block, v, d and log d of the toggled loci's
   // columns, their column offsets and types, and the moved SNPs' proposal
   // weights
@@ -126,6 +127,8 @@ struct Reg {
   int err;
   int learn_on;
   int tj_on, tj_n;                       // journal active, entries
+  long long pre_snp;                     // SNP whose dots are in iacc2 (or -1)
+  int pre_slot;                          // and the slot its value words went to
   double tj_tot[2];                      // tree totals when the move began
 };
 
@@ -241,22 +244,27 @@ __device__ __forceinline__ void planes_c(uint32_t w, int coding, uint32_t& p1, u
 // four columns' words in flight at a time, each column's loads issued as
 // soon as the previous column in its register set is summed; the chunk
 // sums meet in LDS integer accumulators (exact, so the order is free).
-// W / 16 lanes cover a column, so 256 / (W / 16) column groups run side by
+// W / 16 lanes cover a column, so NTH / (W / 16) column groups run side by
 // side; the first group also writes the new column's value words to its
-// slot copy (xnew).  CODING: the new column's coding.
-template <int KMAX, int CODING>
-__device__ __forceinline__ void column_dots(Ctx<KMAX>& C, uint64_t snp_new, int cols, uint32_t* __restrict__ xnew) {
+// slot copy (xnew).  CODING: the new column's coding; threads [T0, T0 + NTH)
+// take part (the whole workgroup, or waves 1-3 while wave 0 runs a
+// triangular sweep); acc: the integer accumulators, zeroed by the caller.
+template <int KMAX, int CODING, int T0, int NTH>
+__device__ __forceinline__ void column_dots(Ctx<KMAX>& C, uint64_t snp_new, int cols, uint32_t* __restrict__ xnew,
+                                            int32_t* acc) {
   const uint32_t W = C.D.W32;                 // a multiple of 32 words
   const uint32_t* __restrict__ wn = C.D.g + snp_new * W;
   const uint32_t* __restrict__ xcol = C.A.xcol;
   Smem<KMAX>& S = C.S;
   const int me = C.D.me;
   const uint32_t cpc = W / 16;                // chunks per column
-  for (uint32_t ch0 = 0; ch0 < cpc; ch0 += BMG_BLOCK) {
-    const uint32_t span = min(cpc - ch0, (uint32_t)BMG_BLOCK);
-    const int groups = (int)(BMG_BLOCK / span);
-    const int grp = (int)(threadIdx.x / span);
-    const uint32_t w0 = (ch0 + threadIdx.x % span) * 16;
+  const int tid = (int)threadIdx.x - T0;
+  if (tid < 0 || tid >= NTH) return;
+  for (uint32_t ch0 = 0; ch0 < cpc; ch0 += NTH) {
+    const uint32_t span = min(cpc - ch0, (uint32_t)NTH);
+    const int groups = (int)(NTH / span);
+    const int grp = (int)(tid / span);
+    const uint32_t w0 = (ch0 + tid % span) * 16;
     if (grp < groups) {
       uint32_t ne[16], no[16];   // the new column's even / odd fields as nibbles
       uint32_t s_one = 0, s_self = 0;
@@ -315,7 +323,7 @@ __device__ __forceinline__ void column_dots(Ctx<KMAX>& C, uint64_t snp_new, int 
           for (int q = 0; q < 4; ++q) cur[u][q] = *(const uint4*)(pn + w0 + 4 * q);
           const uint32_t sum = cu == 0 ? s_one : (cu == cols ? s_self : se + (so >> 2));
           // covariate columns (0 < c < me): cov_dots
-          if (cu <= cols && (cu == 0 || cu >= me)) atomicAdd(&S.iacc[cu], (int)sum);
+          if (cu <= cols && (cu == 0 || cu >= me)) atomicAdd(&acc[cu], (int)sum);
         }
       }
     }
@@ -350,10 +358,10 @@ __device__ __forceinline__ void new_column_dots(Ctx<KMAX>& C, uint64_t snp_new, 
   for (int c = threadIdx.x; c <= cols; c += BMG_BLOCK) S.iacc[c] = 0;
   __syncthreads();
   switch (coding_new) {
-    case 0: column_dots<KMAX, 0>(C, snp_new, cols, xnew); break;
-    case 1: column_dots<KMAX, 1>(C, snp_new, cols, xnew); break;
-    case 2: column_dots<KMAX, 2>(C, snp_new, cols, xnew); break;
-    default: column_dots<KMAX, 3>(C, snp_new, cols, xnew); break;
+    case 0: column_dots<KMAX, 0, 0, BMG_BLOCK>(C, snp_new, cols, xnew, S.iacc); break;
+    case 1: column_dots<KMAX, 1, 0, BMG_BLOCK>(C, snp_new, cols, xnew, S.iacc); break;
+    case 2: column_dots<KMAX, 2, 0, BMG_BLOCK>(C, snp_new, cols, xnew, S.iacc); break;
+    default: column_dots<KMAX, 3, 0, BMG_BLOCK>(C, snp_new, cols, xnew, S.iacc); break;
   }
   __syncthreads();
   for (int c = threadIdx.x; c <= cols; c += BMG_BLOCK) S.work[c] = (double)S.iacc[c];
@@ -1184,11 +1192,15 @@ __device__ __forceinline__ void guard_syx(Ctx<KMAX>& C) {
 // src/model.hpp:432-506 for one column (type A, H, D or R)
 template <int KMAX>
 __device__ __forceinline__ void update_on_add(Ctx<KMAX>& C, uint32_t ind, int model_ind, int coding,
-                              int is_x_ind2, double dval) {
+                              int is_x_ind2, double dval, long long next = -1) {
   Smem<KMAX>& S = C.S;
   Reg& R = C.R;
   int col = R.cols;
   int slot = alloc_slot(C);
+  // dots formed ahead by the previous addition of this move (same SNP,
+  // same slot, every column since unchanged)
+  const bool pre = R.pre_snp == (long long)ind && R.pre_slot == slot;
+  R.pre_snp = -1;
   if (slot < 0 || col >= KMAX) {
     // current and proposed models together need more slots than compiled:
     // the proposal is rejected (log-likelihood -inf) and flagged
@@ -1215,7 +1227,12 @@ __device__ __forceinline__ void update_on_add(Ctx<KMAX>& C, uint32_t ind, int mo
   // the slot keeps the column's value words (coded genotype in each 2-bit
   // field), which the dot products and X beta read without the coding;
   // the dot products write them as they decode the column
-  new_column_dots(C, ind, coding, col, C.A.xcol + (size_t)slot * C.D.W32);
+  if (pre) {
+    for (int c = threadIdx.x; c <= col; c += BMG_BLOCK) S.work[c] = (double)S.iacc2[c];
+    __syncthreads();
+  } else {
+    new_column_dots(C, ind, coding, col, C.A.xcol + (size_t)slot * C.D.W32);
+  }
   const uint64_t tc = wall_clock64();
   // record the new X^T X row/column in the slot matrix
   for (int c = threadIdx.x; c <= col; c += BMG_BLOCK) {
@@ -1224,11 +1241,30 @@ __device__ __forceinline__ void update_on_add(Ctx<KMAX>& C, uint32_t ind, int mo
     C.A.xxs[(size_t)sc * BMG_KMAXG + slot] = S.work[c];
   }
   if (threadIdx.x == 0) S.xy[col] = xy;
+  // the next addition of the move (next >= 0): its column's dots with every
+  // column up to this one are formed by waves 1-3 while wave 0 runs this
+  // column's sweep (one effect type, no covariates); its slot is the one it
+  // will be given, the lowest free once this column's is taken
+  int nslot = -1;
+  if (next >= 0 && !C.D.mixed && C.D.type != BMG_AH && C.D.me <= 1 && col + 1 < KMAX) nslot = alloc_slot(C);
+  if (nslot >= 0)
+    for (int c = threadIdx.x; c <= col + 1; c += BMG_BLOCK) S.iacc2[c] = 0;
   // bordered update (src/symmmatrix.cpp:144-164)
   for (int i = threadIdx.x; i < col; i += BMG_BLOCK) S.L[P(col) + i] = S.work[i];
   __syncthreads();
-  if (wave_id() == 0) trsv_ut_w0(S, S.L + P(col), col);
+  if (wave_id() == 0) {
+    trsv_ut_w0(S, S.L + P(col), col);
+  } else if (nslot >= 0) {
+    uint32_t* xn = C.A.xcol + (size_t)nslot * C.D.W32;
+    switch (C.D.type) {
+      case 0: column_dots<KMAX, 0, 64, BMG_BLOCK - 64>(C, (uint64_t)next, col + 1, xn, S.iacc2); break;
+      case 1: column_dots<KMAX, 1, 64, BMG_BLOCK - 64>(C, (uint64_t)next, col + 1, xn, S.iacc2); break;
+      case 2: column_dots<KMAX, 2, 64, BMG_BLOCK - 64>(C, (uint64_t)next, col + 1, xn, S.iacc2); break;
+      default: column_dots<KMAX, 3, 64, BMG_BLOCK - 64>(C, (uint64_t)next, col + 1, xn, S.iacc2); break;
+    }
+  }
   __syncthreads();
+  if (nslot >= 0) { R.pre_snp = next; R.pre_slot = nslot; }
   const uint64_t td = wall_clock64();
   double ll2 = 0.0, lv = 0.0;
   for (int i = threadIdx.x; i < col; i += BMG_BLOCK) {
@@ -1264,7 +1300,8 @@ __device__ __forceinline__ void update_on_add(Ctx<KMAX>& C, uint32_t ind, int mo
 
 // src/model.hpp:239-268
 template <int KMAX>
-__device__ __forceinline__ void add_term(Ctx<KMAX>& C, uint32_t ind, int t_ind, const double* d2) {
+__device__ __forceinline__ void add_term(Ctx<KMAX>& C, uint32_t ind, int t_ind, const double* d2,
+                                         long long next = -1) {
   Smem<KMAX>& S = C.S;
   Reg& R = C.R;
   int type = C.D.types[t_ind];
@@ -1297,7 +1334,7 @@ __device__ __forceinline__ void add_term(Ctx<KMAX>& C, uint32_t ind, int t_ind, 
       R.cols = col + 1;
     }
   } else {
-    update_on_add(C, ind, model_ind, type, 0, d2[0]);
+    update_on_add(C, ind, model_ind, type, 0, d2[0], next);
   }
 }
 
@@ -2419,14 +2456,19 @@ __device__ __forceinline__ int move_addrem(Ctx<KMAX>& C, double* adapt) {
     remove_term(C, mind);
   }
   double d2[2];
+  R.pre_snp = -1;
   for (int i = 0; i < ms; ++i) {
     if (!S.mi_isadd[i]) continue;
     uint64_t ind = S.mi[i];
+    long long next = -1;   // the move's next addition, whose dots are formed ahead
+    for (int j = i + 1; j < ms; ++j)
+      if (S.mi_isadd[j]) { next = (long long)S.mi[j]; break; }
     tree_set(C, 1, ind, 0);
     lmpc += prior_add(K, R.Ns, R.nloci, type);
     prepare_add_new_term(C, d2);
-    add_term(C, (uint32_t)ind, 0, d2);
+    add_term(C, (uint32_t)ind, 0, d2, next);
   }
+  R.pre_snp = -1;
   const uint64_t tq3 = wall_clock64();
   // backward_prepare_addrem (src/sampler.hpp:1168-1202); the reverse
   // move's weights are loaded together first (thread i loads step i's)
@@ -2834,6 +2876,7 @@ __device__ __forceinline__ void load_chain(Ctx<KMAX>& C, const ChainScal& CS) {
   R.err = CS.err;
   R.learn_on = CS.learn_on && C.D.lw != nullptr && C.D.lH >= 1 && C.D.lH <= BMG_LH_MAX;
   R.tj_on = 0; R.tj_n = 0; R.tj_tot[0] = R.tj_tot[1] = 0.0;
+  R.pre_snp = -1; R.pre_slot = -1;
   const double* vec = C.A.vec;
   const int32_t* idx = C.A.idx;
   for (int e = threadIdx.x; e < P(R.cols); e += BMG_BLOCK) S.L[e] = C.A.Lcur[e];
