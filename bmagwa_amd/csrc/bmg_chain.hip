@@ -2800,12 +2800,15 @@ __device__ __forceinline__ int move_statechange(Ctx<KMAX>& C, double* adapt) {
   }
   double d2[2];
   int t_ind = 0;
+  R.pre_snp = -1;
   for (int i = 0; i < ms_add; ++i) {
     uint64_t ia = S.mi_add[i];
     lmpc += prior_add(K, R.Ns, R.nloci, type);
     prepare_add_new_term(C, d2);
-    add_term(C, (uint32_t)ia, t_ind, d2);
+    // the next addition's dots are formed during this one's sweep
+    add_term(C, (uint32_t)ia, t_ind, d2, i + 1 < ms_add ? (long long)S.mi_add[i + 1] : -1);
   }
+  R.pre_snp = -1;
   double log_r = lmpc + R.ll - R.c_ll;
   log_r += log((double)R.c_nloci) - log((double)R.nloci);
   if (log_r >= 0 || log(rng_u01(R.rng)) <= log_r) {
