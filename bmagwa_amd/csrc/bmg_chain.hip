@@ -556,8 +556,8 @@ __device__ __forceinline__ void rot_gen(double& a, double b, double& c, double& 
 // factor's rows, so after the removal it solves the new factor's system
 // without a second triangular solve (the reference re-solves,
 // src/model.hpp:530-545; the two agree to rounding).
-template <int KMAX>
-__device__ __forceinline__ void chol_downdate_lds(Smem<KMAX>& S, int m, int cr) {
+template <int KMAX, typename Extra>
+__device__ __forceinline__ void chol_downdate_lds(Smem<KMAX>& S, int m, int cr, Extra extra) {
   constexpr int NT = (KMAX + 63) / 64;
   constexpr int LIM = KMAX * (KMAX + 1) / 2 - 1;
   const int lane = lane_id();
@@ -615,21 +615,24 @@ __device__ __forceinline__ void chol_downdate_lds(Smem<KMAX>& S, int m, int cr) 
         for (int t = 0; t < NT; ++t) { const double tmp = nA[t]; nA[t] = nB[t]; nB[t] = tmp; }
       }
     }
-  } else if (wave_id() == 1) {
-    // rows [0, cr) of logical columns [cr, m - 2]: physical column jl + 1 ->
-    // jl, element k = (jl - cr) * cr + i in ascending order
-    const int total = (m - 1 - cr) * cr;
-    const float rcr = cr > 0 ? 1.0f / (float)cr : 0.0f;
-    for (int k0 = 0; k0 < total; k0 += 64) {
-      const int k = k0 + lane;
-      if (k < total) {
-        int q = (int)((float)k * rcr);
-        if (q * cr > k) --q;
-        if ((q + 1) * cr <= k) ++q;
-        const int jl = cr + q, i = k - q * cr;
-        S.L[P(jl) + i] = S.L[P(jl + 1) + i];
+  } else {
+    if (wave_id() == 1) {
+      // rows [0, cr) of logical columns [cr, m - 2]: physical column jl + 1
+      // -> jl, element k = (jl - cr) * cr + i in ascending order
+      const int total = (m - 1 - cr) * cr;
+      const float rcr = cr > 0 ? 1.0f / (float)cr : 0.0f;
+      for (int k0 = 0; k0 < total; k0 += 64) {
+        const int k = k0 + lane;
+        if (k < total) {
+          int q = (int)((float)k * rcr);
+          if (q * cr > k) --q;
+          if ((q + 1) * cr <= k) ++q;
+          const int jl = cr + q, i = k - q * cr;
+          S.L[P(jl) + i] = S.L[P(jl + 1) + i];
+        }
       }
     }
+    extra();   // waves 1-3: work that touches neither the factor nor v
   }
   __syncthreads();
 }
@@ -748,10 +751,15 @@ __device__ __forceinline__ void chol_downdate_glb(Smem<KMAX>& S, int m, int cr) 
   }
 }
 
-template <int KMAX>
-__device__ __forceinline__ void chol_downdate(Smem<KMAX>& S, int m, int cr) {
-  if constexpr (KMAX <= KMAX_LDS) chol_downdate_lds(S, m, cr);
-  else chol_downdate_glb(S, m, cr);
+template <int KMAX, typename Extra>
+__device__ __forceinline__ void chol_downdate(Smem<KMAX>& S, int m, int cr, Extra extra) {
+  if constexpr (KMAX <= KMAX_LDS) {
+    chol_downdate_lds(S, m, cr, extra);
+  } else {
+    chol_downdate_glb(S, m, cr);
+    extra();
+    __syncthreads();
+  }
 }
 
 // Full factorisation of S.L (m x m, in place) in panels of CF_PB rows;
@@ -1340,17 +1348,17 @@ __device__ __forceinline__ void add_term(Ctx<KMAX>& C, uint32_t ind, int t_ind, 
 
 // src/model.hpp:508-556
 template <int KMAX>
-__device__ __forceinline__ void update_on_remove_impl(Ctx<KMAX>& C, int x_ind);
+__device__ __forceinline__ void update_on_remove_impl(Ctx<KMAX>& C, int x_ind, long long next);
 
 template <int KMAX>
-__device__ __forceinline__ void update_on_remove(Ctx<KMAX>& C, int x_ind) {
+__device__ __forceinline__ void update_on_remove(Ctx<KMAX>& C, int x_ind, long long next = -1) {
   const uint64_t t0 = wall_clock64();
-  update_on_remove_impl(C, x_ind);
+  update_on_remove_impl(C, x_ind, next);
   if (threadIdx.x == 0 && C.prof) { C.prof[13] += 1; C.prof[14] += wall_clock64() - t0; }
 }
 
 template <int KMAX>
-__device__ __forceinline__ void update_on_remove_impl(Ctx<KMAX>& C, int x_ind) {
+__device__ __forceinline__ void update_on_remove_impl(Ctx<KMAX>& C, int x_ind, long long next) {
   Smem<KMAX>& S = C.S;
   Reg& R = C.R;
   int k = R.cols;
@@ -1373,9 +1381,30 @@ __device__ __forceinline__ void update_on_remove_impl(Ctx<KMAX>& C, int x_ind) {
     R.ldl -= log(lxx);
     R.syx += vx * vx;
   } else {
+    // the move's first addition after this removal (next >= 0): its column
+    // dots with the remaining columns are formed by waves 1-3 during the
+    // downdate, into the slot it will be given (as during a forward sweep)
+    int nslot = -1;
+    if (KMAX <= KMAX_LDS && next >= 0 && !C.D.mixed && C.D.type != BMG_AH && C.D.me <= 1 && R.cols < KMAX)
+      nslot = alloc_slot(C);
+    if (nslot >= 0) {
+      for (int c = threadIdx.x; c <= R.cols; c += BMG_BLOCK) S.iacc2[c] = 0;
+      __syncthreads();
+    }
+    auto ahead = [&]() {
+      if (nslot < 0) return;
+      uint32_t* xn = C.A.xcol + (size_t)nslot * C.D.W32;
+      switch (C.D.type) {
+        case 0: column_dots<KMAX, 0, 64, BMG_BLOCK - 64>(C, (uint64_t)next, R.cols, xn, S.iacc2); break;
+        case 1: column_dots<KMAX, 1, 64, BMG_BLOCK - 64>(C, (uint64_t)next, R.cols, xn, S.iacc2); break;
+        case 2: column_dots<KMAX, 2, 64, BMG_BLOCK - 64>(C, (uint64_t)next, R.cols, xn, S.iacc2); break;
+        default: column_dots<KMAX, 3, 64, BMG_BLOCK - 64>(C, (uint64_t)next, R.cols, xn, S.iacc2); break;
+      }
+    };
     const uint64_t ta = wall_clock64();
     // v follows the downdate's rotations (no re-solve)
-    chol_downdate(S, k, x_ind);
+    chol_downdate(S, k, x_ind, ahead);
+    if (nslot >= 0) { R.pre_snp = next; R.pre_slot = nslot; }
     const uint64_t tb = wall_clock64();
     if (threadIdx.x == 0 && C.prof) { C.prof[16] += tb - ta; C.prof[17] += wall_clock64() - tb; }
     double ld = 0.0, vv = 0.0;
@@ -1394,7 +1423,7 @@ __device__ __forceinline__ void update_on_remove_impl(Ctx<KMAX>& C, int x_ind) {
 
 // src/model.hpp:270-312
 template <int KMAX>
-__device__ __forceinline__ void remove_term(Ctx<KMAX>& C, int model_ind) {
+__device__ __forceinline__ void remove_term(Ctx<KMAX>& C, int model_ind, long long next = -1) {
   Smem<KMAX>& S = C.S;
   Reg& R = C.R;
   int type = C.D.types[S.tind[model_ind]];
@@ -1417,7 +1446,7 @@ __device__ __forceinline__ void remove_term(Ctx<KMAX>& C, int model_ind) {
     if (xi1 > xi2) { update_on_remove(C, xi1); update_on_remove(C, xi2); }
     else { update_on_remove(C, xi2); update_on_remove(C, xi1); }
   } else {
-    update_on_remove(C, xi1);
+    update_on_remove(C, xi1, next);
   }
   R.mu_computed = 0;
 }
@@ -2441,7 +2470,15 @@ __device__ __forceinline__ int move_addrem(Ctx<KMAX>& C, double* adapt) {
     __syncthreads();
   }
   const uint64_t tq2 = wall_clock64();
-  // do_addrem (src/sampler.hpp:1204-1268): removals first
+  // do_addrem (src/sampler.hpp:1204-1268): removals first; the last removal's
+  // downdate also forms the first addition's column dots
+  int last_rem = -1;
+  long long first_add = -1;
+  for (int i = 0; i < ms; ++i) {
+    if (!S.mi_isadd[i]) last_rem = i;
+    else if (first_add < 0) first_add = (long long)S.mi[i];
+  }
+  R.pre_snp = -1;
   int last = ms - 1;
   for (int i = 0; i < ms; ++i) {
     if (S.mi_isadd[i]) continue;
@@ -2453,10 +2490,9 @@ __device__ __forceinline__ int move_addrem(Ctx<KMAX>& C, double* adapt) {
     __syncthreads();
     int mind = find_locus(C, ind);
     lmpc += prior_rem(K, R.Ns, R.nloci, type);
-    remove_term(C, mind);
+    remove_term(C, mind, i == last_rem ? first_add : -1);
   }
   double d2[2];
-  R.pre_snp = -1;
   for (int i = 0; i < ms; ++i) {
     if (!S.mi_isadd[i]) continue;
     uint64_t ind = S.mi[i];
@@ -2793,14 +2829,15 @@ __device__ __forceinline__ int move_statechange(Ctx<KMAX>& C, double* adapt) {
   }
   __syncthreads();
   double lmpc = 0.0;
+  R.pre_snp = -1;
   for (int i = 0; i < ms_rem; ++i) {
     int mir = find_locus(C, S.mi_rem[i]);
     lmpc += prior_rem(K, R.Ns, R.nloci, type);
-    remove_term(C, mir);
+    // the last removal's downdate also forms the first addition's dots
+    remove_term(C, mir, i + 1 == ms_rem && ms_add > 0 ? (long long)S.mi_add[0] : -1);
   }
   double d2[2];
   int t_ind = 0;
-  R.pre_snp = -1;
   for (int i = 0; i < ms_add; ++i) {
     uint64_t ia = S.mi_add[i];
     lmpc += prior_add(K, R.Ns, R.nloci, type);
