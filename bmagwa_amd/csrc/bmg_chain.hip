@@ -654,8 +654,8 @@ __device__ __forceinline__ void chol_downdate_lds(Smem<KMAX>& S, int m, int cr, 
 // factor's rows, so after the removal it solves the new factor's system
 // without a second triangular solve (the reference re-solves,
 // src/model.hpp:530-545; the two agree to rounding).
-template <int KMAX>
-__device__ __forceinline__ void chol_downdate_glb(Smem<KMAX>& S, int m, int cr) {
+template <int KMAX, typename Extra>
+__device__ __forceinline__ void chol_downdate_glb(Smem<KMAX>& S, int m, int cr, Extra extra) {
   constexpr int NT = (KMAX + 63) / 64;
   if (wave_id() == 0) {
     const int lane = lane_id();
@@ -699,6 +699,8 @@ __device__ __forceinline__ void chol_downdate_glb(Smem<KMAX>& S, int m, int cr) 
         }
       }
     }
+  } else {
+    extra();   // waves 1-3: work that touches neither the factor nor v
   }
   __syncthreads();
   // sign fix over the kept rows i in [cr, m-2]: flags first, then negation
@@ -756,9 +758,7 @@ __device__ __forceinline__ void chol_downdate(Smem<KMAX>& S, int m, int cr, Extr
   if constexpr (KMAX <= KMAX_LDS) {
     chol_downdate_lds(S, m, cr, extra);
   } else {
-    chol_downdate_glb(S, m, cr);
-    extra();
-    __syncthreads();
+    chol_downdate_glb(S, m, cr, extra);
   }
 }
 
@@ -1385,7 +1385,7 @@ __device__ __forceinline__ void update_on_remove_impl(Ctx<KMAX>& C, int x_ind, l
     // dots with the remaining columns are formed by waves 1-3 during the
     // downdate, into the slot it will be given (as during a forward sweep)
     int nslot = -1;
-    if (KMAX <= KMAX_LDS && next >= 0 && !C.D.mixed && C.D.type != BMG_AH && C.D.me <= 1 && R.cols < KMAX)
+    if (next >= 0 && !C.D.mixed && C.D.type != BMG_AH && C.D.me <= 1 && R.cols < KMAX)
       nslot = alloc_slot(C);
     if (nslot >= 0) {
       for (int c = threadIdx.x; c <= R.cols; c += BMG_BLOCK) S.iacc2[c] = 0;
