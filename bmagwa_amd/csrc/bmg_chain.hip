@@ -965,7 +965,7 @@ __device__ __forceinline__ uint64_t tree_pick(Ctx<KMAX>& C, int tr, double u) {
       unsigned long long nz = __ballot(c > 0.0);
       int pick = m1 ? (__ffsll((long long)m1) - 1) : (nz ? 63 - __clzll((long long)nz) : -1);
       if (pick < 0) { res = -1; break; }
-      double sp = __shfl(s, pick > 0 ? pick - 1 : 0, 64);
+      double sp = readlane_d(s, pick > 0 ? pick - 1 : 0);   // uniform lane: a scalar read
       if (pick > 0) carry = carry + sp;
       g = 64 * g + (uint64_t)pick;
       res = (long long)g;
@@ -993,7 +993,7 @@ __device__ __forceinline__ void tree_set(Ctx<KMAX>& C, int tr, uint64_t i, int z
     double c0 = g0[C.D.lev_off[0] + 64 * g1 + lane];
     double c1 = C.D.top >= 1 ? g0[C.D.lev_off[1] + 64 * (g1 >> 6) + lane] : 0.0;
     if (C.R.tj_on) {
-      const double o0 = __shfl(c0, (int)(i & 63), 64), o1 = __shfl(c1, (int)(g1 & 63), 64);
+      const double o0 = readlane_d(c0, (int)(i & 63)), o1 = readlane_d(c1, (int)(g1 & 63));
       if (lane == 0) {
         C.S.tj_old[C.R.tj_n][0] = o0; C.S.tj_old[C.R.tj_n][1] = o1;
         C.S.tj_ind[C.R.tj_n] = i | ((uint64_t)tr << 63);
@@ -1054,7 +1054,7 @@ __device__ __forceinline__ uint64_t tree_pick_zero(Ctx<KMAX>& C, int tr, double 
       unsigned long long nz = __ballot(c > 0.0);
       int pick = m1 ? (__ffsll((long long)m1) - 1) : (nz ? 63 - __clzll((long long)nz) : -1);
       if (pick < 0) { res = -1; break; }
-      double sp = __shfl(s, pick > 0 ? pick - 1 : 0, 64);
+      double sp = readlane_d(s, pick > 0 ? pick - 1 : 0);   // uniform lane: a scalar read
       if (pick > 0) carry = carry + sp;
       g = 64 * g + (uint64_t)pick;
       res = (long long)g;
@@ -1062,9 +1062,9 @@ __device__ __forceinline__ uint64_t tree_pick_zero(Ctx<KMAX>& C, int tr, double 
     if (res >= 0) {
       const uint64_t i = (uint64_t)res;
       const uint64_t g1 = i >> 6;
-      const double wv = __shfl(cl[0], (int)(i & 63), 64);
+      const double wv = readlane_d(cl[0], (int)(i & 63));
       if (C.R.tj_on) {
-        const double o1 = C.D.top >= 1 ? __shfl(cl[1], (int)(g1 & 63), 64) : 0.0;
+        const double o1 = C.D.top >= 1 ? readlane_d(cl[1], (int)(g1 & 63)) : 0.0;
         if (lane == 0) {
           C.S.tj_old[C.R.tj_n][0] = wv; C.S.tj_old[C.R.tj_n][1] = o1;
           C.S.tj_ind[C.R.tj_n] = i | ((uint64_t)tr << 63);
